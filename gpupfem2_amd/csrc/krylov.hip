@@ -1,0 +1,851 @@
+// Krylov solvers: PCG (SolverCG::solve, src/linear_algebra/linear_solver.cu:301-376) and GMRES(m)
+// (SolverGMRES::solve, :421-521), FP64 and FP32, plus FP64 iterative refinement around an FP32 inner
+// solve (the mixed-precision mode).
+//
+// Every scalar stays on the device. Dot products are deterministic grid reductions accumulated in FP64
+// (per-block partials, the last block adds them in block order). A device "done" flag replaces the
+// reference's per-iteration host read-back (linear_solver.cu:360, :499): the host polls it every few
+// iterations and kernels of iterations launched past convergence return at once.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "device.hpp"
+#include "internal.hpp"
+#include "solver.hpp"
+#include "spmv.hpp"
+
+namespace pfem {
+
+inline unsigned vec_grid(int n) { return std::min<unsigned>(blocks_for(n), 1024u); }
+
+__device__ __forceinline__ void cg_start_scalars(CgState* st, double rz, double bmb)
+{
+    st->gamma = rz;
+    st->thr = st->tol_mode == PFEM_TOL_RELATIVE ? st->tol * st->tol * bmb : st->tol * st->tol;
+    st->it = 0;
+    st->nan = 0;
+    st->converged = rz < st->thr ? 1 : 0;
+    st->done = st->converged;
+    st->res = sqrt(rz);
+}
+
+__device__ __forceinline__ void cg_finish_iteration(CgState* st, double gnew)
+{
+    st->it += 1;
+    st->res = sqrt(fabs(gnew));
+    if (gnew != gnew) {
+        st->nan = 1;
+        st->done = 1;
+    } else if (gnew < st->thr) {
+        st->converged = 1;
+        st->done = 1;
+    } else if (st->it >= st->maxit) {
+        st->done = 1;
+    }
+    st->beta = gnew / st->gamma;
+    st->gamma = gnew;
+}
+
+// ------------------------------------------------------------------ PCG kernels
+// r = b - A x, z = invD r (Jacobi) or r, p = z; totals (r,z), (b, M^-1 b)
+template <typename T, bool JAC>
+__global__ __launch_bounds__(kBlock) void k_cg_init(const int* __restrict__ tiles, const int* __restrict__ rowptr,
+                                                    const int* __restrict__ col, const T* __restrict__ val,
+                                                    const T* __restrict__ x, const T* __restrict__ b,
+                                                    const T* __restrict__ invd, T* __restrict__ r, T* __restrict__ z,
+                                                    T* __restrict__ p, CgState* st, double* partials)
+{
+    __shared__ T prod[kTileNnz];
+    int row, r1;
+    const T ax = tile_row_product<T, T>(tiles, rowptr, col, val, x, prod, row, r1);
+    double v[2] = {0.0, 0.0};
+    if (row < r1) {
+        const T bb = b[row];
+        const T rr = bb - ax;
+        const T zz = JAC ? rr * invd[row] : rr;
+        r[row] = rr;
+        z[row] = zz;
+        p[row] = zz;
+        v[0] = (double)(rr * zz);
+        v[1] = JAC ? (double)(bb * (bb * invd[row])) : (double)(bb * bb);
+    }
+    double tot[2];
+    if (grid_sum<2>(v, partials, &st->counter, tot) && threadIdx.x == 0) cg_start_scalars(st, tot[0], tot[1]);
+}
+
+// x = 0 start (inner solves of the refinement): r = b, z = M^-1 b (Jacobi), p = z
+template <typename T, bool JAC>
+__global__ void k_cg_init_zero(int n, const T* __restrict__ b, const T* __restrict__ invd, T* __restrict__ x,
+                               T* __restrict__ r, T* __restrict__ z, T* __restrict__ p, CgState* st, double* partials)
+{
+    double v[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const T bb = b[i];
+        const T zz = JAC ? bb * invd[i] : bb;
+        x[i] = 0;
+        r[i] = bb;
+        z[i] = zz;
+        p[i] = zz;
+        v[0] += (double)(bb * zz);
+    }
+    v[1] = v[0];
+    double tot[2];
+    if (grid_sum<2>(v, partials, &st->counter, tot) && threadIdx.x == 0) cg_start_scalars(st, tot[0], tot[1]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_residual(const int* __restrict__ tiles, const int* __restrict__ rowptr,
+                                                     const int* __restrict__ col, const T* __restrict__ val,
+                                                     const T* __restrict__ x, const T* __restrict__ b, T* __restrict__ r)
+{
+    __shared__ T prod[kTileNnz];
+    int row, r1;
+    const T ax = tile_row_product<T, T>(tiles, rowptr, col, val, x, prod, row, r1);
+    if (row < r1) r[row] = b[row] - ax;
+}
+
+// p = z; totals (r,z), (b,t), t = M^-1 b (t may be null)
+template <typename T>
+__global__ void k_cg_start(int n, const T* __restrict__ r, const T* __restrict__ z, T* __restrict__ p,
+                           const T* __restrict__ b, const T* __restrict__ t, CgState* st, double* partials)
+{
+    double v[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        p[i] = z[i];
+        v[0] += (double)(r[i] * z[i]);
+        if (t) v[1] += (double)(b[i] * t[i]);
+    }
+    double tot[2];
+    if (grid_sum<2>(v, partials, &st->counter, tot) && threadIdx.x == 0) cg_start_scalars(st, tot[0], tot[1]);
+}
+
+// q = A p, alpha = gamma / (p, q)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv(const int* __restrict__ tiles, const int* __restrict__ rowptr,
+                                                    const int* __restrict__ col, const T* __restrict__ val,
+                                                    const T* __restrict__ p, T* __restrict__ q, CgState* st,
+                                                    double* partials)
+{
+    __shared__ T prod[kTileNnz];
+    if (st->done) return;
+    int row, r1;
+    const T ap = tile_row_product<T, T>(tiles, rowptr, col, val, p, prod, row, r1);
+    double v[1] = {0.0};
+    if (row < r1) {
+        q[row] = ap;
+        v[0] = (double)(p[row] * ap);
+    }
+    double tot[1];
+    if (grid_sum<1>(v, partials, &st->counter, tot) && threadIdx.x == 0) {
+        st->pAp = tot[0];
+        st->alpha = st->gamma / tot[0];
+    }
+}
+
+// x += alpha p, r -= alpha q (updateXR, linear_solver.cu:68-77); z = invD r; gamma_new = (r, z)
+template <typename T, bool JAC>
+__global__ void k_cg_update(int n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
+                            const T* __restrict__ q, T* __restrict__ z, const T* __restrict__ invd, CgState* st,
+                            double* partials)
+{
+    if (st->done) return;
+    const T alpha = (T)st->alpha;
+    double v[1] = {0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        x[i] += alpha * p[i];
+        const T rr = r[i] - alpha * q[i];
+        r[i] = rr;
+        const T zz = JAC ? rr * invd[i] : rr;
+        if (JAC) z[i] = zz;
+        v[0] += (double)(rr * zz);
+    }
+    double tot[1];
+    if (grid_sum<1>(v, partials, &st->counter, tot) && threadIdx.x == 0) cg_finish_iteration(st, tot[0]);
+}
+
+template <typename T>
+__global__ void k_cg_update_xr(int n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
+                               const T* __restrict__ q, const CgState* st)
+{
+    if (st->done) return;
+    const T alpha = (T)st->alpha;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        x[i] += alpha * p[i];
+        r[i] = r[i] - alpha * q[i];
+    }
+}
+
+template <typename T>
+__global__ void k_cg_rz(int n, const T* __restrict__ r, const T* __restrict__ z, CgState* st, double* partials)
+{
+    if (st->done) return;
+    double v[1] = {0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        v[0] += (double)(r[i] * z[i]);
+    double tot[1];
+    if (grid_sum<1>(v, partials, &st->counter, tot) && threadIdx.x == 0) cg_finish_iteration(st, tot[0]);
+}
+
+// p = z + beta p (updateP, linear_solver.cu:79-87)
+template <typename T>
+__global__ void k_cg_p(int n, T* __restrict__ p, const T* __restrict__ z, const CgState* st)
+{
+    if (st->done) return;
+    const T beta = (T)st->beta;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = z[i] + beta * p[i];
+}
+
+// ------------------------------------------------------------------ GMRES kernels
+template <typename T, bool JAC>
+__global__ __launch_bounds__(kBlock) void k_gm_residual(const int* __restrict__ tiles, const int* __restrict__ rowptr,
+                                                        const int* __restrict__ col, const T* __restrict__ val,
+                                                        const T* __restrict__ x, const T* __restrict__ b,
+                                                        const T* __restrict__ invd, T* __restrict__ w, GmState* st,
+                                                        double* partials)
+{
+    __shared__ T prod[kTileNnz];
+    int row, r1;
+    const T ax = tile_row_product<T, T>(tiles, rowptr, col, val, x, prod, row, r1);
+    double v[2] = {0.0, 0.0};
+    if (row < r1) {
+        const T bb = b[row];
+        const T rr = bb - ax;
+        const T ww = JAC ? rr * invd[row] : rr;
+        const T mb = JAC ? bb * invd[row] : bb;
+        w[row] = ww;
+        v[0] = (double)(ww * ww);
+        v[1] = (double)(mb * mb);
+    }
+    double tot[2];
+    if (grid_sum<2>(v, partials, &st->counter, tot) && threadIdx.x == 0) gm_cycle_start(st, tot[0], tot[1]);
+}
+
+// x = 0 start: w = M^-1 b (Jacobi or none)
+template <typename T, bool JAC>
+__global__ void k_gm_init_zero(int n, const T* __restrict__ b, const T* __restrict__ invd, T* __restrict__ x,
+                               T* __restrict__ w, GmState* st, double* partials)
+{
+    double v[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const T bb = b[i];
+        const T ww = JAC ? bb * invd[i] : bb;
+        x[i] = 0;
+        w[i] = ww;
+        v[0] += (double)(ww * ww);
+    }
+    v[1] = v[0];
+    double tot[2];
+    if (grid_sum<2>(v, partials, &st->counter, tot) && threadIdx.x == 0) gm_cycle_start(st, tot[0], tot[1]);
+}
+
+template <typename T>
+__global__ void k_gm_norms(int n, const T* __restrict__ w, const T* __restrict__ t, GmState* st, double* partials)
+{
+    double v[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        v[0] += (double)(w[i] * w[i]);
+        if (t) v[1] += (double)(t[i] * t[i]);
+    }
+    double tot[2];
+    if (grid_sum<2>(v, partials, &st->counter, tot) && threadIdx.x == 0) gm_cycle_start(st, tot[0], tot[1]);
+}
+
+// v /= ||v|| (scaleVector with divide = true, linear_solver.cu:16-26, :456, :493)
+template <typename T>
+__global__ void k_gm_scale(int n, T* __restrict__ v, const GmState* st)
+{
+    if (st->cycle_done) return;
+    const T nrm = (T)st->norm;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v[i] = v[i] / nrm;
+}
+
+// w = M^-1 A v_j (Jacobi fused: applyPreconditioner(v_kp, v_kp), linear_solver.cu:473-474)
+template <typename T, bool JAC>
+__global__ __launch_bounds__(kBlock) void k_gm_arnoldi_spmv(const int* __restrict__ tiles, const int* __restrict__ rowptr,
+                                                            const int* __restrict__ col, const T* __restrict__ val,
+                                                            const T* __restrict__ v, const T* __restrict__ invd,
+                                                            T* __restrict__ w, const GmState* st)
+{
+    __shared__ T prod[kTileNnz];
+    if (st->cycle_done) return;
+    int row, r1;
+    const T av = tile_row_product<T, T>(tiles, rowptr, col, val, v, prod, row, r1);
+    if (row < r1) w[row] = JAC ? av * invd[row] : av;
+}
+
+// h[i0 + t] = (V_{i0+t}, w), t < cnt <= 8; pass 0 stores the coefficients, pass 1 adds to them
+template <typename T>
+__global__ void k_gm_multidot(int n, const T* __restrict__ V, long ld, int i0, int cnt, const T* __restrict__ w,
+                              GmState* st, double* partials, int pass)
+{
+    __shared__ double scratch[8 * kWaves];
+    if (st->cycle_done) return;
+    double v[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) v[t] = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const T wi = w[i];
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+            if (t < cnt) v[t] += (double)(V[(size_t)(i0 + t) * ld + i] * wi);
+    }
+    double tot[8];
+    if (grid_sum_dyn(v, cnt, partials, &st->counter, tot, scratch) && threadIdx.x == 0) {
+        double* h = st->H + (size_t)st->k * st->m;
+        for (int t = 0; t < cnt; ++t) {
+            st->hc[i0 + t] = tot[t];
+            h[i0 + t] = pass == 0 ? tot[t] : h[i0 + t] + tot[t];
+        }
+    }
+}
+
+// w -= sum_i hc_i V_i (i <= j); on the last pass ||w|| finishes Arnoldi step j (Givens update)
+template <typename T>
+__global__ void k_gm_orth(int n, const T* __restrict__ V, long ld, T* __restrict__ w, GmState* st, double* partials,
+                          int last_pass)
+{
+    __shared__ double hc[kMaxRestart + 1];
+    if (st->cycle_done) return;
+    const int j = st->k;
+    for (int i = threadIdx.x; i <= j; i += blockDim.x) hc[i] = st->hc[i];
+    __syncthreads();
+    double v[1] = {0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        T wi = w[i];
+        for (int t = 0; t <= j; ++t) wi -= (T)hc[t] * V[(size_t)t * ld + i];
+        w[i] = wi;
+        v[0] += (double)(wi * wi);
+    }
+    if (!last_pass) return;
+    double tot[1];
+    if (grid_sum<1>(v, partials, &st->counter, tot) && threadIdx.x == 0) gm_arnoldi_finish(st, sqrt(tot[0]));
+}
+
+// y = H^-1 beta (cublasDtrsv, upper, non-unit; linear_solver.cu:508-511)
+__global__ void k_gm_solve_y(GmState* st)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    st->apply_update = (st->converged || st->restarted) && st->k > 0 ? 1 : 0;
+    if (!st->apply_update) return;
+    const int k = st->k, m = st->m;
+    for (int i = 0; i < k; ++i) st->y[i] = st->beta[i];
+    for (int i = k - 1; i >= 0; --i) {
+        double s = st->y[i];
+        for (int j = i + 1; j < k; ++j) s -= st->H[(size_t)m * j + i] * st->y[j];
+        st->y[i] = s / st->H[(size_t)m * i + i];
+    }
+}
+
+// x += V y (cublasDgemv with alpha = beta = 1; linear_solver.cu:513-514)
+template <typename T>
+__global__ void k_gm_update_x(int n, const T* __restrict__ V, long ld, T* __restrict__ x, const GmState* st)
+{
+    __shared__ double y[kMaxRestart];
+    if (!st->apply_update) return;
+    const int k = st->k;
+    for (int i = threadIdx.x; i < k; i += blockDim.x) y[i] = st->y[i];
+    __syncthreads();
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        T s = 0;
+        for (int j = 0; j < k; ++j) s += V[(size_t)j * ld + i] * (T)y[j];
+        x[i] = s + x[i];
+    }
+}
+
+// ------------------------------------------------------------------ refinement kernels
+// FP64 refinement state
+struct IrState {
+    double thr, rz, scale;
+    int first, converged, nan;
+    unsigned counter;
+};
+
+// r = b - A x (FP64), z = invD r (Jacobi) / r; totals (r,z) and (b, M^-1 b); gmres: ||z||^2, ||M^-1 b||^2
+template <bool JAC, bool GM>
+__global__ __launch_bounds__(kBlock) void k_ir_residual(const int* __restrict__ tiles, const int* __restrict__ rowptr,
+                                                        const int* __restrict__ col, const double* __restrict__ val,
+                                                        const double* __restrict__ x, const double* __restrict__ b,
+                                                        const double* __restrict__ invd, double* __restrict__ r,
+                                                        double* __restrict__ z, IrState* st, double tol, int tol_mode,
+                                                        double* partials)
+{
+    __shared__ double prod[kTileNnz];
+    int row, r1;
+    const double ax = tile_row_product<double, double>(tiles, rowptr, col, val, x, prod, row, r1);
+    double v[2] = {0.0, 0.0};
+    if (row < r1) {
+        const double bb = b[row];
+        const double rr = bb - ax;
+        const double zz = JAC ? rr * invd[row] : rr;
+        r[row] = rr;
+        z[row] = zz;
+        const double mb = JAC ? bb * invd[row] : bb;
+        v[0] = GM ? zz * zz : rr * zz;
+        v[1] = GM ? mb * mb : bb * mb;
+    }
+    double tot[2];
+    if (grid_sum<2>(v, partials, &st->counter, tot) && threadIdx.x == 0) {
+        if (st->first) {
+            st->thr = GM ? (tol_mode == PFEM_TOL_RELATIVE ? tol * sqrt(tot[1]) : tol)
+                         : (tol_mode == PFEM_TOL_RELATIVE ? tol * tol * tot[1] : tol * tol);
+            st->first = 0;
+        }
+        st->rz = tot[0];
+        const double meas = GM ? sqrt(tot[0]) : tot[0];
+        st->nan = tot[0] != tot[0];
+        st->converged = meas < st->thr ? 1 : 0;
+        st->scale = sqrt(tot[0]) > 0.0 ? sqrt(tot[0]) : 1.0;
+    }
+}
+
+// general preconditioner: z = M^-1 r already applied; totals as above (t = M^-1 b, may be null)
+template <bool GM>
+__global__ void k_ir_norms(int n, const double* __restrict__ r, const double* __restrict__ z,
+                           const double* __restrict__ b, const double* __restrict__ t, IrState* st, double tol,
+                           int tol_mode, double* partials)
+{
+    double v[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        v[0] += GM ? z[i] * z[i] : r[i] * z[i];
+        if (t) v[1] += GM ? t[i] * t[i] : b[i] * t[i];
+    }
+    double tot[2];
+    if (grid_sum<2>(v, partials, &st->counter, tot) && threadIdx.x == 0) {
+        if (st->first) {
+            st->thr = GM ? (tol_mode == PFEM_TOL_RELATIVE ? tol * sqrt(tot[1]) : tol)
+                         : (tol_mode == PFEM_TOL_RELATIVE ? tol * tol * tot[1] : tol * tol);
+            st->first = 0;
+        }
+        st->rz = tot[0];
+        const double meas = GM ? sqrt(tot[0]) : tot[0];
+        st->nan = tot[0] != tot[0];
+        st->converged = meas < st->thr ? 1 : 0;
+        st->scale = sqrt(tot[0]) > 0.0 ? sqrt(tot[0]) : 1.0;
+    }
+}
+
+__global__ void k_ir_down(int n, float* __restrict__ d, const double* __restrict__ s, const IrState* st)
+{
+    const double inv = 1.0 / st->scale;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) d[i] = (float)(s[i] * inv);
+}
+
+__global__ void k_ir_up(int n, double* __restrict__ x, const float* __restrict__ d, const IrState* st)
+{
+    const double sc = st->scale;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] += sc * (double)d[i];
+}
+
+__global__ void k_to32(long n, float* __restrict__ d, const double* __restrict__ s)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = (float)s[i];
+}
+
+// ------------------------------------------------------------------ host drivers
+template <typename T>
+struct KArgs {
+    const Pattern* P;
+    const T* val;
+    const T* invd;       // Jacobi diagonal (null otherwise)
+    int kind;            // preconditioner kind
+    pfem_precond_t* pc;
+    T* x;
+    const T* b;
+    bool x_zero;
+    double tol;
+    int tol_mode, maxit;
+    T *r, *z, *p, *q, *t, *V;
+};
+
+template <typename T>
+static void papply(Context* c, pfem_precond_t* pc, T* z, const T* r);
+template <>
+void papply<double>(Context* c, pfem_precond_t* pc, double* z, const double* r) { precond_apply(c, pc, z, r); }
+template <>
+void papply<float>(Context* c, pfem_precond_t* pc, float* z, const float* r) { precond_apply32(c, pc, z, r); }
+
+template <typename T>
+static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st, CgState* hst)
+{
+    const Pattern& P = *A.P;
+    const int n = P.n;
+    const unsigned g = vec_grid(n);
+    const bool jac = A.kind == PFEM_PRECOND_JACOBI, none = A.kind == PFEM_PRECOND_NONE;
+    double* part = s->partials.p;
+    CgState init{};
+    init.tol = A.tol;
+    init.tol_mode = A.tol_mode;
+    init.maxit = A.maxit;
+    PFEM_HIP(hipMemcpyAsync(st, &init, offsetof(CgState, counter), hipMemcpyHostToDevice, c->stream));
+    if (jac || none) {
+        if (A.x_zero) {
+            if (jac) hipLaunchKernelGGL((k_cg_init_zero<T, true>), dim3(g), dim3(kBlock), 0, c->stream, n, A.b, A.invd, A.x, A.r, A.z, A.p, st, part);
+            else hipLaunchKernelGGL((k_cg_init_zero<T, false>), dim3(g), dim3(kBlock), 0, c->stream, n, A.b, A.invd, A.x, A.r, A.z, A.p, st, part);
+        } else {
+            if (jac) hipLaunchKernelGGL((k_cg_init<T, true>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.x, A.b, A.invd, A.r, A.z, A.p, st, part);
+            else hipLaunchKernelGGL((k_cg_init<T, false>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.x, A.b, A.invd, A.r, A.z, A.p, st, part);
+        }
+    } else {
+        if (A.x_zero) {
+            PFEM_HIP(hipMemsetAsync(A.x, 0, sizeof(T) * n, c->stream));
+            PFEM_HIP(hipMemcpyAsync(A.r, A.b, sizeof(T) * n, hipMemcpyDeviceToDevice, c->stream));
+        } else {
+            hipLaunchKernelGGL((k_residual<T>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.x, A.b, A.r);
+        }
+        papply<T>(c, A.pc, A.z, A.r);
+        const T* t = nullptr;
+        if (A.tol_mode == PFEM_TOL_RELATIVE) {
+            if (A.x_zero) t = A.z;
+            else { papply<T>(c, A.pc, A.t, A.b); t = A.t; }
+        }
+        hipLaunchKernelGGL((k_cg_start<T>), dim3(g), dim3(kBlock), 0, c->stream, n, A.r, A.z, A.p, A.b, t, st, part);
+    }
+    PFEM_HIP(hipGetLastError());
+    const int batch = s->poll;
+    long launched = 0;
+    while (true) {
+        for (int k = 0; k < batch; ++k) {
+            hipLaunchKernelGGL((k_cg_spmv<T>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.p, A.q, st, part);
+            if (jac) {
+                hipLaunchKernelGGL((k_cg_update<T, true>), dim3(g), dim3(kBlock), 0, c->stream, n, A.x, A.r, A.p, A.q, A.z, A.invd, st, part);
+            } else if (none) {
+                hipLaunchKernelGGL((k_cg_update<T, false>), dim3(g), dim3(kBlock), 0, c->stream, n, A.x, A.r, A.p, A.q, A.r, A.invd, st, part);
+            } else {
+                hipLaunchKernelGGL((k_cg_update_xr<T>), dim3(g), dim3(kBlock), 0, c->stream, n, A.x, A.r, A.p, A.q, st);
+                papply<T>(c, A.pc, A.z, A.r);
+                hipLaunchKernelGGL((k_cg_rz<T>), dim3(g), dim3(kBlock), 0, c->stream, n, A.r, A.z, st, part);
+            }
+            hipLaunchKernelGGL((k_cg_p<T>), dim3(g), dim3(kBlock), 0, c->stream, n, A.p, none ? A.r : A.z, st);
+        }
+        launched += batch;
+        PFEM_HIP(hipGetLastError());
+        PFEM_HIP(hipMemcpyAsync(hst, st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+        PFEM_HIP(hipStreamSynchronize(c->stream));
+        if (hst->done || launched > (long)A.maxit + batch) break;
+    }
+}
+
+template <typename T>
+static void gmres_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, GmState* st, GmState* hst)
+{
+    const Pattern& P = *A.P;
+    const int n = P.n;
+    const unsigned g = vec_grid(n);
+    const bool jac = A.kind == PFEM_PRECOND_JACOBI, none = A.kind == PFEM_PRECOND_NONE;
+    const int m = s->m;
+    const long ld = n;
+    T* V = A.V;
+    double* part = s->partials.p;
+    GmState init{};
+    init.tol = A.tol;
+    init.tol_mode = A.tol_mode;
+    init.maxit = A.maxit;
+    init.m = m;
+    init.restarted = s->restarted;
+    init.thr = -1.0;
+    init.H = s->H.p;
+    init.beta = s->beta.p;
+    init.cs = s->cs.p;
+    init.sn = s->sn.p;
+    init.y = s->y.p;
+    init.hc = s->hc.p;
+    PFEM_HIP(hipMemcpyAsync(st, &init, offsetof(GmState, counter), hipMemcpyHostToDevice, c->stream));
+    bool first = true;
+    while (true) {
+        if (jac || none) {
+            if (first && A.x_zero) {
+                if (jac) hipLaunchKernelGGL((k_gm_init_zero<T, true>), dim3(g), dim3(kBlock), 0, c->stream, n, A.b, A.invd, A.x, V, st, part);
+                else hipLaunchKernelGGL((k_gm_init_zero<T, false>), dim3(g), dim3(kBlock), 0, c->stream, n, A.b, A.invd, A.x, V, st, part);
+            } else {
+                if (jac) hipLaunchKernelGGL((k_gm_residual<T, true>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.x, A.b, A.invd, V, st, part);
+                else hipLaunchKernelGGL((k_gm_residual<T, false>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.x, A.b, A.invd, V, st, part);
+            }
+        } else {
+            if (first && A.x_zero) {
+                PFEM_HIP(hipMemsetAsync(A.x, 0, sizeof(T) * n, c->stream));
+                PFEM_HIP(hipMemcpyAsync(V, A.b, sizeof(T) * n, hipMemcpyDeviceToDevice, c->stream));
+            } else {
+                hipLaunchKernelGGL((k_residual<T>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.x, A.b, V);
+            }
+            papply<T>(c, A.pc, V, V);
+            const T* t = nullptr;
+            if (first && A.tol_mode == PFEM_TOL_RELATIVE) {
+                if (A.x_zero) t = V;
+                else { papply<T>(c, A.pc, A.t, A.b); t = A.t; }
+            }
+            hipLaunchKernelGGL((k_gm_norms<T>), dim3(g), dim3(kBlock), 0, c->stream, n, V, t, st, part);
+        }
+        first = false;
+        hipLaunchKernelGGL((k_gm_scale<T>), dim3(g), dim3(kBlock), 0, c->stream, n, V, st);
+        for (int j0 = 0; j0 < m; j0 += s->poll) {
+            const int j1 = std::min(m, j0 + s->poll);
+            for (int j = j0; j < j1; ++j) {
+                T* vj = V + (size_t)j * ld;
+                T* w = V + (size_t)(j + 1) * ld;
+                if (jac) hipLaunchKernelGGL((k_gm_arnoldi_spmv<T, true>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, vj, A.invd, w, st);
+                else hipLaunchKernelGGL((k_gm_arnoldi_spmv<T, false>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, vj, A.invd, w, st);
+                if (!jac && !none) papply<T>(c, A.pc, w, w);
+                const int passes = s->reorth ? 2 : 1;
+                for (int pass = 0; pass < passes; ++pass) {
+                    for (int i0 = 0; i0 <= j; i0 += 8)
+                        hipLaunchKernelGGL((k_gm_multidot<T>), dim3(g), dim3(kBlock), 0, c->stream, n, V, ld, i0, std::min(8, j + 1 - i0), w, st, part, pass);
+                    hipLaunchKernelGGL((k_gm_orth<T>), dim3(g), dim3(kBlock), 0, c->stream, n, V, ld, w, st, part, pass == passes - 1 ? 1 : 0);
+                }
+                hipLaunchKernelGGL((k_gm_scale<T>), dim3(g), dim3(kBlock), 0, c->stream, n, w, st);
+            }
+            PFEM_HIP(hipGetLastError());
+            PFEM_HIP(hipMemcpyAsync(hst, st, sizeof(GmState), hipMemcpyDeviceToHost, c->stream));
+            PFEM_HIP(hipStreamSynchronize(c->stream));
+            if (hst->cycle_done) break;
+        }
+        hipLaunchKernelGGL(k_gm_solve_y, dim3(1), dim3(64), 0, c->stream, st);
+        hipLaunchKernelGGL((k_gm_update_x<T>), dim3(g), dim3(kBlock), 0, c->stream, n, V, ld, A.x, st);
+        PFEM_HIP(hipGetLastError());
+        PFEM_HIP(hipMemcpyAsync(hst, st, sizeof(GmState), hipMemcpyDeviceToHost, c->stream));
+        PFEM_HIP(hipStreamSynchronize(c->stream));
+        if (hst->converged || hst->nan || hst->it >= A.maxit || !s->restarted) break;
+        PFEM_HIP(hipMemsetAsync(&st->cycle_done, 0, sizeof(int), c->stream));
+    }
+}
+
+static KArgs<double> args64(pfem_solver_t* s, pfem_csr_t* a, double* x, const double* b)
+{
+    KArgs<double> A{};
+    A.P = a->pat.get();
+    A.val = a->val.p;
+    A.kind = s->pc ? s->pc->kind : PFEM_PRECOND_NONE;
+    A.invd = A.kind == PFEM_PRECOND_JACOBI ? s->pc->invd.p : nullptr;
+    A.pc = s->pc;
+    A.x = x;
+    A.b = b;
+    A.x_zero = false;
+    A.tol = s->tol;
+    A.tol_mode = s->tol_mode;
+    A.maxit = s->maxit;
+    A.r = s->r.p; A.z = s->z.p; A.p = s->p.p; A.q = s->q.p; A.t = s->t.p; A.V = s->V.p;
+    return A;
+}
+
+void cg_solve_fp64(Context* c, pfem_solver_t* s, pfem_csr_t* a, double* x, const double* b, pfem_solve_stats* out)
+{
+    if (s->pc) precond_initialize(c, s->pc, a);
+    cg_run<double>(c, s, args64(s, a, x, b), s->cg.p, s->h_cg);
+    if (out) {
+        out->converged = s->h_cg->converged;
+        out->iterations = s->h_cg->it;
+        out->final_residual = s->h_cg->res;
+        out->nan_detected = s->h_cg->nan;
+        out->refinements = 0;
+    }
+}
+
+void gmres_solve_fp64(Context* c, pfem_solver_t* s, pfem_csr_t* a, double* x, const double* b, pfem_solve_stats* out)
+{
+    if (s->pc) precond_initialize(c, s->pc, a);
+    gmres_run<double>(c, s, args64(s, a, x, b), s->gm.p, s->h_gm);
+    if (out) {
+        out->converged = s->h_gm->converged;
+        out->iterations = s->h_gm->it;
+        out->final_residual = s->h_gm->res;
+        out->nan_detected = s->h_gm->nan;
+        out->refinements = 0;
+    }
+}
+
+// FP64 iterative refinement around FP32 inner solves: r = b - A x and the stopping test in FP64
+// (the same criterion as the FP64 solver), the correction A d = r / ||r|| solved in FP32 to the
+// relative tolerance inner_tol, x += ||r|| d in FP64.
+void solve_mixed(Context* c, pfem_solver_t* s, pfem_csr_t* a, double* x, const double* b, pfem_solve_stats* out)
+{
+    const Pattern& P = *a->pat;
+    const int n = P.n;
+    const unsigned g = vec_grid(n);
+    if (!a->val32.p) a->val32.alloc(P.nnz);
+    hipLaunchKernelGGL(k_to32, dim3(blocks_for(P.nnz)), dim3(kBlock), 0, c->stream, (long)P.nnz, a->val32.p, a->val.p);
+    if (s->pc) precond_initialize(c, s->pc, a);   // also refreshes the FP32 copies of the factor
+    const int kind = s->pc ? s->pc->kind : PFEM_PRECOND_NONE;
+    const bool jac = kind == PFEM_PRECOND_JACOBI, none = kind == PFEM_PRECOND_NONE;
+    const bool gm = s->kind == PFEM_SOLVER_GMRES;
+    IrState* ist = reinterpret_cast<IrState*>(s->xr.p + 2 * (size_t)n);   // state lives after the two n-vectors
+    IrState init{};
+    init.first = 1;
+    PFEM_HIP(hipMemcpyAsync(ist, &init, offsetof(IrState, counter), hipMemcpyHostToDevice, c->stream));
+    double* r = s->xr.p;
+    double* z = s->xr.p + n;
+    int total_inner = 0, refinements = 0, conv = 0, nan = 0;
+    double res = 0.0;
+    KArgs<float> A{};
+    A.P = &P;
+    A.val = a->val32.p;
+    A.kind = kind;
+    A.invd = jac ? s->pc->invd32.p : nullptr;
+    A.pc = s->pc;
+    A.x = s->d32.p;
+    A.b = s->r32.p;
+    A.x_zero = true;
+    A.tol = s->inner_tol;
+    A.tol_mode = PFEM_TOL_RELATIVE;
+    A.r = s->z32.p; A.z = s->p32.p; A.p = s->q32.p; A.q = s->t32.p; A.t = nullptr; A.V = s->V32.p;
+    IrState h{};
+    while (true) {
+        if (jac || none) {
+            if (gm) {
+                if (jac) hipLaunchKernelGGL((k_ir_residual<true, true>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, a->val.p, x, b, s->pc->invd.p, r, z, ist, s->tol, s->tol_mode, s->partials.p);
+                else hipLaunchKernelGGL((k_ir_residual<false, true>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, a->val.p, x, b, (const double*)nullptr, r, z, ist, s->tol, s->tol_mode, s->partials.p);
+            } else {
+                if (jac) hipLaunchKernelGGL((k_ir_residual<true, false>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, a->val.p, x, b, s->pc->invd.p, r, z, ist, s->tol, s->tol_mode, s->partials.p);
+                else hipLaunchKernelGGL((k_ir_residual<false, false>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, a->val.p, x, b, (const double*)nullptr, r, z, ist, s->tol, s->tol_mode, s->partials.p);
+            }
+        } else {
+            hipLaunchKernelGGL((k_residual<double>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, a->val.p, x, b, r);
+            precond_apply(c, s->pc, z, r);
+            const double* t = nullptr;
+            if (refinements == 0 && s->tol_mode == PFEM_TOL_RELATIVE) {
+                precond_apply(c, s->pc, s->t.p, b);
+                t = s->t.p;
+            }
+            if (gm) hipLaunchKernelGGL((k_ir_norms<true>), dim3(g), dim3(kBlock), 0, c->stream, n, r, z, b, t, ist, s->tol, s->tol_mode, s->partials.p);
+            else hipLaunchKernelGGL((k_ir_norms<false>), dim3(g), dim3(kBlock), 0, c->stream, n, r, z, b, t, ist, s->tol, s->tol_mode, s->partials.p);
+        }
+        PFEM_HIP(hipMemcpyAsync(s->h_cg, ist, sizeof(IrState), hipMemcpyDeviceToHost, c->stream));
+        PFEM_HIP(hipStreamSynchronize(c->stream));
+        std::memcpy(&h, s->h_cg, sizeof(IrState));
+        res = gm ? std::sqrt(h.rz) : std::sqrt(std::fabs(h.rz));
+        if (h.converged) { conv = 1; break; }
+        if (h.nan) { nan = 1; break; }
+        if (refinements >= s->max_refine || total_inner >= s->maxit) break;
+        hipLaunchKernelGGL(k_ir_down, dim3(g), dim3(kBlock), 0, c->stream, n, s->r32.p, r, ist);
+        A.maxit = std::max(1, s->maxit - total_inner);
+        if (gm) {
+            gmres_run<float>(c, s, A, s->gm.p, s->h_gm);
+            total_inner += s->h_gm->it;
+        } else {
+            cg_run<float>(c, s, A, s->cg.p, s->h_cg);
+            total_inner += s->h_cg->it;
+        }
+        hipLaunchKernelGGL(k_ir_up, dim3(g), dim3(kBlock), 0, c->stream, n, x, s->d32.p, ist);
+        ++refinements;
+    }
+    if (out) {
+        out->converged = conv;
+        out->iterations = total_inner;
+        out->final_residual = res;
+        out->nan_detected = nan;
+        out->refinements = refinements;
+    }
+}
+
+void precond_enable_fp32(Context* c, pfem_precond_t* pc);
+
+static void alloc_mixed(pfem_solver_t* s)
+{
+    const int n = s->n;
+    if (s->r32.p) return;
+    s->r32.alloc(n); s->z32.alloc(n); s->p32.alloc(n); s->q32.alloc(n); s->d32.alloc(n); s->t32.alloc(n);
+    if (s->kind == PFEM_SOLVER_GMRES) s->V32.alloc((size_t)n * (s->m + 1));
+    // two FP64 n-vectors followed by the refinement state
+    s->xr.alloc(2 * (size_t)n + (sizeof(IrState) + sizeof(double) - 1) / sizeof(double) + 1);
+    PFEM_HIP(hipMemset(s->xr.p, 0, s->xr.bytes()));
+}
+
+}  // namespace pfem
+
+using namespace pfem;
+
+extern "C" {
+
+int pfem_solver_create(pfem_csr a, int kind, double tol, int max_iterations, int restart, int tol_mode,
+                       pfem_precond precond, pfem_solver* out)
+{
+    PFEM_TRY(a ? a->ctx : nullptr, {
+        PFEM_REQUIRE(a && out, "null argument");
+        PFEM_REQUIRE(kind == PFEM_SOLVER_CG || kind == PFEM_SOLVER_GMRES, "unknown solver kind");
+        PFEM_REQUIRE(max_iterations > 0, "max_iterations must be positive");
+        auto s = std::make_unique<pfem_solver_t>();
+        s->ctx = a->ctx;
+        s->kind = kind;
+        s->tol = tol;
+        s->maxit = max_iterations;
+        s->tol_mode = tol_mode;
+        s->pc = precond;
+        const int n = a->pat->n;
+        s->n = n;
+        if (precond) PFEM_REQUIRE(precond->n == n, "preconditioner size differs from the matrix");
+        if (kind == PFEM_SOLVER_GMRES) {
+            // SolverGMRES::init clips maxIterations to the matrix size (linear_solver.cu:398-401)
+            if (s->maxit > n) s->maxit = n;
+            s->restart = restart <= 0 ? s->maxit : restart;
+            s->m = std::min(s->restart, s->maxit);
+            s->restarted = s->restart < s->maxit ? 1 : 0;
+            PFEM_REQUIRE(s->m <= kMaxRestart, "GMRES cycle longer than 1024 vectors");
+            s->V.alloc((size_t)n * (s->m + 1));
+            s->H.alloc((size_t)s->m * s->m);
+            s->beta.alloc(s->m + 1);
+            s->cs.alloc(s->m);
+            s->sn.alloc(s->m);
+            s->y.alloc(s->m);
+            s->hc.alloc(s->m + 1);
+            s->gm.alloc(1);
+            PFEM_HIP(hipMemset(s->gm.p, 0, sizeof(GmState)));
+            PFEM_HIP(hipHostMalloc((void**)&s->h_gm, sizeof(GmState), hipHostMallocDefault));
+            s->poll = 4;
+        } else {
+            s->r.alloc(n); s->z.alloc(n); s->p.alloc(n); s->q.alloc(n);
+            s->poll = 8;
+        }
+        s->t.alloc(n);
+        s->cg.alloc(1);
+        PFEM_HIP(hipMemset(s->cg.p, 0, sizeof(CgState)));
+        PFEM_HIP(hipHostMalloc((void**)&s->h_cg, std::max(sizeof(CgState), (size_t)128), hipHostMallocDefault));
+        const size_t np = 8 * (size_t)std::max<unsigned>(a->pat->ntiles, 1024u);
+        s->partials.alloc(np);
+        *out = s.release();
+    });
+}
+
+int pfem_solver_set_precision(pfem_solver s, int precision, double inner_tol, int max_refinements)
+{
+    PFEM_TRY(s ? s->ctx : nullptr, {
+        PFEM_REQUIRE(s, "null solver");
+        PFEM_REQUIRE(precision == PFEM_PRECISION_FP64 || precision == PFEM_PRECISION_MIXED, "unknown precision");
+        s->precision = precision;
+        if (inner_tol > 0) s->inner_tol = inner_tol;
+        if (max_refinements > 0) s->max_refine = max_refinements;
+        if (precision == PFEM_PRECISION_MIXED) {
+            if (s->kind == PFEM_SOLVER_GMRES && !s->gm.p) {
+                s->gm.alloc(1);
+                PFEM_HIP(hipMemset(s->gm.p, 0, sizeof(GmState)));
+            }
+            alloc_mixed(s);
+            if (s->pc) precond_enable_fp32(C(s->ctx), s->pc);
+        }
+    });
+}
+
+int pfem_solver_destroy(pfem_solver s)
+{
+    if (!s) return PFEM_OK;
+    (void)hipStreamSynchronize(s->ctx->stream);
+    if (s->h_cg) (void)hipHostFree(s->h_cg);
+    if (s->h_gm) (void)hipHostFree(s->h_gm);
+    delete s;
+    return PFEM_OK;
+}
+
+int pfem_solver_solve(pfem_solver s, pfem_csr a, double* d_x, const double* d_b, pfem_solve_stats* stats)
+{
+    PFEM_TRY(s ? s->ctx : nullptr, {
+        PFEM_REQUIRE(s && a && d_x && d_b, "null argument");
+        PFEM_REQUIRE(a->pat->n == s->n, "matrix size differs from the solver's");
+        Context* c = C(s->ctx);
+        pfem_solve_stats local{};
+        if (s->precision == PFEM_PRECISION_MIXED) solve_mixed(c, s, a, d_x, d_b, &local);
+        else if (s->kind == PFEM_SOLVER_CG) cg_solve_fp64(c, s, a, d_x, d_b, &local);
+        else gmres_solve_fp64(c, s, a, d_x, d_b, &local);
+        if (stats) *stats = local;
+    });
+}
+
+}  // extern "C"

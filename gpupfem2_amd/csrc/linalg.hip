@@ -1,0 +1,193 @@
+// Sparse linear algebra on gfx950: CSR SpMV (tile routine in spmv.hpp), vector helpers, Jacobi and
+// Dirichlet application.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "device.hpp"
+#include "internal.hpp"
+#include "solver.hpp"
+#include "spmv.hpp"
+
+namespace pfem {
+
+template <typename VT, typename XT>
+__global__ __launch_bounds__(kBlock) void k_spmv(const int* __restrict__ tiles, const int* __restrict__ rowptr,
+                                                 const int* __restrict__ col, const VT* __restrict__ val,
+                                                 const XT* __restrict__ x, XT* __restrict__ y)
+{
+    __shared__ XT prod[kTileNnz];
+    int r, r1;
+    const XT s = tile_row_product<VT, XT>(tiles, rowptr, col, val, x, prod, r, r1);
+    if (r < r1) y[r] = s;
+}
+
+void launch_spmv(Context* c, const Pattern& P, const double* val, const double* x, double* y)
+{
+    if (P.ntiles == 0) return;
+    hipLaunchKernelGGL((k_spmv<double, double>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p,
+                       P.rowptr.p, P.col.p, val, x, y);
+}
+
+void launch_spmv32(Context* c, const Pattern& P, const float* val, const float* x, float* y)
+{
+    if (P.ntiles == 0) return;
+    hipLaunchKernelGGL((k_spmv<float, float>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p,
+                       P.rowptr.p, P.col.p, val, x, y);
+}
+
+// ------------------------------------------------------------------ small vector kernels
+__global__ void k_fill(double* d, double v, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = v;
+}
+
+__global__ void k_copy(double* __restrict__ d, const double* __restrict__ s, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = s[i];
+}
+
+void launch_fill(Context* c, double* d, double v, long n)
+{
+    if (n > 0) hipLaunchKernelGGL(k_fill, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, d, v, n);
+}
+
+void launch_copy(Context* c, double* dst, const double* src, long n)
+{
+    if (n > 0) hipLaunchKernelGGL(k_copy, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, dst, src, n);
+}
+
+__global__ void k_to_f32(float* __restrict__ d, const double* __restrict__ s, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = (float)s[i];
+}
+
+// ------------------------------------------------------------------ Jacobi
+// extractDiagonal (src/linear_algebra/preconditioners.cu:3-14): invD = 1 / a_rr (diagonal positions are
+// precomputed instead of searched; same entry).
+__global__ void k_jacobi_setup(int n, const int* __restrict__ diag, const double* __restrict__ val,
+                               double* __restrict__ invd, float* __restrict__ invd32)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int k = diag[r];
+    if (k >= 0) {
+        const double v = 1.0 / val[k];
+        invd[r] = v;
+        if (invd32) invd32[r] = (float)v;
+    }
+}
+
+// applyJacobiPreconditioner (preconditioners.cu:16-23): dest = src * invD (dest may alias src)
+template <typename T>
+__global__ void k_jacobi_apply(int n, T* dest, const T* src, const T* __restrict__ invd)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) dest[r] = src[r] * invd[r];
+}
+
+void jacobi_setup(Context* c, const Pattern& P, const double* val, double* invd, float* invd32)
+{
+    hipLaunchKernelGGL(k_jacobi_setup, dim3(blocks_for(P.n)), dim3(kBlock), 0, c->stream, P.n, P.diag.p, val,
+                       invd, invd32);
+}
+
+void jacobi_apply(Context* c, int n, double* dest, const double* src, const double* invd)
+{
+    hipLaunchKernelGGL((k_jacobi_apply<double>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, dest, src,
+                       invd);
+}
+
+void jacobi_apply32(Context* c, int n, float* dest, const float* src, const float* invd)
+{
+    hipLaunchKernelGGL((k_jacobi_apply<float>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, dest, src,
+                       invd);
+}
+
+// ------------------------------------------------------------------ Dirichlet
+// kEnforceValueForBoundaryDoFs (src/Dirichlet_bcs.cu:3-17)
+__global__ void k_dir_enforce(int nbc, const int* __restrict__ node, const double* __restrict__ g,
+                              const double* __restrict__ g1, const int* __restrict__ rowptr,
+                              const int* __restrict__ col, double* __restrict__ val, double* __restrict__ rhs,
+                              double* __restrict__ rhs1)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nbc) return;
+    const int b = node[k];
+    for (int e = rowptr[b]; e < rowptr[b + 1]; ++e) val[e] = (col[e] == b) ? 1.0 : 0.0;
+    rhs[b] = g[k];
+    if (rhs1) rhs1[b] = g1[k];
+}
+
+// kAccountForBoundaryValues (src/Dirichlet_bcs.cu:19-51) as a row-owner gather: the lane owning row c
+// eliminates every Dirichlet column of c in ascending column order (no atomics). mode 0 keeps the
+// reference's skip test |first off-diagonal of c| < 2e-6, read before row c is modified.
+__global__ void k_dir_eliminate(int nrows, const int* __restrict__ rows, const int* __restrict__ rowptr,
+                                const int* __restrict__ col, double* __restrict__ val, const int* __restrict__ map,
+                                const double* __restrict__ g, const double* __restrict__ g1,
+                                double* __restrict__ rhs, double* __restrict__ rhs1, int mode)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrows) return;
+    const int c = rows[i];
+    const int s = rowptr[c], e = rowptr[c + 1];
+    if (mode == PFEM_DIRICHLET_PARITY) {
+        const int first = s + (col[s] == c ? 1 : 0);
+        if (fabs(val[first]) < 2e-6) return;
+    } else if (map[c] >= 0) {
+        return;
+    }
+    double r0 = rhs[c];
+    double r1 = rhs1 ? rhs1[c] : 0.0;
+    for (int k = s; k < e; ++k) {
+        const int b = col[k];
+        const int m = map[b];
+        if (b == c || m < 0) continue;
+        const double a = val[k];
+        r0 += a * (-g[m]);
+        if (rhs1) r1 += a * (-g1[m]);
+        val[k] = 0.0;
+    }
+    rhs[c] = r0;
+    if (rhs1) rhs1[c] = r1;
+}
+
+void dirichlet_rows(Context* c, pfem_dirichlet_t* bc, const Pattern& P)
+{
+    if (bc->rows_for == &P) return;
+    std::vector<char> isbc(P.n, 0);
+    for (int v : bc->h_node) isbc[v] = 1;
+    std::vector<int> rows;
+    for (int r = 0; r < P.n; ++r)
+        for (int k = P.h_rowptr[r]; k < P.h_rowptr[r + 1]; ++k)
+            if (P.h_col[k] != r && isbc[P.h_col[k]]) { rows.push_back(r); break; }
+    bc->nrows = (int)rows.size();
+    upload(c, bc->rows, rows);
+    bc->rows_for = &P;
+}
+
+void dirichlet_apply(Context* c, pfem_dirichlet_t* bc0, pfem_dirichlet_t* bc1, pfem_csr_t* a, double* rhs0,
+                     double* rhs1, int mode)
+{
+    const Pattern& P = *a->pat;
+    PFEM_REQUIRE(bc0->n == P.n, "BC node map size differs from the matrix rows");
+    if (bc1) {
+        PFEM_REQUIRE(bc1->h_node == bc0->h_node, "apply2 needs identical BC node lists");
+    }
+    if (bc0->nbc == 0) return;
+    dirichlet_rows(c, bc0, P);
+    hipLaunchKernelGGL(k_dir_enforce, dim3(blocks_for(bc0->nbc)), dim3(kBlock), 0, c->stream, bc0->nbc,
+                       bc0->node.p, bc0->value.p, bc1 ? bc1->value.p : nullptr, P.rowptr.p, P.col.p, a->val.p,
+                       rhs0, rhs1);
+    if (bc0->nrows > 0)
+        hipLaunchKernelGGL(k_dir_eliminate, dim3(blocks_for(bc0->nrows)), dim3(kBlock), 0, c->stream, bc0->nrows,
+                           bc0->rows.p, P.rowptr.p, P.col.p, a->val.p, bc0->map.p, bc0->value.p,
+                           bc1 ? bc1->value.p : nullptr, rhs0, rhs1, mode);
+    PFEM_HIP(hipGetLastError());
+}
+
+
+}  // namespace pfem

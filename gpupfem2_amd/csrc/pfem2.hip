@@ -1,0 +1,257 @@
+// One PFEM-2 time step: the loop body of cases/Cylinder2D/main.cu:793-880 (and of
+// cases/PoiseuilleFlow2D/main.cu:652-736), with outer iterations, scheme and solvers as parameters.
+// All systems share the mesh's CSR pattern; the two correction matrices of the reference are one matrix
+// with two right-hand sides (they are identical, :248-251).
+#include <cstring>
+
+#include "device.hpp"
+#include "internal.hpp"
+#include "solver.hpp"
+
+namespace pfem {
+void particles_advect(pfem_particles_t* ph, const double* u0, const double* u1, double dt, int substeps);
+void particles_project(pfem_particles_t* ph, double* u0, double* u1);
+void particles_correct(pfem_particles_t* ph, const double* u0, const double* u1, const double* uo0, const double* uo1);
+void assemble_prediction(pfem_mesh2d m, pfem_csr a0, pfem_csr a1, double* rhs0, double* rhs1, const double* u0,
+                         const double* u1, const double* uo0, const double* uo1, const double* p_old,
+                         const pfem_pfem2_params* prm);
+void assemble_pressure(pfem_mesh2d m, pfem_csr a, double* rhs, const double* vp0, const double* vp1,
+                       const double* p_old, const pfem_pfem2_params* prm);
+void assemble_correction(pfem_mesh2d m, pfem_csr a, double* rhs0, double* rhs1, const double* vp0, const double* vp1,
+                         const double* p, const double* p_old, const pfem_pfem2_params* prm);
+void prediction_bc_values(pfem_mesh2d m, pfem_dirichlet target, pfem_dirichlet source, const double* p,
+                          const double* p_old, int component, const pfem_pfem2_params* prm);
+void body_forces(pfem_mesh2d m, int body_id, double cx, double cy, const double* u0, const double* u1,
+                 const double* p, const pfem_pfem2_params* prm, double* d_out);
+}  // namespace pfem
+
+struct pfem_pfem2_t {
+    pfem_ctx ctx = nullptr;
+    pfem_mesh2d m = nullptr;
+    pfem_particles ph = nullptr;
+    pfem_step_config cfg{};
+    pfem_csr A0 = nullptr, A1 = nullptr, Ap = nullptr, Ac = nullptr;
+    pfem_dirichlet bcu[2] = {nullptr, nullptr}, bcpred[2] = {nullptr, nullptr}, bcp = nullptr;
+    pfem_precond pc_pred = nullptr, pc_pres = nullptr, pc_corr = nullptr;
+    pfem_solver s_pred = nullptr, s_pres = nullptr, s_corr = nullptr;
+    pfem::DevArray<double> f;   // u0 u1 p vp0 vp1 uo0 uo1 pold rhs0 rhs1 (n each) + 4 forces
+    int n = 0;
+    std::vector<int> its;
+    bool timing = false;
+    double phase_ms[8] = {0};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    double* field(int k) { return f.p + (size_t)k * n; }
+    ~pfem_pfem2_t()
+    {
+        pfem_solver_destroy(s_pred); pfem_solver_destroy(s_pres); pfem_solver_destroy(s_corr);
+        pfem_precond_destroy(pc_pred); pfem_precond_destroy(pc_pres); pfem_precond_destroy(pc_corr);
+        for (int k = 0; k < 2; ++k) { pfem_dirichlet_destroy(bcu[k]); pfem_dirichlet_destroy(bcpred[k]); }
+        pfem_dirichlet_destroy(bcp);
+        pfem_csr_destroy(A0); pfem_csr_destroy(A1); pfem_csr_destroy(Ap); pfem_csr_destroy(Ac);
+        if (ev[0]) (void)hipEventDestroy(ev[0]);
+        if (ev[1]) (void)hipEventDestroy(ev[1]);
+    }
+};
+
+using namespace pfem;
+
+namespace {
+enum { U0, U1, P, VP0, VP1, UO0, UO1, POLD, RHS0, RHS1, NF };
+
+void check(int rc, pfem_ctx ctx)
+{
+    if (rc != PFEM_OK) throw Error(rc, ctx->err);
+}
+
+struct Phase {
+    pfem_pfem2_t* s;
+    int k;
+    Phase(pfem_pfem2_t* s_, int k_) : s(s_), k(k_)
+    {
+        if (s->timing) PFEM_HIP(hipEventRecord(s->ev[0], s->ctx->stream));
+    }
+    ~Phase()
+    {
+        if (!s->timing) return;
+        float ms = 0.f;
+        (void)hipEventRecord(s->ev[1], s->ctx->stream);
+        (void)hipEventSynchronize(s->ev[1]);
+        (void)hipEventElapsedTime(&ms, s->ev[0], s->ev[1]);
+        s->phase_ms[k] += ms;
+    }
+};
+
+void make_solver(pfem_pfem2_t* s, pfem_csr a, int kind, int pc_kind, int ordering, pfem_precond* pc, pfem_solver* sv)
+{
+    const pfem_step_config& c = s->cfg;
+    check(pfem_precond_create(a, pc_kind, ordering, pc), s->ctx);
+    check(pfem_solver_create(a, kind, c.tol, c.maxit, c.restart, c.tol_mode, *pc, sv), s->ctx);
+    if (c.precision == PFEM_PRECISION_MIXED)
+        check(pfem_solver_set_precision(*sv, PFEM_PRECISION_MIXED, c.inner_tol, 0), s->ctx);
+}
+
+void solve(pfem_pfem2_t* s, pfem_solver sv, pfem_csr a, double* x, const double* b)
+{
+    pfem_solve_stats st{};
+    check(pfem_solver_solve(sv, a, x, b, &st), s->ctx);
+    s->its.push_back(st.iterations);
+}
+}  // namespace
+
+extern "C" {
+
+int pfem_pfem2_create(pfem_mesh2d m, pfem_particles ph, const pfem_step_config* cfg, int nbcu, const int* bcu_node,
+                      const double* bcu_val0, const double* bcu_val1, int nbcp, const int* bcp_node,
+                      const double* bcp_val, pfem_pfem2* out)
+{
+    PFEM_TRY(m ? m->ctx : nullptr, {
+        PFEM_REQUIRE(m && ph && cfg && out, "null argument");
+        auto s = std::make_unique<pfem_pfem2_t>();
+        s->ctx = m->ctx;
+        s->m = m;
+        s->ph = ph;
+        s->cfg = *cfg;
+        s->n = m->nv;
+        Context* c = C(m->ctx);
+        s->f.alloc((size_t)NF * s->n + 4);
+        zero_async(c, s->f.p, s->f.bytes());
+        check(pfem_csr_create_from_mesh2d(m, &s->A0), s->ctx);
+        check(pfem_csr_create_like(s->A0, &s->A1), s->ctx);
+        check(pfem_csr_create_like(s->A0, &s->Ap), s->ctx);
+        check(pfem_csr_create_like(s->A0, &s->Ac), s->ctx);
+        check(pfem_dirichlet_create(s->ctx, s->n, nbcu, bcu_node, bcu_val0, &s->bcu[0]), s->ctx);
+        check(pfem_dirichlet_create(s->ctx, s->n, nbcu, bcu_node, bcu_val1, &s->bcu[1]), s->ctx);
+        check(pfem_dirichlet_create(s->ctx, s->n, nbcu, bcu_node, bcu_val0, &s->bcpred[0]), s->ctx);
+        check(pfem_dirichlet_create(s->ctx, s->n, nbcu, bcu_node, bcu_val1, &s->bcpred[1]), s->ctx);
+        check(pfem_dirichlet_create(s->ctx, s->n, nbcp, bcp_node, bcp_val, &s->bcp), s->ctx);
+        make_solver(s.get(), s->A0, cfg->pred_solver, cfg->pred_precond, PFEM_ORDER_NATURAL, &s->pc_pred, &s->s_pred);
+        make_solver(s.get(), s->Ap, cfg->pres_solver, cfg->pres_precond, cfg->pres_ordering, &s->pc_pres, &s->s_pres);
+        make_solver(s.get(), s->Ac, cfg->corr_solver, cfg->corr_precond, PFEM_ORDER_NATURAL, &s->pc_corr, &s->s_corr);
+        PFEM_HIP(hipEventCreate(&s->ev[0]));
+        PFEM_HIP(hipEventCreate(&s->ev[1]));
+        PFEM_HIP(hipStreamSynchronize(c->stream));
+        *out = s.release();
+    });
+}
+
+int pfem_pfem2_destroy(pfem_pfem2 s)
+{
+    if (!s) return PFEM_OK;
+    (void)hipStreamSynchronize(s->ctx->stream);
+    delete s;
+    return PFEM_OK;
+}
+
+int pfem_pfem2_step(pfem_pfem2 s, double forces[2])
+{
+    PFEM_TRY(s ? s->ctx : nullptr, {
+        PFEM_REQUIRE(s, "null argument");
+        Context* c = C(s->ctx);
+        const pfem_step_config& cfg = s->cfg;
+        const pfem_pfem2_params* prm = &cfg.prm;
+        const int n = s->n;
+        const bool B = prm->scheme == 1;
+        double* pold = B ? s->field(POLD) : nullptr;
+        s->its.clear();
+        {
+            Phase ph(s, 0);
+            particles_advect(s->ph, s->field(U0), s->field(U1), prm->dt, cfg.substeps);
+        }
+        {
+            Phase ph(s, 1);
+            particles_project(s->ph, s->field(U0), s->field(U1));
+            launch_copy(c, s->field(UO0), s->field(U0), n);
+            launch_copy(c, s->field(UO1), s->field(U1), n);
+            if (B) launch_copy(c, s->field(POLD), s->field(P), n);
+        }
+        for (int outer = 0; outer < cfg.outer_iterations; ++outer) {
+            {
+                Phase ph(s, 2);
+                assemble_prediction(s->m, s->A0, s->A1, s->field(RHS0), s->field(RHS1), s->field(U0), s->field(U1),
+                                    s->field(UO0), s->field(UO1), pold, prm);
+                for (int k = 0; k < 2; ++k) {
+                    prediction_bc_values(s->m, s->bcpred[k], s->bcu[k], s->field(P), pold, k, prm);
+                    dirichlet_apply(c, s->bcpred[k], nullptr, k == 0 ? s->A0 : s->A1, s->field(RHS0 + k), nullptr,
+                                    cfg.dirichlet_mode);
+                }
+            }
+            {
+                Phase ph(s, 3);
+                solve(s, s->s_pred, s->A0, s->field(VP0), s->field(RHS0));
+                solve(s, s->s_pred, s->A1, s->field(VP1), s->field(RHS1));
+            }
+            {
+                Phase ph(s, 2);
+                assemble_pressure(s->m, s->Ap, s->field(RHS0), s->field(VP0), s->field(VP1), pold, prm);
+                dirichlet_apply(c, s->bcp, nullptr, s->Ap, s->field(RHS0), nullptr, cfg.dirichlet_mode);
+            }
+            {
+                Phase ph(s, 4);
+                solve(s, s->s_pres, s->Ap, s->field(P), s->field(RHS0));
+            }
+            {
+                Phase ph(s, 2);
+                assemble_correction(s->m, s->Ac, s->field(RHS0), s->field(RHS1), s->field(VP0), s->field(VP1),
+                                    s->field(P), pold, prm);
+                dirichlet_apply(c, s->bcu[0], s->bcu[1], s->Ac, s->field(RHS0), s->field(RHS1), cfg.dirichlet_mode);
+            }
+            {
+                Phase ph(s, 5);
+                solve(s, s->s_corr, s->Ac, s->field(U0), s->field(RHS0));
+                solve(s, s->s_corr, s->Ac, s->field(U1), s->field(RHS1));
+            }
+        }
+        {
+            Phase ph(s, 6);
+            particles_correct(s->ph, s->field(U0), s->field(U1), s->field(UO0), s->field(UO1));
+            double* fd = s->f.p + (size_t)NF * n;
+            if (cfg.body_id >= 0) {
+                body_forces(s->m, cfg.body_id, cfg.body_cx, cfg.body_cy, s->field(U0), s->field(U1), s->field(P), prm,
+                            fd);
+                if (forces) {
+                    double h[4];
+                    d2h_sync(c, h, fd, sizeof(h));
+                    forces[0] = (h[0] + h[2]) * cfg.force_coeff;
+                    forces[1] = (h[1] + h[3]) * cfg.force_coeff;
+                }
+            } else if (forces) {
+                forces[0] = forces[1] = 0.0;
+            }
+        }
+        PFEM_HIP(hipGetLastError());
+    });
+}
+
+int pfem_pfem2_field(pfem_pfem2 s, int which, double** d_ptr)
+{
+    if (!s || !d_ptr) return PFEM_EINVAL;
+    static const int map[8] = {U0, U1, P, VP0, VP1, UO0, UO1, POLD};
+    if (which < 0 || which > 7) return PFEM_EINVAL;
+    *d_ptr = s->field(map[which]);
+    return PFEM_OK;
+}
+
+int pfem_pfem2_last_iterations(pfem_pfem2 s, int* its, int cap)
+{
+    if (!s || !its) return PFEM_EINVAL;
+    const int k = std::min<int>(cap, (int)s->its.size());
+    for (int i = 0; i < k; ++i) its[i] = s->its[i];
+    return k;
+}
+
+int pfem_pfem2_enable_timing(pfem_pfem2 s, int on)
+{
+    if (!s) return PFEM_EINVAL;
+    s->timing = on != 0;
+    std::memset(s->phase_ms, 0, sizeof(s->phase_ms));
+    return PFEM_OK;
+}
+
+int pfem_pfem2_phase_times(pfem_pfem2 s, double* ms, int cap)
+{
+    if (!s || !ms) return PFEM_EINVAL;
+    for (int k = 0; k < std::min(cap, 8); ++k) ms[k] = s->phase_ms[k];
+    return PFEM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,148 @@
+// Krylov solver and preconditioner state (internal).
+#pragma once
+
+#include <cstddef>
+
+#include "internal.hpp"
+
+namespace pfem {
+
+constexpr int kMaxRestart = 1024;   // largest GMRES cycle (the reference's maxIterations default is 1000)
+
+// Device-resident PCG scalars. Fields before `counter` are (re)written by the host at each solve.
+struct CgState {
+    double tol;
+    int tol_mode, maxit;
+    double gamma, thr, pAp, alpha, beta, res;
+    int it, done, converged, nan;
+    unsigned counter;   // grid-reduction arrival counter, kept at zero between kernels
+};
+
+// Device-resident GMRES(m) scalars and small arrays (H column-major with leading dimension m).
+struct GmState {
+    double tol;
+    int tol_mode, maxit, m, restarted;
+    double thr;
+    double *H, *beta, *cs, *sn, *y, *hc;
+    double res, norm;
+    int k, it, done, converged, cycle_done, nan, apply_update;
+    unsigned counter;
+};
+
+#ifdef __HIPCC__
+// Start of a cycle (thread 0 of the last block): beta_0 = ||w||, convergence test
+// (linear_solver.cu:446-453; tolerance relative to ||M^-1 b|| when tol_mode is relative).
+__device__ inline void gm_cycle_start(GmState* st, double w2, double mb2)
+{
+    if (st->thr < 0.0) st->thr = st->tol_mode == PFEM_TOL_RELATIVE ? st->tol * sqrt(mb2) : st->tol;
+    const double b0 = sqrt(w2);
+    st->beta[0] = b0;
+    st->res = b0;
+    st->norm = b0;
+    st->k = 0;
+    st->cycle_done = 0;
+    if (b0 != b0) {
+        st->nan = 1; st->done = 1; st->cycle_done = 1;
+    } else if (b0 < st->thr) {
+        st->converged = 1; st->done = 1; st->cycle_done = 1;
+    }
+}
+
+// End of Arnoldi step j = st->k (thread 0 of the last block): updateHcolumn (linear_solver.cu:40-66)
+// with h_{j+1,j} = ||w||. GivensRotation (cuda_math.cuh:105-113) uses 1/sqrt here.
+__device__ inline void gm_arnoldi_finish(GmState* st, double hn)
+{
+    const int j = st->k, m = st->m;
+    double* h = st->H + (size_t)j * m;
+    double* cs = st->cs;
+    double* sn = st->sn;
+    for (int i = 0; i < j; ++i) {
+        const double t = cs[i] * h[i] + sn[i] * h[i + 1];
+        h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
+        h[i] = t;
+    }
+    const double coeff = 1.0 / sqrt(h[j] * h[j] + hn * hn);
+    cs[j] = h[j] * coeff;
+    sn[j] = hn * coeff;
+    h[j] = cs[j] * h[j] + sn[j] * hn;
+    st->beta[j + 1] = -sn[j] * st->beta[j];
+    st->beta[j] *= cs[j];
+    const double res = fabs(st->beta[j + 1]);
+    st->res = res;
+    st->norm = hn;
+    st->it += 1;
+    st->k = j + 1;
+    if (res != res) {
+        st->nan = 1; st->done = 1; st->cycle_done = 1;
+    } else if (res < st->thr) {
+        st->converged = 1; st->done = 1; st->cycle_done = 1;
+    } else if (st->it >= st->maxit) {
+        st->done = 1; st->cycle_done = 1;
+    } else if (st->k >= m) {
+        st->cycle_done = 1;
+    }
+}
+#endif
+
+}  // namespace pfem
+
+struct pfem_precond_t {
+    pfem_ctx ctx = nullptr;
+    int kind = PFEM_PRECOND_NONE;
+    int ordering = PFEM_ORDER_NATURAL;
+    int n = 0;
+    std::shared_ptr<pfem::Pattern> pat;   // pattern of the matrices it is initialised from
+    pfem::DevArray<double> invd;
+    pfem::DevArray<float> invd32;
+    // IC0 / ILU0: factor of (permuted) A, stored on the permuted pattern
+    std::shared_ptr<pfem::Pattern> fpat;  // factor pattern (== pat for natural ordering)
+    pfem::DevArray<int> src_pos;          // factor slot k takes A's value at src_pos[k]
+    pfem::DevArray<int> perm, iperm;      // new -> old, old -> new (multicolour)
+    std::vector<int> h_perm;
+    pfem::DevArray<int> mirror;           // for each slot (i,k) with k > i: position of (k,i)
+    pfem::DevArray<int> lrows, urows;     // rows grouped by forward / backward level
+    std::vector<int> h_lptr, h_uptr;      // level offsets into lrows / urows
+    pfem::DevArray<double> fac;
+    pfem::DevArray<float> fac32;
+    pfem::DevArray<double> tmp, tmp2;
+    pfem::DevArray<float> tmp32, tmp32b;
+    pfem::DevArray<int> status;           // first failing row (+1), 0 if none
+    int nlevels = 0;
+};
+
+struct pfem_solver_t {
+    pfem_ctx ctx = nullptr;
+    int kind = PFEM_SOLVER_CG;
+    double tol = 1e-8;
+    int maxit = 1000, restart = 1000, m = 0, restarted = 0, tol_mode = PFEM_TOL_ABSOLUTE;
+    pfem_precond_t* pc = nullptr;
+    int poll = 8;       // iterations launched between two host polls of the done flag
+    int reorth = 1;     // GMRES: classical Gram-Schmidt with one re-orthogonalisation pass
+    int precision = PFEM_PRECISION_FP64;
+    double inner_tol = 1e-6;
+    int max_refine = 50;
+    int n = 0;
+    pfem::DevArray<double> r, z, p, q, t, V, H, beta, cs, sn, y, hc, partials;
+    pfem::DevArray<pfem::CgState> cg;
+    pfem::DevArray<pfem::GmState> gm;
+    pfem::CgState* h_cg = nullptr;   // pinned
+    pfem::GmState* h_gm = nullptr;   // pinned
+    // mixed precision (FP32 inner CG, FP64 refinement)
+    pfem::DevArray<float> r32, z32, p32, q32, d32, t32, V32;
+    pfem::DevArray<double> xr;       // refinement residual, its preconditioned copy, refinement state
+};
+
+namespace pfem {
+void precond_initialize(Context* c, pfem_precond_t* pc, pfem_csr_t* a);
+void precond_apply(Context* c, pfem_precond_t* pc, double* z, const double* r);
+void precond_apply32(Context* c, pfem_precond_t* pc, float* z, const float* r);
+void jacobi_setup(Context* c, const Pattern& P, const double* val, double* invd, float* invd32);
+void jacobi_apply(Context* c, int n, double* dest, const double* src, const double* invd);
+void jacobi_apply32(Context* c, int n, float* dest, const float* src, const float* invd);
+void cg_solve_fp64(Context* c, pfem_solver_t* s, pfem_csr_t* a, double* x, const double* b, pfem_solve_stats* out);
+void gmres_solve_fp64(Context* c, pfem_solver_t* s, pfem_csr_t* a, double* x, const double* b,
+                      pfem_solve_stats* out);
+void solve_mixed(Context* c, pfem_solver_t* s, pfem_csr_t* a, double* x, const double* b, pfem_solve_stats* out);
+void dirichlet_apply(Context* c, pfem_dirichlet_t* bc0, pfem_dirichlet_t* bc1, pfem_csr_t* a, double* rhs0,
+                     double* rhs1, int mode);
+}  // namespace pfem

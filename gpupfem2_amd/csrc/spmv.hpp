@@ -1,0 +1,50 @@
+// CSR SpMV tile routine shared by the SpMV-based kernels.
+//
+// One 256-lane workgroup per row tile (<= 256 rows, <= kTileNnz non-zeros). The tile's val/col stream is
+// read with unit stride into LDS as products val*x[col]; each lane then sums its row's products left to
+// right. Loads are fully coalesced, the only irregular access is the x gather, and the per-row order is
+// fixed (bit-identical to a sequential CPU loop without FMA contraction).
+#pragma once
+
+#include "device.hpp"
+
+namespace pfem {
+
+constexpr int kTileNnz = 2048;
+
+// Product of the calling lane's row with x (tile t = blockIdx.x). Returns the row index in r
+// (r >= r1 means the lane owns no row). prod is an LDS buffer of kTileNnz entries.
+template <typename VT, typename XT>
+__device__ __forceinline__ XT tile_row_product(const int* __restrict__ tiles, const int* __restrict__ rowptr,
+                                               const int* __restrict__ col, const VT* __restrict__ val,
+                                               const XT* __restrict__ x, XT* prod, int& r, int& r1)
+{
+    const int t = blockIdx.x;
+    const int r0 = tiles[t];
+    r1 = tiles[t + 1];
+    const int k0 = rowptr[r0];
+    const int nk = rowptr[r1] - k0;
+    r = r0 + (int)threadIdx.x;
+    XT sum = 0;
+    if (nk <= kTileNnz) {
+        for (int k = threadIdx.x; k < nk; k += kBlock) prod[k] = (XT)val[k0 + k] * x[col[k0 + k]];
+        __syncthreads();
+        if (r < r1) {
+            const int a = rowptr[r] - k0, b = rowptr[r + 1] - k0;
+            for (int k = a; k < b; ++k) sum += prod[k];
+        }
+    } else {
+        // a single row longer than the LDS tile: streamed in chunks, summed in order by lane 0
+        for (int base = 0; base < nk; base += kTileNnz) {
+            const int m = min(kTileNnz, nk - base);
+            for (int k = threadIdx.x; k < m; k += kBlock) prod[k] = (XT)val[k0 + base + k] * x[col[k0 + base + k]];
+            __syncthreads();
+            if (threadIdx.x == 0)
+                for (int k = 0; k < m; ++k) sum += prod[k];
+            __syncthreads();
+        }
+    }
+    return sum;
+}
+
+}  // namespace pfem

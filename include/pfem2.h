@@ -1,0 +1,302 @@
+/*
+ * pfem2.h -- C ABI of libpfem2_hip.so, the MI355X (gfx950) PFEM-2 hot path.
+ *
+ * Every entry point returns 0 on success or a negative PFEM_E* code; pfem_last_error() gives the
+ * message. Nothing in the library calls exit(). All device work is queued on the context's stream;
+ * the calls that hand a value back to the host (counts, solver statistics, downloads) synchronise
+ * that stream. Pointers named d_* are device pointers; every other pointer is host memory that is
+ * only borrowed for the duration of the call.
+ *
+ * Each declaration names the reference interface it replaces (paths relative to the reference
+ * repository andreyypopov/gpuPfem2).
+ */
+#ifndef PFEM2_H
+#define PFEM2_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PFEM_OK 0
+#define PFEM_EINVAL (-1)   /* bad argument */
+#define PFEM_EHIP (-2)     /* HIP runtime error */
+#define PFEM_EIO (-3)      /* file could not be opened or parsed */
+#define PFEM_ENUMERIC (-4) /* zero/negative pivot, NaN */
+#define PFEM_ECAPACITY (-5)/* a fixed-size buffer would overflow */
+
+typedef struct pfem_context* pfem_ctx;
+typedef struct pfem_mesh2d_t* pfem_mesh2d;
+typedef struct pfem_mesh3d_t* pfem_mesh3d;
+typedef struct pfem_csr_t* pfem_csr;
+typedef struct pfem_dirichlet_t* pfem_dirichlet;
+typedef struct pfem_precond_t* pfem_precond;
+typedef struct pfem_solver_t* pfem_solver;
+typedef struct pfem_particles_t* pfem_particles;
+typedef struct pfem_pfem2_t* pfem_pfem2;
+
+/* ------------------------------------------------------------------ context and device memory */
+/* replaces the implicit CUDA context + legacy default stream (src/common/cuda_memory.cuh:17-148) */
+int pfem_ctx_create(int device, pfem_ctx* out);
+int pfem_ctx_destroy(pfem_ctx ctx);
+const char* pfem_last_error(pfem_ctx ctx);
+int pfem_ctx_synchronize(pfem_ctx ctx);
+/* the context stream, as a void* (hipStream_t) for callers that order their own work on it */
+void* pfem_ctx_stream(pfem_ctx ctx);
+/* deviceVector<T> substrate: allocate_device / free_device / copy_h2d / copy_d2h / copy_d2d /
+ * zero_value_device (src/common/cuda_memory.cuh:17-148, src/common/device_vector.cuh:19-146) */
+int pfem_malloc(pfem_ctx ctx, size_t bytes, void** d_out);
+int pfem_free(pfem_ctx ctx, void* d_ptr);
+int pfem_memcpy_h2d(pfem_ctx ctx, void* d_dst, const void* src, size_t bytes);
+int pfem_memcpy_d2h(pfem_ctx ctx, void* dst, const void* d_src, size_t bytes);
+int pfem_memcpy_d2d(pfem_ctx ctx, void* d_dst, const void* d_src, size_t bytes);
+int pfem_memset_zero(pfem_ctx ctx, void* d_ptr, size_t bytes);
+
+/* ------------------------------------------------------------------ 2D mesh (src/mesh_2d.cuh:11-69) */
+/* Mesh2D::loadMeshFromFile(filename, fillNeighborLists, scale) (src/mesh_2d.cu:36-96), including the
+ * reader's quirk that a trailing newline drops the last cell from the device mesh. */
+int pfem_mesh2d_load_dat(pfem_ctx ctx, const char* path, int fill_neighbors, double scale, pfem_mesh2d* out);
+/* Mesh2D from host arrays (vertices xy[2*nv], triangles tri[3*nt], 0-based) -- generated meshes */
+int pfem_mesh2d_create(pfem_ctx ctx, int nv, const double* xy, int nt, const int* tri, int fill_neighbors,
+                       pfem_mesh2d* out);
+int pfem_mesh2d_destroy(pfem_mesh2d m);
+/* sizes: vertices, device cells, host cells, neighbour entries */
+int pfem_mesh2d_info(pfem_mesh2d m, int* nv, int* nt, int* nt_host, long* n_neighbors);
+/* getters of src/mesh_2d.cuh:18-53, as host copies. what: 0 vertices (2 dbl/vtx), 1 cells (3 int),
+ * 2 cell area (1 dbl), 3 inverse Jacobian (4 dbl, row-major), 4 neighbour offsets (nt+1 int),
+ * 5 neighbour indices (int), 6 edge boundary IDs (3 int per cell), 7 host cells (3 int per host cell) */
+int pfem_mesh2d_download(pfem_mesh2d m, int what, void* dst);
+/* kSetEdgeBoundaryIDs (cases/Cylinder2D/main.cu:35-70, cases/PoiseuilleFlow2D/main.cu:48-79), with the
+ * geometry as parameters: edge midpoints within 2e-6 of x0 / x1 / y0 / y1 get id_x0 / id_x1 / id_y0 /
+ * id_y1 (checked in that order); when circle_r > 0, midpoints with | |m - c| - r | < circle_tol get
+ * id_circle. Other edges get -1. */
+int pfem_mesh2d_classify_edges(pfem_mesh2d m, double x0, double x1, double y0, double y1, int id_x0, int id_x1,
+                               int id_y0, int id_y1, double circle_cx, double circle_cy, double circle_r,
+                               double circle_tol, int id_circle);
+/* explicit edge IDs (3 per device cell) */
+int pfem_mesh2d_set_edge_ids(pfem_mesh2d m, const int* ids);
+
+/* ------------------------------------------------------------------ 3D mesh (src/mesh_3d.cuh) */
+/* Mesh3D::loadMeshFromFile (src/mesh_3d.cu:84-147); face neighbours are not built (no 3D particles).
+ * zero_small_det = 1 keeps the reference's zero inverse Jacobian for |det| < 2e-6
+ * (src/common/matrix3x3.cuh:139-140); 0 keeps the true inverse. */
+int pfem_mesh3d_load_dat(pfem_ctx ctx, const char* path, double scale, int zero_small_det, pfem_mesh3d* out);
+int pfem_mesh3d_create(pfem_ctx ctx, int nv, const double* xyz, int nt, const int* tet, int zero_small_det,
+                       pfem_mesh3d* out);
+int pfem_mesh3d_destroy(pfem_mesh3d m);
+int pfem_mesh3d_info(pfem_mesh3d m, int* nv, int* nt);
+/* what: 0 vertices (3 dbl), 1 cells (4 int), 2 volume (1 dbl), 3 inverse Jacobian (9 dbl, row-major) */
+int pfem_mesh3d_download(pfem_mesh3d m, int what, void* dst);
+
+/* ------------------------------------------------------------------ CSR (src/linear_algebra/sparse_matrix.cuh) */
+/* SparseMatrixCSR(const Mesh2D&) / (const Mesh3D&) (sparse_matrix.cu:8-99): pattern {i} U {vertices
+ * sharing a cell with i}, columns ascending, values zero. Built in O(nnz) instead of std::set. */
+int pfem_csr_create_from_mesh2d(pfem_mesh2d m, pfem_csr* out);
+int pfem_csr_create_from_mesh3d(pfem_mesh3d m, pfem_csr* out);
+/* a matrix from host CSR arrays (values may be NULL: zero) */
+int pfem_csr_create(pfem_ctx ctx, int n, int nnz, const int* rowptr, const int* col, const double* val,
+                    pfem_csr* out);
+/* a second value array on the same (shared) pattern */
+int pfem_csr_create_like(pfem_csr a, pfem_csr* out);
+int pfem_csr_destroy(pfem_csr a);
+int pfem_csr_info(pfem_csr a, int* n, int* nnz);
+/* getRowOffset / getColIndices / getMatrixValues (sparse_matrix.cuh:24-34): device pointers */
+int pfem_csr_pointers(pfem_csr a, int** d_rowptr, int** d_col, double** d_val);
+/* clearValues (sparse_matrix.cuh:46-48) */
+int pfem_csr_clear_values(pfem_csr a);
+int pfem_csr_upload_values(pfem_csr a, const double* val);
+int pfem_csr_download(pfem_csr a, int* rowptr, int* col, double* val);
+/* y = A x: LinearAlgebra::sparseMV (src/linear_algebra/linear_algebra.h:75-80, cusparseSpMV CSR, FP64).
+ * Per-row sums run left to right, so results are deterministic. */
+int pfem_spmv(pfem_csr a, const double* d_x, double* d_y);
+/* FP32 copy of the values (mixed-precision inner solves) and its product y32 = A32 x32 */
+int pfem_csr_refresh_fp32(pfem_csr a);
+int pfem_spmv_fp32(pfem_csr a, const float* d_x, float* d_y);
+
+/* ------------------------------------------------------------------ Dirichlet BCs (src/Dirichlet_bcs.cuh) */
+#define PFEM_DIRICHLET_PARITY 0  /* reference heuristic: skip row c when |first off-diagonal| < 2e-6 */
+#define PFEM_DIRICHLET_MASK 1    /* skip exactly the Dirichlet rows (symmetric result) */
+/* DirichletBCs::setupDirichletBCs + setupNodeMap (Dirichlet_bcs.cu:53-71). n = matrix rows. */
+int pfem_dirichlet_create(pfem_ctx ctx, int n, int nbc, const int* node, const double* value, pfem_dirichlet* out);
+int pfem_dirichlet_destroy(pfem_dirichlet bc);
+int pfem_dirichlet_set_values(pfem_dirichlet bc, const double* value);            /* host values */
+int pfem_dirichlet_values_device(pfem_dirichlet bc, double** d_value);            /* device value array */
+/* DirichletBCs::applyBCs(matrix, rhs) (Dirichlet_bcs.cu:73-86) */
+int pfem_dirichlet_apply(pfem_dirichlet bc, pfem_csr a, double* d_rhs, int mode);
+/* two right-hand sides with their own values on one matrix (both velocity components share their BC
+ * node set, cases/Cylinder2D/main.cu:675-689): same result as applying bc0 and bc1 to two copies */
+int pfem_dirichlet_apply2(pfem_dirichlet bc0, pfem_dirichlet bc1, pfem_csr a, double* d_rhs0, double* d_rhs1,
+                          int mode);
+
+/* ------------------------------------------------------------------ preconditioners (preconditioners.cuh) */
+#define PFEM_PRECOND_NONE 0
+#define PFEM_PRECOND_JACOBI 1
+#define PFEM_PRECOND_IC0 2
+#define PFEM_PRECOND_ILU0 3
+/* ordering for IC0/ILU0: natural (exact reference factor, level-scheduled) or multicolour (the
+ * factor of the symmetrically permuted matrix; few levels) */
+#define PFEM_ORDER_NATURAL 0
+#define PFEM_ORDER_MULTICOLOR 1
+/* PreconditionerJacobi(n, LA) / PreconditionerIC(matrix, LA) / PreconditionerILU(matrix, LA)
+ * (preconditioners.cuh:9-90); the pattern of `a` fixes the structure */
+int pfem_precond_create(pfem_csr a, int kind, int ordering, pfem_precond* out);
+int pfem_precond_destroy(pfem_precond pc);
+/* Preconditioner::initialize(const SparseMatrixCSR&) */
+int pfem_precond_initialize(pfem_precond pc, pfem_csr a);
+/* Preconditioner::applyPreconditioner(dest, src); dest may equal src */
+int pfem_precond_apply(pfem_precond pc, double* d_dest, const double* d_src);
+/* factor values (nnz doubles, in the matrix's own CSR layout and row order), levels, permutation */
+int pfem_precond_download(pfem_precond pc, double* factor, int* n_levels, int* perm);
+
+/* ------------------------------------------------------------------ Krylov solvers (linear_solver.cuh) */
+#define PFEM_SOLVER_CG 0
+#define PFEM_SOLVER_GMRES 1
+#define PFEM_TOL_ABSOLUTE 0  /* reference: CG (r, M^-1 r) < tol^2; GMRES |beta| < tol */
+#define PFEM_TOL_RELATIVE 1  /* CG (r, M^-1 r) < tol^2 (b, M^-1 b); GMRES |beta| < tol ||M^-1 b|| */
+#define PFEM_PRECISION_FP64 0
+#define PFEM_PRECISION_MIXED 1 /* FP32 inner Krylov + FP64 iterative refinement */
+typedef struct {
+    int converged;
+    int iterations;        /* Krylov iterations (for MIXED: inner iterations summed) */
+    double final_residual; /* CG: sqrt((r, M^-1 r)); GMRES: |beta| */
+    int nan_detected;
+    int refinements;       /* MIXED: outer FP64 refinement steps */
+} pfem_solve_stats;
+/* SolverCG(tol, maxIt, LA, precond) / SolverGMRES(...) (linear_solver.cuh:60-137) + init(matrix).
+ * restart: GMRES(m) restart length; restart >= max_iterations reproduces the reference's
+ * non-restarted GMRES (x untouched unless converged). precond may be NULL. */
+int pfem_solver_create(pfem_csr a, int kind, double tol, int max_iterations, int restart, int tol_mode,
+                       pfem_precond precond, pfem_solver* out);
+int pfem_solver_set_precision(pfem_solver s, int precision, double inner_tol, int max_refinements);
+int pfem_solver_destroy(pfem_solver s);
+/* LinearSolver::solve(A, x, b) -> bool (linear_solver.cu:146-154, 301-376, 421-521): warm start from x,
+ * the preconditioner is re-initialised from A first. stats may be NULL. */
+int pfem_solver_solve(pfem_solver s, pfem_csr a, double* d_x, const double* d_b, pfem_solve_stats* stats);
+
+/* ------------------------------------------------------------------ quadrature */
+/* createEdgeQuadratureFormula / createFaceQuadratureFormula / createCellQuadratureFormula
+ * (src/integration/quadrature_formula_{1d,2d,3d}.cu). kind 1: edge rule i in 0..4 -> (coord, weight);
+ * kind 2: face rule i in 0..7 -> (l1, l2, l3, weight); kind 3: tet rule i in 0..1 -> (l1..l4, weight).
+ * Returns the number of points (or a negative error code); out needs room for 13*5 doubles. */
+int pfem_quadrature(int kind, int index, double* out);
+
+/* ------------------------------------------------------------------ assembly (case integrands) */
+/* cases/Poisson2D/main.cu:28-68 (kIntegrateOverCell) as a library operator: coeff replaces the
+ * hard-coded 0.5; rhs_kind 0 exp(-(x^2+10y^2)), 1 rhs_param*sin(pi x)sin(pi y), 2 constant rhs_param.
+ * Overwrites the matrix values and d_rhs (no clearValues needed). quad_index: face rule. */
+int pfem_assemble_poisson2d(pfem_mesh2d m, pfem_csr a, double* d_rhs, double coeff, int rhs_kind,
+                            double rhs_param, int quad_index);
+/* cases/Poisson3D/main.cu:26-67; rhs_kind as above with z; quad_index: tet rule (0 = 4 pt, 1 = 6 pt) */
+int pfem_assemble_poisson3d(pfem_mesh3d m, pfem_csr a, double* d_rhs, double coeff, int rhs_kind,
+                            double rhs_param, int quad_index);
+
+/* SimulationParameters (src/parameters.cuh:6-51), field for field, plus the build's extras */
+typedef struct {
+    double rho, mu, tFinal, dt, tolerance;
+    int maxIterations, simulationScheme;
+    const char* meshFileName;
+    int outputFrequency, particleAdvectionSubsteps, exportParticles, calculateLoads, bodyBoundaryID;
+    double thickness, meanVelocity;
+} pfem_simulation_parameters;
+void pfem_simulation_parameters_default(pfem_simulation_parameters* p); /* setDefaultParameters() */
+
+typedef struct {
+    double rho, mu, dt;
+    int scheme;            /* 0 Scheme A, 1 Scheme B (reference: pressureOld != nullptr) */
+    int face_quad, edge_quad;
+    int traction_mask;     /* bit b: boundary ID b carries the viscous traction edge term */
+    double normal[8][2];   /* outward unit normal per boundary ID */
+} pfem_pfem2_params;
+/* kIntegrateVelocityPrediction (cases/Cylinder2D/main.cu:72-167, PoiseuilleFlow2D/main.cu:81-176):
+ * two matrices on one pattern + two rhs; d_p_old NULL for Scheme A */
+int pfem_assemble_prediction(pfem_mesh2d m, pfem_csr a0, pfem_csr a1, double* d_rhs0, double* d_rhs1,
+                             const double* d_u0, const double* d_u1, const double* d_uo0, const double* d_uo1,
+                             const double* d_p_old, const pfem_pfem2_params* prm);
+/* kIntegratePressureEquation (cases/Cylinder2D/main.cu:169-211) */
+int pfem_assemble_pressure(pfem_mesh2d m, pfem_csr a, double* d_rhs, const double* d_vp0, const double* d_vp1,
+                           const double* d_p_old, const pfem_pfem2_params* prm);
+/* kIntegrateVelocityCorrection (cases/Cylinder2D/main.cu:213-259): the two matrices are identical, one
+ * is written */
+int pfem_assemble_correction(pfem_mesh2d m, pfem_csr a, double* d_rhs0, double* d_rhs1, const double* d_vp0,
+                             const double* d_vp1, const double* d_p, const double* d_p_old,
+                             const pfem_pfem2_params* prm);
+/* VelocityDirichletBCs::setDirichletValues (cases/Cylinder2D/main.cu:261-297, 467-479): writes
+ * g*_b = g_b + dt/rho * avg_cells(d_component(p - p_old)) into target's device values */
+int pfem_prediction_bc_values(pfem_mesh2d m, pfem_dirichlet target, pfem_dirichlet source, const double* d_p,
+                              const double* d_p_old, int component, const pfem_pfem2_params* prm);
+/* boundaryLoadsCalculator (cases/Cylinder2D/main.cu:299-445): forces {Fp_x, Fp_y, Fv_x, Fv_y} on the
+ * edges with boundary ID body_id, outward normal taken from the centre (cx, cy) */
+int pfem_body_forces(pfem_mesh2d m, int body_id, double cx, double cy, const double* d_u0, const double* d_u1,
+                     const double* d_p, const pfem_pfem2_params* prm, double out[4]);
+
+/* ------------------------------------------------------------------ particles (particle_handler_2d.cuh) */
+/* ParticleHandler2D(const Mesh2D*, int cellDivisionLevel) (particle_handler_2d.cu:238-294) */
+int pfem_particles_create(pfem_mesh2d m, int level, pfem_particles* out);
+int pfem_particles_destroy(pfem_particles ph);
+/* seedParticles (particle_handler_2d.cu:304-320). mode 0: L^2 particles at the subcell centroids
+ * (reference); mode 1: ppc particles per cell at uniform random barycentric positions from
+ * Philox4x32-10(key = seed, counter = (cell, k, 0, 0)). */
+int pfem_particles_seed(pfem_particles ph, int mode, int ppc, uint64_t seed);
+/* initParticleVelocity (:322-326) */
+int pfem_particles_init_velocity(pfem_particles ph, const double* d_u0, const double* d_u1);
+/* advectParticles(velocity, dt, substeps) (:328-342): advection + relocation per substep, deletion of
+ * particles that leave the 1-ring, subcell refill; the particles are then ordered by cell */
+int pfem_particles_advect(pfem_particles ph, const double* d_u0, const double* d_u1, double dt, int substeps);
+/* projectVelocityOntoGrid (:350-361) */
+int pfem_particles_project(pfem_particles ph, double* d_u0, double* d_u1);
+/* correctParticleVelocity (:344-348) */
+int pfem_particles_correct(pfem_particles ph, const double* d_u0, const double* d_u1, const double* d_uo0,
+                           const double* d_uo1);
+/* getParticleCount (:30-32) */
+int pfem_particles_count(pfem_particles ph, long* count);
+/* host copy of the particle state: cell, lambda (3 per), position (2), velocity (2), ID; NULL skips */
+int pfem_particles_download(pfem_particles ph, int* cell, double* lam, double* pos, double* vel, uint32_t* id);
+int pfem_particles_upload(pfem_particles ph, long count, const int* cell, const double* lam, const double* pos,
+                          const double* vel, const uint32_t* id, uint32_t next_id);
+/* particles deleted / added by the last advect */
+int pfem_particles_stats(pfem_particles ph, long* deleted_last, long* added_last);
+
+/* ------------------------------------------------------------------ PFEM-2 time step (cases: Cylinder2D, PoiseuilleFlow2D main.cu) */
+typedef struct {
+    pfem_pfem2_params prm;
+    int outer_iterations;      /* Cylinder2D: 2 (main.cu:810); PoiseuilleFlow2D: 1 (main.cu:669) */
+    int substeps;              /* particleAdvectionSubsteps */
+    int pred_solver, pred_precond, pres_solver, pres_precond, corr_solver, corr_precond;
+    double tol;
+    int tol_mode, maxit, restart;
+    int dirichlet_mode;
+    int body_id;               /* -1: no loads */
+    double body_cx, body_cy, force_coeff;
+    int precision;             /* PFEM_PRECISION_FP64 or PFEM_PRECISION_MIXED */
+    double inner_tol;          /* MIXED: relative tolerance of each FP32 inner solve */
+    int pres_ordering;         /* PFEM_ORDER_* for an IC0/ILU0 pressure preconditioner */
+} pfem_step_config;
+/* The time loop body of cases/Cylinder2D/main.cu:793-880 on a mesh with edge IDs set. Velocity BCs
+ * share one node list with a value per component; pressure BCs have their own list. */
+int pfem_pfem2_create(pfem_mesh2d m, pfem_particles ph, const pfem_step_config* cfg, int nbcu, const int* bcu_node,
+                      const double* bcu_val0, const double* bcu_val1, int nbcp, const int* bcp_node,
+                      const double* bcp_val, pfem_pfem2* out);
+int pfem_pfem2_destroy(pfem_pfem2 s);
+/* one time step; forces (may be NULL) receives {Cx, Cy} = 2 F / (rho U^2 D) */
+int pfem_pfem2_step(pfem_pfem2 s, double forces[2]);
+/* device pointers of the fields: 0 u_x, 1 u_y, 2 p, 3 u*_x, 4 u*_y, 5 u_old_x, 6 u_old_y, 7 p_old */
+int pfem_pfem2_field(pfem_pfem2 s, int which, double** d_ptr);
+/* Krylov iterations of the last step, 5 per outer iteration (pred x, pred y, pressure, corr x, corr y) */
+int pfem_pfem2_last_iterations(pfem_pfem2 s, int* its, int cap);
+/* accumulated device time per phase since the last reset, in ms: 0 particles (advect+refill+sort),
+ * 1 projection, 2 assembly+BCs, 3 prediction solves, 4 pressure solves, 5 correction solves,
+ * 6 particle correction + loads. Timing is off unless enabled (it synchronises at phase ends). */
+int pfem_pfem2_enable_timing(pfem_pfem2 s, int on);
+int pfem_pfem2_phase_times(pfem_pfem2 s, double* ms, int cap);
+
+/* ------------------------------------------------------------------ export (src/data_export.cuh) */
+/* DataExport::exportToVTK (src/data_export.cu:17-95): ASCII VTU with nodal scalars (device arrays) */
+int pfem_export_vtk(pfem_mesh2d m, const char* path, int nfields, const char* const* names,
+                    const double* const* d_fields);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

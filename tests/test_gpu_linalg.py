@@ -1,0 +1,300 @@
+"""GPU parity: mesh import, setup builders, SpMV, assembly, Dirichlet, preconditioners and solvers
+against the CPU oracle on the same inputs.
+
+Bar: bit-exact for integer data (cells, CSR structure, neighbour lists) and for every kernel whose
+floating-point operation order matches the oracle (geometry, SpMV, assembly, Dirichlet, Jacobi, IC0/ILU0
+factors); relative L2 <= 1e-10 for Krylov solutions (their dot products are reduced in a different
+order, which moves the last bits and can change the iteration count by a few).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import golden, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+FIX2D = ["TestProblem2.dat", "ChannelMesh.dat", "CylinderMesh.dat", "CylinderMesh3.dat"]
+
+
+@pytest.mark.parametrize("fname", FIX2D)
+def test_mesh2d_import_and_setup(ctx, oracle, fname):
+    from gpupfem2_amd import capi
+    print("mesh2d import", fname, flush=True)
+    v, c, ndev = oracle.load_dat(golden(fname), 2)
+    m = capi.Mesh2D(ctx, path=golden(fname))
+    assert (m.nv, m.nt, m.nt_host) == (v.shape[0], ndev, c.shape[0])
+    assert np.array_equal(m.download(0).reshape(-1, 2), v)
+    assert np.array_equal(m.download(1).reshape(-1, 3), c[:ndev])
+    area, invJ = oracle.geometry2d(v, c)
+    assert np.array_equal(m.download(2), area)
+    assert np.array_equal(m.download(3).reshape(-1, 4), invJ)
+    if c.shape[0] < 20000:
+        off, idx = oracle.neighbors_bruteforce(c)
+        assert np.array_equal(m.download(4), off) and np.array_equal(m.download(5), idx)
+    rp, col = oracle.csr_pattern(v.shape[0], c)
+    a = capi.CSR(ctx, mesh=m)
+    grp, gcol, gval = a.download()
+    assert np.array_equal(grp, rp) and np.array_equal(gcol, col) and np.all(gval == 0)
+
+
+def test_mesh2d_trailing_newline(ctx, oracle, tmp_path):
+    from gpupfem2_amd import capi
+    p = tmp_path / "nl.dat"
+    p.write_text(open(golden("TestProblem2.dat")).read() + "\n")
+    m = capi.Mesh2D(ctx, path=str(p))
+    assert m.nt == 1711 and m.nt_host == 1712
+    with pytest.raises(capi.PfemError):
+        capi.Mesh2D(ctx, path=str(tmp_path / "missing.dat"))
+
+
+def test_mesh3d_import(ctx, oracle):
+    from gpupfem2_amd import capi
+    v, c, ndev = oracle.load_dat(golden("BoxMesh.dat"), 3)
+    m = capi.Mesh3D(ctx, path=golden("BoxMesh.dat"))
+    assert (m.nv, m.nt) == (v.shape[0], ndev)
+    assert np.array_equal(m.download(0).reshape(-1, 3), v)
+    assert np.array_equal(m.download(1).reshape(-1, 4), c)
+    vol, invJ = oracle.geometry3d(v, c)
+    assert np.array_equal(m.download(2), vol) and np.array_equal(m.download(3).reshape(-1, 9), invJ)
+    rp, col = oracle.csr_pattern(v.shape[0], c)
+    grp, gcol, _ = capi.CSR(ctx, mesh=m).download()
+    assert np.array_equal(grp, rp) and np.array_equal(gcol, col)
+
+
+def test_generated_cylinder_setup(ctx, oracle):
+    from gpupfem2_amd import capi, cases
+    case = cases.cylinder_case(h=1.0 / 8)
+    m = capi.Mesh2D(ctx, case.xy, case.tri)
+    off, idx = oracle.neighbors_bruteforce(case.tri)
+    assert np.array_equal(m.download(4), off) and np.array_equal(m.download(5), idx)
+    m.classify_edges(**{k: v for k, v in case.classify.items() if k != "circle_id"})
+    assert np.array_equal(m.download(6).reshape(-1, 3), case.edge_ids)
+
+
+def laplace(ctx, oracle, n=32):
+    from gpupfem2_amd import capi, meshgen
+    xy, tri = meshgen.unit_square(n)
+    m = capi.Mesh2D(ctx, xy, tri)
+    a = capi.CSR(ctx, mesh=m)
+    area, invJ = oracle.geometry2d(xy, tri)
+    rp, col = oracle.csr_pattern(xy.shape[0], tri)
+    return xy, tri, m, a, area, invJ, rp, col
+
+
+@pytest.mark.parametrize("rhs_kind", [2, 1, 0])
+def test_poisson2d_assembly(ctx, oracle, rhs_kind):
+    from gpupfem2_amd import capi
+    xy, tri, m, a, area, invJ, rp, col = laplace(ctx, oracle)
+    rhs = ctx.zeros(xy.shape[0])
+    capi.assemble_poisson2d(m, a, rhs, coeff=0.5, rhs_kind=rhs_kind, rhs_param=2 * math.pi ** 2)
+    val, r = oracle.assemble_poisson2d(xy, tri, area, invJ, rp, col, coeff=0.5, rhs_kind=rhs_kind,
+                                       rhs_param=2 * math.pi ** 2)
+    assert np.array_equal(a.download()[2], val)
+    if rhs_kind == 2:
+        assert np.array_equal(rhs.download(), r)
+    else:   # sin/exp come from different math libraries: a few ulp
+        assert rel_l2(rhs.download(), r) < 1e-15
+
+
+def test_poisson3d_assembly(ctx, oracle):
+    from gpupfem2_amd import capi, meshgen
+    xyz, tet = meshgen.box3d(6)
+    m = capi.Mesh3D(ctx, xyz, tet)
+    a = capi.CSR(ctx, mesh=m)
+    rhs = ctx.zeros(xyz.shape[0])
+    capi.assemble_poisson3d(m, a, rhs, coeff=1.0, rhs_kind=2, rhs_param=3.0, quad=0)
+    vol, invJ = oracle.geometry3d(xyz, tet)
+    rp, col = oracle.csr_pattern(xyz.shape[0], tet)
+    val, r = oracle.assemble_poisson3d(xyz, tet, vol, invJ, rp, col, 1.0, 2, 3.0, 0)
+    assert np.array_equal(a.download()[2], val) and np.array_equal(rhs.download(), r)
+
+
+def test_spmv_bitexact(ctx, oracle):
+    from gpupfem2_amd import capi, cases
+    rng = np.random.default_rng(0)
+    for h in (1.0 / 16, 1.0 / 112):     # the second is the 2M-triangle cylinder pattern (n ~ 1.0M)
+        case = cases.cylinder_case(h=h)
+        print("spmv", case.name, flush=True)
+        m = capi.Mesh2D(ctx, case.xy, case.tri, fill_neighbors=False)
+        a = capi.CSR(ctx, mesh=m)
+        rp, col, _ = a.download()
+        val = rng.standard_normal(col.shape[0])
+        a.upload_values(val)
+        x = rng.standard_normal(rp.shape[0] - 1)
+        dx, dy = ctx.to_device(x), ctx.empty(x.size)
+        a.spmv(dx, dy)
+        assert np.array_equal(dy.download(), oracle.spmv(rp, col, val, x))
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_dirichlet(ctx, oracle, mode):
+    from gpupfem2_amd import capi
+    xy, tri, m, a, area, invJ, rp, col = laplace(ctx, oracle, n=24)
+    rhs = ctx.zeros(xy.shape[0])
+    capi.assemble_poisson2d(m, a, rhs, rhs_kind=2, rhs_param=1.0)
+    val, r = oracle.assemble_poisson2d(xy, tri, area, invJ, rp, col, rhs_kind=2, rhs_param=1.0)
+    bn = np.nonzero((xy[:, 0] < 1e-9) | (xy[:, 1] < 1e-9) | (xy[:, 0] > 1 - 1e-9))[0].astype(np.int32)
+    g = np.sin(3 * xy[bn, 0]) + xy[bn, 1]
+    bc = capi.Dirichlet(ctx, xy.shape[0], bn, g)
+    bc.apply(a, rhs, mode)
+    oracle.dirichlet_apply(rp, col, val, r, bn, g, mode)
+    assert np.array_equal(a.download()[2], val) and np.array_equal(rhs.download(), r)
+
+
+def test_dirichlet_two_rhs(ctx, oracle):
+    from gpupfem2_amd import capi
+    xy, tri, m, a, area, invJ, rp, col = laplace(ctx, oracle, n=16)
+    r0, r1 = ctx.zeros(xy.shape[0]), ctx.zeros(xy.shape[0])
+    capi.assemble_poisson2d(m, a, r0, rhs_kind=2, rhs_param=1.0)
+    r1.copy_from(r0)
+    val, r = oracle.assemble_poisson2d(xy, tri, area, invJ, rp, col, rhs_kind=2, rhs_param=1.0)
+    bn = np.nonzero(xy[:, 1] < 1e-9)[0].astype(np.int32)
+    g0, g1 = xy[bn, 0] + 1, -xy[bn, 0]
+    b0, b1 = capi.Dirichlet(ctx, xy.shape[0], bn, g0), capi.Dirichlet(ctx, xy.shape[0], bn, g1)
+    b0.apply2(b1, a, r0, r1, 1)
+    v0, q0 = val.copy(), r.copy()
+    oracle.dirichlet_apply(rp, col, v0, q0, bn, g0, 1)
+    v1, q1 = val.copy(), r.copy()
+    oracle.dirichlet_apply(rp, col, v1, q1, bn, g1, 1)
+    assert np.array_equal(a.download()[2], v0) and np.array_equal(v0, v1)
+    assert np.array_equal(r0.download(), q0) and np.array_equal(r1.download(), q1)
+
+
+def bc_system(ctx, oracle, n=40):
+    from gpupfem2_amd import capi
+    xy, tri, m, a, area, invJ, rp, col = laplace(ctx, oracle, n=n)
+    rhs = ctx.zeros(xy.shape[0])
+    capi.assemble_poisson2d(m, a, rhs, rhs_kind=1, rhs_param=2 * math.pi ** 2)
+    bn = np.nonzero((xy[:, 0] < 1e-9) | (xy[:, 0] > 1 - 1e-9) | (xy[:, 1] < 1e-9) | (xy[:, 1] > 1 - 1e-9))[0]
+    capi.Dirichlet(ctx, xy.shape[0], bn, np.zeros(bn.size)).apply(a, rhs, 1)
+    rp_, col_, val = a.download()
+    return xy, m, a, rhs, rp, col, val
+
+
+@pytest.mark.parametrize("kind", [1, 2, 3])
+def test_precond_factors(ctx, oracle, kind):
+    from gpupfem2_amd import capi
+    xy, m, a, rhs, rp, col, val = bc_system(ctx, oracle, n=24)
+    pc = capi.Preconditioner(a, kind, capi.ORDER_NATURAL)
+    pc.initialize(a)
+    fac, nl, perm = pc.download()
+    if kind == 1:
+        assert np.array_equal(fac, oracle.jacobi(rp, col, val))
+        return
+    ref = (oracle.ic0 if kind == 2 else oracle.ilu0)(rp, col, val)[1]
+    if kind == 2:
+        low = col <= np.repeat(np.arange(rp.size - 1), np.diff(rp))
+        assert np.array_equal(fac[low], ref[low])
+    else:
+        assert np.array_equal(fac, ref)
+    r = np.random.default_rng(2).standard_normal(rp.size - 1)
+    dr, dz = ctx.to_device(r), ctx.empty(r.size)
+    pc.apply(dz, dr)
+    z_ref = (oracle.ic0_apply if kind == 2 else oracle.ilu0_apply)(rp, col, ref, r)
+    assert np.array_equal(dz.download(), z_ref)
+
+
+@pytest.mark.parametrize("kind", [2, 3])
+def test_precond_multicolor(ctx, oracle, kind):
+    """The multicolour factor is the factor of P A P^T: same bits as the oracle on the permuted matrix."""
+    from gpupfem2_amd import capi
+    import scipy.sparse as sp
+    xy, m, a, rhs, rp, col, val = bc_system(ctx, oracle, n=24)
+    pc = capi.Preconditioner(a, kind, capi.ORDER_MULTICOLOR)
+    pc.initialize(a)
+    fac, nl, perm = pc.download()
+    assert nl <= 12
+    n = rp.size - 1
+    A = sp.csr_matrix((val, col, rp), shape=(n, n))
+    B = A[perm][:, perm].tocsr()
+    B.sort_indices()
+    rpb, colb, valb = B.indptr.astype(np.int32), B.indices.astype(np.int32), B.data.copy()
+    ref = (oracle.ic0 if kind == 2 else oracle.ilu0)(rpb, colb, valb)[1]
+    if kind == 2:
+        low = colb <= np.repeat(np.arange(n), np.diff(rpb))
+        assert np.array_equal(fac[low], ref[low])
+    else:
+        assert np.array_equal(fac, ref)
+    r = np.random.default_rng(3).standard_normal(n)
+    dr, dz = ctx.to_device(r), ctx.empty(n)
+    pc.apply(dz, dr)
+    zp = (oracle.ic0_apply if kind == 2 else oracle.ilu0_apply)(rpb, colb, ref, r[perm])
+    z = np.empty(n)
+    z[perm] = zp
+    assert np.array_equal(dz.download(), z)
+
+
+@pytest.mark.parametrize("precond,ordering", [(0, 0), (1, 0), (2, 0), (3, 0), (2, 1)])
+@pytest.mark.parametrize("tol_mode", [0, 1])
+def test_pcg(ctx, oracle, precond, ordering, tol_mode):
+    from gpupfem2_amd import capi
+    xy, m, a, rhs, rp, col, val = bc_system(ctx, oracle)
+    pc = capi.Preconditioner(a, precond, ordering) if precond else None
+    tol = 1e-9 if tol_mode == 0 else 1e-11
+    s = capi.Solver(a, capi.SOLVER_CG, tol, 3000, 0, tol_mode, pc)
+    x = ctx.zeros(xy.shape[0])
+    st = s.solve(a, x, rhs)
+    assert st.converged
+    b = rhs.download()
+    xr = np.zeros(b.size)
+    if ordering == 0:
+        ost = oracle.pcg(rp, col, val, xr, b, tol, 3000, tol_mode, ["none", "jacobi", "ic0", "ilu0"][precond])
+        assert ost.converged and abs(ost.iterations - st.iterations) <= 3
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    exact = spla.spsolve(sp.csr_matrix((val, col, rp)).tocsc(), b)
+    assert rel_l2(x.download(), exact) < 1e-7
+
+
+@pytest.mark.parametrize("restart", [30, 3000])
+def test_gmres(ctx, oracle, restart):
+    from gpupfem2_amd import capi
+    xy, m, a, rhs, rp, col, val = bc_system(ctx, oracle, n=24)
+    n = rp.size - 1
+    rng = np.random.default_rng(4)
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    v = val + np.where(col == rows, 1.0, 0.3 * rng.uniform(-1, 1, col.size) * np.abs(val))
+    a.upload_values(v)
+    b = rng.standard_normal(n)
+    db = ctx.to_device(b)
+    pc = capi.Preconditioner(a, capi.PRECOND_JACOBI)
+    s = capi.Solver(a, capi.SOLVER_GMRES, 1e-12, 3000, restart, capi.TOL_RELATIVE, pc)
+    x = ctx.zeros(n)
+    st = s.solve(a, x, db)
+    assert st.converged
+    xr = np.zeros(n)
+    ost = oracle.gmres(rp, col, v, xr, b, 1e-12, 3000, restart, 1, "jacobi")
+    assert ost.converged and abs(ost.iterations - st.iterations) <= 3
+    assert rel_l2(x.download(), xr) < 1e-10
+
+
+@pytest.mark.parametrize("kind,precond", [(0, 1), (0, 2), (1, 1)])
+def test_mixed_precision_refinement(ctx, oracle, kind, precond):
+    """FP32 inner Krylov + FP64 refinement reaches the FP64 tolerance."""
+    from gpupfem2_amd import capi
+    xy, m, a, rhs, rp, col, val = bc_system(ctx, oracle)
+    pc = capi.Preconditioner(a, precond, capi.ORDER_MULTICOLOR if precond == 2 else 0)
+    s = capi.Solver(a, kind, 1e-12, 5000, 30, capi.TOL_RELATIVE, pc)
+    s.set_precision(capi.PRECISION_MIXED, 1e-5, 20)
+    x = ctx.zeros(xy.shape[0])
+    st = s.solve(a, x, rhs)
+    assert st.converged and st.refinements >= 1
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    exact = spla.spsolve(sp.csr_matrix((val, col, rp)).tocsc(), rhs.download())
+    assert rel_l2(x.download(), exact) < 1e-9
+
+
+def test_solver_failure_paths(ctx, oracle):
+    """maxIterations reached -> converged = 0 (the reference returns false); GMRES without restart leaves
+    x untouched."""
+    from gpupfem2_amd import capi
+    xy, m, a, rhs, rp, col, val = bc_system(ctx, oracle, n=24)
+    x = ctx.zeros(xy.shape[0])
+    st = capi.Solver(a, capi.SOLVER_CG, 1e-30, 5, 0, 0, capi.Preconditioner(a)).solve(a, x, rhs)
+    assert not st.converged and st.iterations == 5
+    x.upload(np.full(xy.shape[0], 0.25))
+    st = capi.Solver(a, capi.SOLVER_GMRES, 1e-30, 6, 0, 0, capi.Preconditioner(a)).solve(a, x, rhs)
+    assert not st.converged and st.iterations == 6 and np.all(x.download() == 0.25)
