@@ -22,7 +22,8 @@ inline unsigned vec_grid(int n) { return std::min<unsigned>(blocks_for(n), 1024u
 __device__ __forceinline__ void cg_start_scalars(CgState* st, double rz, double bmb)
 {
     st->gamma = rz;
-    st->thr = st->tol_mode == PFEM_TOL_RELATIVE ? st->tol * st->tol * bmb : st->tol * st->tol;
+    // relative test against (b, M^-1 b); a zero right-hand side falls back to the absolute test
+    st->thr = st->tol_mode == PFEM_TOL_RELATIVE && bmb > 0.0 ? st->tol * st->tol * bmb : st->tol * st->tol;
     st->it = 0;
     st->nan = 0;
     st->converged = rz < st->thr ? 1 : 0;
@@ -326,7 +327,7 @@ __global__ void k_gm_orth(int n, const T* __restrict__ V, long ld, T* __restrict
 __global__ void k_gm_solve_y(GmState* st)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    st->apply_update = (st->converged || st->restarted) && st->k > 0 ? 1 : 0;
+    st->apply_update = (st->converged || st->restarted) && st->k > 0 && !st->nan ? 1 : 0;
     if (!st->apply_update) return;
     const int k = st->k, m = st->m;
     for (int i = 0; i < k; ++i) st->y[i] = st->beta[i];
@@ -387,8 +388,8 @@ __global__ __launch_bounds__(kBlock) void k_ir_residual(const int* __restrict__ 
     double tot[2];
     if (grid_sum<2>(v, partials, &st->counter, tot) && threadIdx.x == 0) {
         if (st->first) {
-            st->thr = GM ? (tol_mode == PFEM_TOL_RELATIVE ? tol * sqrt(tot[1]) : tol)
-                         : (tol_mode == PFEM_TOL_RELATIVE ? tol * tol * tot[1] : tol * tol);
+            const bool rel = tol_mode == PFEM_TOL_RELATIVE && tot[1] > 0.0;
+            st->thr = GM ? (rel ? tol * sqrt(tot[1]) : tol) : (rel ? tol * tol * tot[1] : tol * tol);
             st->first = 0;
         }
         st->rz = tot[0];
@@ -413,8 +414,8 @@ __global__ void k_ir_norms(int n, const double* __restrict__ r, const double* __
     double tot[2];
     if (grid_sum<2>(v, partials, &st->counter, tot) && threadIdx.x == 0) {
         if (st->first) {
-            st->thr = GM ? (tol_mode == PFEM_TOL_RELATIVE ? tol * sqrt(tot[1]) : tol)
-                         : (tol_mode == PFEM_TOL_RELATIVE ? tol * tol * tot[1] : tol * tol);
+            const bool rel = tol_mode == PFEM_TOL_RELATIVE && tot[1] > 0.0;
+            st->thr = GM ? (rel ? tol * sqrt(tot[1]) : tol) : (rel ? tol * tol * tot[1] : tol * tol);
             st->first = 0;
         }
         st->rz = tot[0];

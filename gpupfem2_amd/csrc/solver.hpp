@@ -34,7 +34,8 @@ struct GmState {
 // (linear_solver.cu:446-453; tolerance relative to ||M^-1 b|| when tol_mode is relative).
 __device__ inline void gm_cycle_start(GmState* st, double w2, double mb2)
 {
-    if (st->thr < 0.0) st->thr = st->tol_mode == PFEM_TOL_RELATIVE ? st->tol * sqrt(mb2) : st->tol;
+    // relative to ||M^-1 b||; a zero right-hand side falls back to the absolute test
+    if (st->thr < 0.0) st->thr = st->tol_mode == PFEM_TOL_RELATIVE && mb2 > 0.0 ? st->tol * sqrt(mb2) : st->tol;
     const double b0 = sqrt(w2);
     st->beta[0] = b0;
     st->res = b0;

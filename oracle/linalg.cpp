@@ -229,7 +229,7 @@ void orc_ilu0_apply(int n, const int* rowptr, const int* col, const double* lu, 
 
 // SolverCG::solve (src/linear_algebra/linear_solver.cu:301-376): warm start from x, preconditioner
 // re-initialised per solve, stop when (r, M^-1 r) < tol^2 (tol_mode 0, reference) or
-// (r, M^-1 r) < tol^2 (b, M^-1 b) (tol_mode 1). x holds the last iterate either way.
+// (r, M^-1 r) < tol^2 (b, M^-1 b) (tol_mode 1; absolute when b = 0). x holds the last iterate.
 int orc_pcg(int n, const int* rowptr, const int* col, const double* val, double* x, const double* b,
             double tol, int maxit, int tol_mode, int precond, orc_solve_stats* st)
 {
@@ -245,7 +245,8 @@ int orc_pcg(int n, const int* rowptr, const int* col, const double* val, double*
     if (tol_mode == 1) {
         std::vector<double> mb(n);
         M.apply(mb.data(), b);
-        thr *= dot(n, b, mb.data());
+        const double ref = dot(n, b, mb.data());
+        if (ref > 0.0) thr *= ref;   // a zero right-hand side falls back to the absolute test
     }
     st->converged = 0; st->iterations = 0; st->nan_detected = 0;
     if (gamma < thr) { st->converged = 1; st->final_residual = std::sqrt(gamma); return 1; }
@@ -286,7 +287,8 @@ int orc_gmres(int n, const int* rowptr, const int* col, const double* val, doubl
     double thr = tol;
     if (tol_mode == 1) {
         M.apply(w.data(), b);
-        thr = tol * std::sqrt(dot(n, w.data(), w.data()));
+        const double ref = std::sqrt(dot(n, w.data(), w.data()));
+        if (ref > 0.0) thr = tol * ref;   // a zero right-hand side falls back to the absolute test
     }
     st->converged = 0; st->iterations = 0; st->nan_detected = 0; st->final_residual = 0.0;
     int it = 0;

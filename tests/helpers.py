@@ -1,0 +1,36 @@
+"""Shared test helpers: build the oracle's side of a case from the same data the GPU gets."""
+import numpy as np
+
+
+def oracle_case(O, case):
+    area, invJ = O.geometry2d(case.xy, case.tri)
+    off, idx = O.neighbors_bruteforce(case.tri) if case.tri.shape[0] <= 60000 else (None, None)
+    return area, invJ, off, idx
+
+
+def oracle_params(O, case):
+    return O.make_params(case.rho, case.mu, case.dt, case.scheme, case.traction_mask, case.normals)
+
+
+def oracle_stepper(O, case, gpu_mesh=None):
+    """Oracle particles (seeded like the GPU) and stepper. Neighbour lists come from the GPU mesh for
+    meshes too large for the brute-force builder (their equality is tested separately)."""
+    area, invJ = O.geometry2d(case.xy, case.tri)
+    if gpu_mesh is not None:
+        off, idx = gpu_mesh.download(4), gpu_mesh.download(5)
+    else:
+        off, idx = O.neighbors_bruteforce(case.tri)
+    ph = O.Particles(case.xy, case.tri, invJ, off, idx, case.level)
+    ph.seed(case.seed_mode, case.ppc, case.seed)
+    z = np.zeros(case.xy.shape[0])
+    ph.init_velocity(z, z)
+    cfg = O.StepConfig()
+    cfg.prm = oracle_params(O, case)
+    for k in ("outer_iterations", "substeps", "pred_solver", "pred_precond", "pres_solver", "pres_precond",
+              "corr_solver", "corr_precond", "tol", "tol_mode", "maxit", "restart", "dirichlet_mode", "body_id",
+              "force_coeff"):
+        setattr(cfg, k, getattr(case, k))
+    cfg.body_cx, cfg.body_cy = case.body_center
+    st = O.Pfem2(case.xy, case.tri, case.edge_ids, cfg, ph, case.bcu_node, case.bcu_val0, case.bcu_val1,
+                 case.bcp_node, case.bcp_val)
+    return ph, st
