@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""PFEM-2 steps/s on one MI355X: flow past a cylinder, ~2.02M triangles, ~40M particles (scheme B,
+dt = 2e-4, two outer iterations, pressure PCG + IC0, prediction GMRES(30) + Jacobi, correction
+PCG + Jacobi, relative tolerance 1e-12), in the mixed-precision pipeline and in plain FP64.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+One step = advection + relocation + refill + reorder of all particles, projection, the two outer
+iterations of prediction / pressure / correction (5 assemblies + 5 solves each), particle velocity
+correction and the drag/lift integral. The flow starts from rest; W warm-up steps, then K timed
+steps bracketed by device synchronisation. Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def gpu_run(ctx, case, precision, steps, warmup, inner_tol):
+    from gpupfem2_amd import capi, cases
+    t0 = time.perf_counter()
+    mesh, ph, st = cases.build_gpu(ctx, case, precision=precision, inner_tol=inner_tol)
+    ctx.sync()
+    setup = time.perf_counter() - t0
+    for _ in range(warmup):
+        st.step()
+    ctx.sync()
+    its = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        forces = st.step()
+        its.append(st.iterations().tolist())
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    out = dict(steps_per_s=steps / dt, ms_per_step=1e3 * dt / steps, setup_s=setup, iterations=its,
+               particles=ph.count(), cd_cl=forces.tolist())
+    return out, (mesh, ph, st)
+
+
+def roofline_spmv(ctx, st, precision):
+    """Pressure-matrix SpMV: the kernel every Krylov iteration of the dominant solve runs."""
+    from gpupfem2_amd import capi
+    a = st.matrix_handle(2)
+    n, nnz = capi.C.c_int(), capi.C.c_int()
+    capi.lib().pfem_csr_info(a, capi.C.byref(n), capi.C.byref(nnz))
+    n, nnz = n.value, nnz.value
+    op = 1 if precision == 1 else 0
+    sv = 4 if op == 1 else 8
+    ms = capi.time_kernel(ctx, op, a, None, launches=200)
+    # algorithmic bytes: values + column indices + row offsets + x + y
+    bytes_ = nnz * (sv + 4) + 4 * (n + 1) + n * sv + n * sv
+    gbs = bytes_ / (ms * 1e-3) / 1e9
+    return dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None,
+                kernel=f"k_spmv<{'float' if op else 'double'}> pressure matrix n={n} nnz={nnz}",
+                avg_us=round(ms * 1e3, 2), bytes_per_launch=bytes_)
+
+
+def cpu_baseline(case, st, mesh, gpu_its):
+    """The oracle (1 thread, FP64) on the same full-size inputs: particle step, the 5 assemblies, and
+    per-iteration Krylov costs measured with capped iteration counts, combined with the iteration
+    counts of the GPU FP64 step into a time per step."""
+    import numpy as np
+    from oracle import oracle as O
+    T0 = time.perf_counter()
+    f = st.fields()
+    nv = case.xy.shape[0]
+    area, invJ = O.geometry2d(case.xy, case.tri)
+    off, idx = mesh.download(4), mesh.download(5)
+    prm = O.make_params(case.rho, case.mu, case.dt, case.scheme, case.traction_mask, case.normals)
+    rp, col = O.csr_pattern(nv, case.tri)
+    # particles: seed, one advect (3 substeps, relocation, refill, reorder), projection, correction
+    ph = O.Particles(case.xy, case.tri, invJ, off, idx, case.level)
+    ph.seed(case.seed_mode, case.ppc, case.seed)
+    ph.init_velocity(f["u0"], f["u1"])
+    t = time.perf_counter()
+    ph.advect(f["u0"], f["u1"], case.dt, case.substeps)
+    u0, u1 = ph.project(nv)
+    ph.correct(f["u0"], f["u1"], u0, u1)
+    t_part = time.perf_counter() - t
+    del ph
+    # assemblies (one of each system) + Dirichlet
+    t = time.perf_counter()
+    v0, v1, r0, r1 = O.assemble_prediction(case.xy, case.tri, area, invJ, case.edge_ids, rp, col, f["u0"], f["u1"],
+                                           f["u0"], f["u1"], f["p"], prm)
+    t_pred_asm = time.perf_counter() - t
+    t = time.perf_counter()
+    vp, rpv = O.assemble_pressure(case.tri, area, invJ, rp, col, f["vp0"], f["vp1"], f["p"], prm)
+    O.dirichlet_apply(rp, col, vp, rpv, case.bcp_node, case.bcp_val, case.dirichlet_mode)
+    t_pres_asm = time.perf_counter() - t
+    t = time.perf_counter()
+    vc, c0, c1 = O.assemble_correction(case.tri, area, invJ, rp, col, f["vp0"], f["vp1"], f["p"], f["p"], prm)
+    t_corr_asm = time.perf_counter() - t
+    O.dirichlet_apply(rp, col, v0, r0, case.bcu_node, case.bcu_val0, case.dirichlet_mode)
+    O.dirichlet_apply(rp, col, vc, c0, case.bcu_node, case.bcu_val0, case.dirichlet_mode)
+
+    def per_iter(fn, lo, hi):
+        ta = time.perf_counter(); fn(lo); tb = time.perf_counter(); fn(hi); tc = time.perf_counter()
+        it = ((tc - tb) - (tb - ta)) / (hi - lo)
+        return max(it, 1e-9), max((tb - ta) - lo * it, 0.0)
+
+    x = np.zeros(nv)
+    pres_it, pres_setup = per_iter(lambda k: O.pcg(rp, col, vp, x * 0, rpv, 1e-300, k, 1, "ic0"), 5, 25)
+    corr_it, corr_setup = per_iter(lambda k: O.pcg(rp, col, vc, x * 0, c0, 1e-300, k, 1, "jacobi"), 5, 25)
+    pred_it, pred_setup = per_iter(lambda k: O.gmres(rp, col, v0, x * 0, r0, 1e-300, k, 30, 1, "jacobi"), 5, 25)
+    its = np.asarray(gpu_its[-1]).reshape(case.outer_iterations, 5)
+    t_solve = (its[:, 0:2].sum() * pred_it + 2 * case.outer_iterations * pred_setup +
+               its[:, 2].sum() * pres_it + case.outer_iterations * pres_setup +
+               its[:, 3:5].sum() * corr_it + 2 * case.outer_iterations * corr_setup)
+    t_step = t_part + case.outer_iterations * (2 * t_pred_asm + t_pres_asm + t_corr_asm) + t_solve
+    return dict(value=round(1.0 / t_step, 6), unit="steps/s", cores=O.threads(), kind="port",
+                sample=(f"oracle (1 thread, FP64) on the full {case.tri.shape[0]}-triangle case: one particle "
+                        f"advect+project+correct ({t_part:.1f} s), one assembly of each system, and 20 "
+                        f"Krylov iterations per solver type (IC0-PCG {pres_it * 1e3:.1f} ms/it, Jacobi-PCG "
+                        f"{corr_it * 1e3:.1f} ms/it, GMRES(30) {pred_it * 1e3:.1f} ms/it), combined with the "
+                        f"GPU FP64 step's iteration counts; measured in {time.perf_counter() - T0:.0f} s"),
+                seconds_per_step=round(t_step, 2))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--h", type=float, default=1.0 / 112, help="background mesh spacing (D = 1)")
+    ap.add_argument("--mode", choices=["both", "fp64", "mixed"], default="both")
+    ap.add_argument("--inner-tol", type=float, default=1e-5)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        sys.exit("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)")
+    from gpupfem2_amd import capi, cases
+    case = cases.cylinder_case(h=args.h)
+    ctx = capi.Context(0)
+    res = {}
+    roof = None
+    cpu = None
+    for prec in ([0, 1] if args.mode == "both" else [0] if args.mode == "fp64" else [1]):
+        print(f"[bench] precision {'mixed' if prec else 'fp64'}: {case.tri.shape[0]} triangles, "
+              f"{args.warmup} warm-up + {args.steps} timed steps", file=sys.stderr, flush=True)
+        r, (mesh, ph, st) = gpu_run(ctx, case, prec, args.steps, args.warmup, args.inner_tol)
+        res[prec] = r
+        print(f"[bench] {r['steps_per_s']:.3f} steps/s, iterations {r['iterations'][-1]}", file=sys.stderr,
+              flush=True)
+        if prec == (1 if args.mode != "fp64" else 0):
+            roof = roofline_spmv(ctx, st, prec)
+        if prec == 0 and not args.no_cpu:
+            print("[bench] cpu baseline (oracle)", file=sys.stderr, flush=True)
+            cpu = cpu_baseline(case, st, mesh, r["iterations"])
+        del st, ph, mesh
+    head = res.get(1, res.get(0))
+    line = {
+        "metric": "PFEM steps/s mixed vs FP64 on 2M-tri cylinder mesh; solver HBM TB/s",
+        "value": round(head["steps_per_s"], 4),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64" if 1 in res else "f64",
+        "data": "synthetic (generated mesh, Philox-seeded particles, flow from rest)",
+        "config": {"workload": "cylinder Re=100, scheme B, dt=2e-4", "triangles": int(case.tri.shape[0]),
+                   "nodes": int(case.xy.shape[0]), "particles_per_cell": case.ppc,
+                   "particles_after_steps": head["particles"], "parallelism": "1 GPU"},
+        "fp64_steps_per_s": round(res[0]["steps_per_s"], 4) if 0 in res else None,
+        "mixed_steps_per_s": round(res[1]["steps_per_s"], 4) if 1 in res else None,
+        "mixed_over_fp64": round(res[1]["steps_per_s"] / res[0]["steps_per_s"], 3) if len(res) == 2 else None,
+        "krylov_iterations_last_step": {("mixed" if k else "fp64"): v["iterations"][-1] for k, v in res.items()},
+        "cd_cl_last_step": {("mixed" if k else "fp64"): v["cd_cl"] for k, v in res.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -133,6 +133,9 @@ SIGNATURES = {
     "pfem_pfem2_last_iterations": (_i, [_vp, _vp, _i]),
     "pfem_pfem2_enable_timing": (_i, [_vp, _i]),
     "pfem_pfem2_phase_times": (_i, [_vp, _vp, _i]),
+    "pfem_pfem2_matrix": (_i, [_vp, _i, _pp]),
+    "pfem_pfem2_precond": (_i, [_vp, _i, _pp]),
+    "pfem_time_kernel": (_i, [_i, _vp, _vp, _i, C.POINTER(_d)]),
     "pfem_export_vtk": (_i, [_vp, C.c_char_p, _i, _vp, _vp]),
 }
 
@@ -613,6 +616,25 @@ class Pfem2(_Handle):
         ms = np.zeros(8, np.float64)
         lib().pfem_pfem2_phase_times(self.h, _ptr(ms), 8)
         return ms
+
+    def matrix_handle(self, which):
+        """raw pfem_csr of the stepper: 0 prediction x, 1 prediction y, 2 pressure, 3 correction"""
+        p = C.c_void_p()
+        self.ctx.check(lib().pfem_pfem2_matrix(self.h, which, C.byref(p)))
+        return p.value
+
+    def precond_handle(self, which):
+        p = C.c_void_p()
+        self.ctx.check(lib().pfem_pfem2_precond(self.h, which, C.byref(p)))
+        return p.value
+
+
+def time_kernel(ctx, op, csr_handle, precond_handle=None, launches=50):
+    """average ms per launch (HIP events on the context stream): 0 SpMV FP64, 1 SpMV FP32,
+    2 preconditioner apply FP64, 3 preconditioner apply FP32"""
+    ms = C.c_double()
+    ctx.check(lib().pfem_time_kernel(op, csr_handle, precond_handle, launches, C.byref(ms)))
+    return ms.value
 
 
 def export_vtk(mesh, path, fields):

@@ -825,6 +825,50 @@ int pfem_solver_set_precision(pfem_solver s, int precision, double inner_tol, in
     });
 }
 
+int pfem_time_kernel(int op, pfem_csr a, pfem_precond pc, int launches, double* avg_ms)
+{
+    PFEM_TRY(a ? a->ctx : nullptr, {
+        PFEM_REQUIRE(a && avg_ms && launches > 0 && op >= 0 && op <= 3, "bad arguments");
+        PFEM_REQUIRE(op < 2 || pc, "ops 2 and 3 need a preconditioner");
+        Context* c = C(a->ctx);
+        const Pattern& P = *a->pat;
+        const int n = P.n;
+        DevArray<double> x(n), y(n);
+        DevArray<float> x32(n), y32(n);
+        launch_fill(c, x.p, 1.0, n);
+        hipLaunchKernelGGL(k_to32, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, (long)n, x32.p, x.p);
+        if (op == 1) {
+            if (!a->val32.p) a->val32.alloc(P.nnz);
+            hipLaunchKernelGGL(k_to32, dim3(blocks_for(P.nnz)), dim3(kBlock), 0, c->stream, (long)P.nnz, a->val32.p, a->val.p);
+        }
+        if (op >= 2) {
+            if (op == 3) precond_enable_fp32(c, pc);
+            precond_initialize(c, pc, a);
+        }
+        auto launch = [&]() {
+            switch (op) {
+            case 0: launch_spmv(c, P, a->val.p, x.p, y.p); break;
+            case 1: launch_spmv32(c, P, a->val32.p, x32.p, y32.p); break;
+            case 2: precond_apply(c, pc, y.p, x.p); break;
+            default: precond_apply32(c, pc, y32.p, x32.p); break;
+            }
+        };
+        launch();   // warm-up
+        hipEvent_t e0, e1;
+        PFEM_HIP(hipEventCreate(&e0));
+        PFEM_HIP(hipEventCreate(&e1));
+        PFEM_HIP(hipEventRecord(e0, c->stream));
+        for (int k = 0; k < launches; ++k) launch();
+        PFEM_HIP(hipEventRecord(e1, c->stream));
+        PFEM_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        PFEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *avg_ms = ms / launches;
+    });
+}
+
 int pfem_solver_destroy(pfem_solver s)
 {
     if (!s) return PFEM_OK;

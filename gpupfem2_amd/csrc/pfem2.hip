@@ -254,4 +254,20 @@ int pfem_pfem2_phase_times(pfem_pfem2 s, double* ms, int cap)
     return PFEM_OK;
 }
 
+int pfem_pfem2_matrix(pfem_pfem2 s, int which, pfem_csr* out)
+{
+    if (!s || !out || which < 0 || which > 3) return PFEM_EINVAL;
+    pfem_csr m[4] = {s->A0, s->A1, s->Ap, s->Ac};
+    *out = m[which];
+    return PFEM_OK;
+}
+
+int pfem_pfem2_precond(pfem_pfem2 s, int which, pfem_precond* out)
+{
+    if (!s || !out || which < 0 || which > 2) return PFEM_EINVAL;
+    pfem_precond p[3] = {s->pc_pred, s->pc_pres, s->pc_corr};
+    *out = p[which];
+    return PFEM_OK;
+}
+
 }  // extern "C"

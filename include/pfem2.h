@@ -290,6 +290,17 @@ int pfem_pfem2_last_iterations(pfem_pfem2 s, int* its, int cap);
  * 6 particle correction + loads. Timing is off unless enabled (it synchronises at phase ends). */
 int pfem_pfem2_enable_timing(pfem_pfem2 s, int on);
 int pfem_pfem2_phase_times(pfem_pfem2 s, double* ms, int cap);
+/* the stepper's matrices (borrowed, owned by s): 0 prediction x, 1 prediction y, 2 pressure,
+ * 3 correction; and its preconditioners: 0 prediction, 1 pressure, 2 correction */
+int pfem_pfem2_matrix(pfem_pfem2 s, int which, pfem_csr* out);
+int pfem_pfem2_precond(pfem_pfem2 s, int which, pfem_precond* out);
+
+/* ------------------------------------------------------------------ measurement */
+/* Average device time of one launch, measured with HIP events around `launches` back-to-back
+ * launches on the context stream (no counterpart in the reference, whose GpuTimer times phases,
+ * src/common/gpu_timer.cuh:14-98). op 0: FP64 SpMV of a; op 1: FP32 SpMV of a (values converted
+ * first); op 2: FP64 apply of pc (initialised from a first); op 3: the FP32 apply of pc. */
+int pfem_time_kernel(int op, pfem_csr a, pfem_precond pc, int launches, double* avg_ms);
 
 /* ------------------------------------------------------------------ export (src/data_export.cuh) */
 /* DataExport::exportToVTK (src/data_export.cu:17-95): ASCII VTU with nodal scalars (device arrays) */
