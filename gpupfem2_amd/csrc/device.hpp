@@ -46,81 +46,44 @@ __device__ __forceinline__ void block_sum(double (&v)[NV])
     __syncthreads();
 }
 
-// Deterministic grid-wide reduction of NV doubles: every block deposits its partial sums, the last
-// block to arrive adds all partials in block order and returns true (in all of its threads) with
-// the totals in tot[] (valid in thread 0). partials has NV * gridDim.x entries; counter must be zero
-// before the launch and is reset to zero by the last block.
+// Deterministic grid reductions carried across a kernel boundary: the producing kernel stores one
+// partial per block, the consuming kernel re-sums the partials in a fixed order (lane-strided, then
+// block_sum), so the total depends only on the producer's grid size. There is no inter-block
+// synchronisation inside a kernel: on gfx950 an agent-scope fence writes back and invalidates the L2 of
+// the issuing XCD, and a last-block-done reduction (one fence and one counter update per block) costs an
+// SpMV several times its own run time by evicting the gathered vector.
+//
+// part[i * nb + b] holds value i of block b (nb = producer's grid size).
 template <int NV>
-__device__ bool grid_sum(double (&v)[NV], double* partials, unsigned* counter, double (&tot)[NV])
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* __restrict__ part)
 {
-    __shared__ bool last;
     block_sum<NV>(v);
     if (threadIdx.x == 0) {
-        for (int i = 0; i < NV; ++i) partials[(size_t)i * gridDim.x + blockIdx.x] = v[i];
-        __threadfence();
-        const unsigned t = atomicAdd(counter, 1u);
-        last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    double s[NV];
 #pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        s[i] = 0.0;
-        for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x)
-            s[i] += __hip_atomic_load(&partials[(size_t)i * gridDim.x + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int i = 0; i < NV; ++i) part[(size_t)i * gridDim.x + blockIdx.x] = v[i];
     }
-    block_sum<NV>(s);
-    if (threadIdx.x == 0) {
-        for (int i = 0; i < NV; ++i) tot[i] = s[i];
-        *counter = 0u;
-    }
-    return true;
 }
 
-// Same for a runtime number of values (m <= 64), used by the GMRES multi-dot.
-__device__ inline bool grid_sum_dyn(double* v /* m per thread, in registers or LDS */, int m, double* partials,
-                                    unsigned* counter, double* tot /* m */, double* scratch /* LDS m*kWaves */)
+// Totals of the nb partials of each of the NV values; valid in every thread.
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double* __restrict__ part, int nb, double (&tot)[NV])
 {
-    __shared__ bool last;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int i = 0; i < m; ++i) {
-        const double w = wave_sum(v[i]);
-        if (lane == 0) scratch[i * kWaves + wid] = w;
+    __shared__ double res[NV];
+    double v[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = 0.0;
+    for (int k = threadIdx.x; k < nb; k += kBlock) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) v[i] += part[(size_t)i * nb + k];
     }
-    __syncthreads();
+    block_sum<NV>(v);
     if (threadIdx.x == 0) {
-        for (int i = 0; i < m; ++i) {
-            double s = scratch[i * kWaves];
-            for (int w = 1; w < kWaves; ++w) s += scratch[i * kWaves + w];
-            partials[(size_t)i * gridDim.x + blockIdx.x] = s;
-        }
-        __threadfence();
-        const unsigned t = atomicAdd(counter, 1u);
-        last = (t == gridDim.x - 1);
+#pragma unroll
+        for (int i = 0; i < NV; ++i) res[i] = v[i];
     }
     __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    for (int i = 0; i < m; ++i) {
-        double s = 0.0;
-        for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x)
-            s += __hip_atomic_load(&partials[(size_t)i * gridDim.x + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s = wave_sum(s);
-        if (lane == 0) scratch[i * kWaves + wid] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int i = 0; i < m; ++i) {
-            double s = scratch[i * kWaves];
-            for (int w = 1; w < kWaves; ++w) s += scratch[i * kWaves + w];
-            tot[i] = s;
-        }
-        *counter = 0u;
-    }
-    __syncthreads();
-    return true;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) tot[i] = res[i];
 }
 
 inline unsigned blocks_for(long n, int bs = kBlock) { return (unsigned)((n + bs - 1) / bs); }

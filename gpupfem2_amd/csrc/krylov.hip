@@ -2,8 +2,9 @@
 // (SolverGMRES::solve, :421-521), FP64 and FP32, plus FP64 iterative refinement around an FP32 inner
 // solve (the mixed-precision mode).
 //
-// Every scalar stays on the device. Dot products are deterministic grid reductions accumulated in FP64
-// (per-block partials, the last block adds them in block order). A device "done" flag replaces the
+// Every scalar stays on the device. Dot products are deterministic reductions accumulated in FP64: the
+// producing kernel stores per-block partials, the kernel that needs the total re-sums them in a fixed
+// order (no atomics, no inter-block waits). A device "done" flag replaces the
 // reference's per-iteration host read-back (linear_solver.cu:360, :499): the host polls it every few
 // iterations and kernels of iterations launched past convergence return at once.
 #include <algorithm>
@@ -17,45 +18,20 @@
 
 namespace pfem {
 
-inline unsigned vec_grid(int n) { return std::min<unsigned>(blocks_for(n), 1024u); }
-
-__device__ __forceinline__ void cg_start_scalars(CgState* st, double rz, double bmb)
-{
-    st->gamma = rz;
-    // relative test against (b, M^-1 b); a zero right-hand side falls back to the absolute test
-    st->thr = st->tol_mode == PFEM_TOL_RELATIVE && bmb > 0.0 ? st->tol * st->tol * bmb : st->tol * st->tol;
-    st->it = 0;
-    st->nan = 0;
-    st->converged = rz < st->thr ? 1 : 0;
-    st->done = st->converged;
-    st->res = sqrt(rz);
-}
-
-__device__ __forceinline__ void cg_finish_iteration(CgState* st, double gnew)
-{
-    st->it += 1;
-    st->res = sqrt(fabs(gnew));
-    if (gnew != gnew) {
-        st->nan = 1;
-        st->done = 1;
-    } else if (gnew < st->thr) {
-        st->converged = 1;
-        st->done = 1;
-    } else if (st->it >= st->maxit) {
-        st->done = 1;
-    }
-    st->beta = gnew / st->gamma;
-    st->gamma = gnew;
-}
+inline unsigned vec_grid(int n) { return std::min<unsigned>(blocks_for(n), (unsigned)kVecGrid); }
 
 // ------------------------------------------------------------------ PCG kernels
-// r = b - A x, z = invD r (Jacobi) or r, p = z; totals (r,z), (b, M^-1 b)
+// Iteration k (counted from the start of the solve, passed by the host) reads gamma_k = (r_k, z_k)
+// from gam[k & 1]; its last kernel stores gamma_{k+1} in the other slot, so no kernel reads a value
+// that another block of the same kernel writes.
+
+// r = b - A x, z = invD r (Jacobi) or r, p = z; partials (r,z), (b, M^-1 b)
 template <typename T, bool JAC>
 __global__ __launch_bounds__(kBlock) void k_cg_init(const int* __restrict__ tiles, const int* __restrict__ rowptr,
                                                     const int* __restrict__ col, const T* __restrict__ val,
                                                     const T* __restrict__ x, const T* __restrict__ b,
                                                     const T* __restrict__ invd, T* __restrict__ r, T* __restrict__ z,
-                                                    T* __restrict__ p, CgState* st, double* partials)
+                                                    T* __restrict__ p, double* __restrict__ part)
 {
     __shared__ T prod[kTileNnz];
     int row, r1;
@@ -71,14 +47,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(const int* __restrict__ tile
         v[0] = (double)(rr * zz);
         v[1] = JAC ? (double)(bb * (bb * invd[row])) : (double)(bb * bb);
     }
-    double tot[2];
-    if (grid_sum<2>(v, partials, &st->counter, tot) && threadIdx.x == 0) cg_start_scalars(st, tot[0], tot[1]);
+    block_partials<2>(v, part);
 }
 
 // x = 0 start (inner solves of the refinement): r = b, z = M^-1 b (Jacobi), p = z
 template <typename T, bool JAC>
 __global__ void k_cg_init_zero(int n, const T* __restrict__ b, const T* __restrict__ invd, T* __restrict__ x,
-                               T* __restrict__ r, T* __restrict__ z, T* __restrict__ p, CgState* st, double* partials)
+                               T* __restrict__ r, T* __restrict__ z, T* __restrict__ p, double* __restrict__ part)
 {
     double v[2] = {0.0, 0.0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -91,8 +66,7 @@ __global__ void k_cg_init_zero(int n, const T* __restrict__ b, const T* __restri
         v[0] += (double)(bb * zz);
     }
     v[1] = v[0];
-    double tot[2];
-    if (grid_sum<2>(v, partials, &st->counter, tot) && threadIdx.x == 0) cg_start_scalars(st, tot[0], tot[1]);
+    block_partials<2>(v, part);
 }
 
 template <typename T>
@@ -106,10 +80,10 @@ __global__ __launch_bounds__(kBlock) void k_residual(const int* __restrict__ til
     if (row < r1) r[row] = b[row] - ax;
 }
 
-// p = z; totals (r,z), (b,t), t = M^-1 b (t may be null)
+// p = z; partials (r,z), (b,t), t = M^-1 b (t may be null)
 template <typename T>
 __global__ void k_cg_start(int n, const T* __restrict__ r, const T* __restrict__ z, T* __restrict__ p,
-                           const T* __restrict__ b, const T* __restrict__ t, CgState* st, double* partials)
+                           const T* __restrict__ b, const T* __restrict__ t, double* __restrict__ part)
 {
     double v[2] = {0.0, 0.0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -117,16 +91,32 @@ __global__ void k_cg_start(int n, const T* __restrict__ r, const T* __restrict__
         v[0] += (double)(r[i] * z[i]);
         if (t) v[1] += (double)(b[i] * t[i]);
     }
-    double tot[2];
-    if (grid_sum<2>(v, partials, &st->counter, tot) && threadIdx.x == 0) cg_start_scalars(st, tot[0], tot[1]);
+    block_partials<2>(v, part);
 }
 
-// q = A p, alpha = gamma / (p, q)
+// gamma_0 = (r_0, z_0) and the stopping threshold: absolute (r, M^-1 r) < tol^2 (linear_solver.cu:360),
+// or relative to (b, M^-1 b); a zero right-hand side falls back to the absolute test. One block.
+__global__ void k_cg_begin(CgState* st, const double* __restrict__ part, int nb)
+{
+    double tot[2];
+    sum_partials<2>(part, nb, tot);
+    if (threadIdx.x != 0) return;
+    const double rz = tot[0], bmb = tot[1];
+    st->gam[0] = rz;
+    st->thr = st->tol_mode == PFEM_TOL_RELATIVE && bmb > 0.0 ? st->tol * st->tol * bmb : st->tol * st->tol;
+    st->it = 0;
+    st->nan = rz != rz;
+    st->converged = rz < st->thr ? 1 : 0;
+    st->done = st->converged | st->nan;
+    st->res = sqrt(fabs(rz));
+}
+
+// q = A p; partials (p, q)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv(const int* __restrict__ tiles, const int* __restrict__ rowptr,
                                                     const int* __restrict__ col, const T* __restrict__ val,
-                                                    const T* __restrict__ p, T* __restrict__ q, CgState* st,
-                                                    double* partials)
+                                                    const T* __restrict__ p, T* __restrict__ q, const CgState* st,
+                                                    double* __restrict__ part)
 {
     __shared__ T prod[kTileNnz];
     if (st->done) return;
@@ -137,21 +127,25 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(const int* __restrict__ tile
         q[row] = ap;
         v[0] = (double)(p[row] * ap);
     }
-    double tot[1];
-    if (grid_sum<1>(v, partials, &st->counter, tot) && threadIdx.x == 0) {
-        st->pAp = tot[0];
-        st->alpha = st->gamma / tot[0];
-    }
+    block_partials<1>(v, part);
 }
 
-// x += alpha p, r -= alpha q (updateXR, linear_solver.cu:68-77); z = invD r; gamma_new = (r, z)
+// alpha_k = gamma_k / (p, q) from the SpMV partials
+__device__ __forceinline__ double cg_alpha(const CgState* st, const double* pa, int na, int k)
+{
+    double pq[1];
+    sum_partials<1>(pa, na, pq);
+    return st->gam[k & 1] / pq[0];
+}
+
+// x += alpha p, r -= alpha q (updateXR, linear_solver.cu:68-77); z = invD r; partials (r, z)
 template <typename T, bool JAC>
 __global__ void k_cg_update(int n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
-                            const T* __restrict__ q, T* __restrict__ z, const T* __restrict__ invd, CgState* st,
-                            double* partials)
+                            const T* __restrict__ q, T* __restrict__ z, const T* __restrict__ invd, const CgState* st,
+                            const double* __restrict__ pa, int na, double* __restrict__ pb, int k)
 {
     if (st->done) return;
-    const T alpha = (T)st->alpha;
+    const T alpha = (T)cg_alpha(st, pa, na, k);
     double v[1] = {0.0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         x[i] += alpha * p[i];
@@ -161,16 +155,15 @@ __global__ void k_cg_update(int n, T* __restrict__ x, T* __restrict__ r, const T
         if (JAC) z[i] = zz;
         v[0] += (double)(rr * zz);
     }
-    double tot[1];
-    if (grid_sum<1>(v, partials, &st->counter, tot) && threadIdx.x == 0) cg_finish_iteration(st, tot[0]);
+    block_partials<1>(v, pb);
 }
 
 template <typename T>
 __global__ void k_cg_update_xr(int n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
-                               const T* __restrict__ q, const CgState* st)
+                               const T* __restrict__ q, const CgState* st, const double* __restrict__ pa, int na, int k)
 {
     if (st->done) return;
-    const T alpha = (T)st->alpha;
+    const T alpha = (T)cg_alpha(st, pa, na, k);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         x[i] += alpha * p[i];
         r[i] = r[i] - alpha * q[i];
@@ -178,23 +171,42 @@ __global__ void k_cg_update_xr(int n, T* __restrict__ x, T* __restrict__ r, cons
 }
 
 template <typename T>
-__global__ void k_cg_rz(int n, const T* __restrict__ r, const T* __restrict__ z, CgState* st, double* partials)
+__global__ void k_cg_rz(int n, const T* __restrict__ r, const T* __restrict__ z, const CgState* st,
+                        double* __restrict__ pb)
 {
     if (st->done) return;
     double v[1] = {0.0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         v[0] += (double)(r[i] * z[i]);
-    double tot[1];
-    if (grid_sum<1>(v, partials, &st->counter, tot) && threadIdx.x == 0) cg_finish_iteration(st, tot[0]);
+    block_partials<1>(v, pb);
 }
 
-// p = z + beta p (updateP, linear_solver.cu:79-87)
+// gamma_{k+1} from the (r, z) partials; p = z + beta p (updateP, linear_solver.cu:79-87); block 0
+// records the iteration and the stopping test
 template <typename T>
-__global__ void k_cg_p(int n, T* __restrict__ p, const T* __restrict__ z, const CgState* st)
+__global__ void k_cg_p(int n, T* __restrict__ p, const T* __restrict__ z, CgState* st,
+                       const double* __restrict__ pb, int nb, int k)
 {
     if (st->done) return;
-    const T beta = (T)st->beta;
+    double g[1];
+    sum_partials<1>(pb, nb, g);
+    const double gnew = g[0];
+    const T beta = (T)(gnew / st->gam[k & 1]);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = z[i] + beta * p[i];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->gam[(k + 1) & 1] = gnew;
+        st->it = k + 1;
+        st->res = sqrt(fabs(gnew));
+        if (gnew != gnew) {
+            st->nan = 1;
+            st->done = 1;
+        } else if (gnew < st->thr) {
+            st->converged = 1;
+            st->done = 1;
+        } else if (k + 1 >= st->maxit) {
+            st->done = 1;
+        }
+    }
 }
 
 // ------------------------------------------------------------------ GMRES kernels
@@ -202,8 +214,8 @@ template <typename T, bool JAC>
 __global__ __launch_bounds__(kBlock) void k_gm_residual(const int* __restrict__ tiles, const int* __restrict__ rowptr,
                                                         const int* __restrict__ col, const T* __restrict__ val,
                                                         const T* __restrict__ x, const T* __restrict__ b,
-                                                        const T* __restrict__ invd, T* __restrict__ w, GmState* st,
-                                                        double* partials)
+                                                        const T* __restrict__ invd, T* __restrict__ w,
+                                                        double* __restrict__ part)
 {
     __shared__ T prod[kTileNnz];
     int row, r1;
@@ -218,14 +230,13 @@ __global__ __launch_bounds__(kBlock) void k_gm_residual(const int* __restrict__ 
         v[0] = (double)(ww * ww);
         v[1] = (double)(mb * mb);
     }
-    double tot[2];
-    if (grid_sum<2>(v, partials, &st->counter, tot) && threadIdx.x == 0) gm_cycle_start(st, tot[0], tot[1]);
+    block_partials<2>(v, part);
 }
 
 // x = 0 start: w = M^-1 b (Jacobi or none)
 template <typename T, bool JAC>
 __global__ void k_gm_init_zero(int n, const T* __restrict__ b, const T* __restrict__ invd, T* __restrict__ x,
-                               T* __restrict__ w, GmState* st, double* partials)
+                               T* __restrict__ w, double* __restrict__ part)
 {
     double v[2] = {0.0, 0.0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -236,20 +247,26 @@ __global__ void k_gm_init_zero(int n, const T* __restrict__ b, const T* __restri
         v[0] += (double)(ww * ww);
     }
     v[1] = v[0];
-    double tot[2];
-    if (grid_sum<2>(v, partials, &st->counter, tot) && threadIdx.x == 0) gm_cycle_start(st, tot[0], tot[1]);
+    block_partials<2>(v, part);
 }
 
 template <typename T>
-__global__ void k_gm_norms(int n, const T* __restrict__ w, const T* __restrict__ t, GmState* st, double* partials)
+__global__ void k_gm_norms(int n, const T* __restrict__ w, const T* __restrict__ t, double* __restrict__ part)
 {
     double v[2] = {0.0, 0.0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         v[0] += (double)(w[i] * w[i]);
         if (t) v[1] += (double)(t[i] * t[i]);
     }
+    block_partials<2>(v, part);
+}
+
+// start of a cycle from the ||w||^2, ||M^-1 b||^2 partials (one block)
+__global__ void k_gm_begin(GmState* st, const double* __restrict__ part, int nb)
+{
     double tot[2];
-    if (grid_sum<2>(v, partials, &st->counter, tot) && threadIdx.x == 0) gm_cycle_start(st, tot[0], tot[1]);
+    sum_partials<2>(part, nb, tot);
+    if (threadIdx.x == 0) gm_cycle_start(st, tot[0], tot[1]);
 }
 
 // v /= ||v|| (scaleVector with divide = true, linear_solver.cu:16-26, :456, :493)
@@ -275,12 +292,12 @@ __global__ __launch_bounds__(kBlock) void k_gm_arnoldi_spmv(const int* __restric
     if (row < r1) w[row] = JAC ? av * invd[row] : av;
 }
 
-// h[i0 + t] = (V_{i0+t}, w), t < cnt <= 8; pass 0 stores the coefficients, pass 1 adds to them
+// partials of (V_{i0+t}, w), t < cnt <= 8, stored as coefficient i0+t of part
 template <typename T>
 __global__ void k_gm_multidot(int n, const T* __restrict__ V, long ld, int i0, int cnt, const T* __restrict__ w,
-                              GmState* st, double* partials, int pass)
+                              const GmState* st, double* __restrict__ part)
 {
-    __shared__ double scratch[8 * kWaves];
+    __shared__ double sh[8][kWaves];
     if (st->cycle_done) return;
     double v[8];
 #pragma unroll
@@ -291,20 +308,41 @@ __global__ void k_gm_multidot(int n, const T* __restrict__ V, long ld, int i0, i
         for (int t = 0; t < 8; ++t)
             if (t < cnt) v[t] += (double)(V[(size_t)(i0 + t) * ld + i] * wi);
     }
-    double tot[8];
-    if (grid_sum_dyn(v, cnt, partials, &st->counter, tot, scratch) && threadIdx.x == 0) {
-        double* h = st->H + (size_t)st->k * st->m;
-        for (int t = 0; t < cnt; ++t) {
-            st->hc[i0 + t] = tot[t];
-            h[i0 + t] = pass == 0 ? tot[t] : h[i0 + t] + tot[t];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        if (t < cnt) {
+            const double s = wave_sum(v[t]);
+            if (lane == 0) sh[t][wid] = s;
         }
+    }
+    __syncthreads();
+    if (threadIdx.x < cnt) {
+        const int t = threadIdx.x;
+        double s = sh[t][0];
+        for (int w2 = 1; w2 < kWaves; ++w2) s += sh[t][w2];
+        part[(size_t)(i0 + t) * gridDim.x + blockIdx.x] = s;
     }
 }
 
-// w -= sum_i hc_i V_i (i <= j); on the last pass ||w|| finishes Arnoldi step j (Givens update)
+// block i sums the partials of coefficient i: hc_i, and h_{i,j} stored (pass 0) or corrected (pass 1)
+__global__ void k_gm_reduce_h(GmState* st, const double* __restrict__ part, int nb, int pass)
+{
+    if (st->cycle_done) return;
+    double tot[1];
+    sum_partials<1>(part + (size_t)blockIdx.x * nb, nb, tot);
+    if (threadIdx.x == 0) {
+        const int i = blockIdx.x;
+        double* h = st->H + (size_t)st->k * st->m;
+        st->hc[i] = tot[0];
+        h[i] = pass == 0 ? tot[0] : h[i] + tot[0];
+    }
+}
+
+// w -= sum_i hc_i V_i (i <= j); on the last pass also the partials of ||w||^2
 template <typename T>
-__global__ void k_gm_orth(int n, const T* __restrict__ V, long ld, T* __restrict__ w, GmState* st, double* partials,
-                          int last_pass)
+__global__ void k_gm_orth(int n, const T* __restrict__ V, long ld, T* __restrict__ w, const GmState* st,
+                          double* __restrict__ pn, int last_pass)
 {
     __shared__ double hc[kMaxRestart + 1];
     if (st->cycle_done) return;
@@ -318,9 +356,16 @@ __global__ void k_gm_orth(int n, const T* __restrict__ V, long ld, T* __restrict
         w[i] = wi;
         v[0] += (double)(wi * wi);
     }
-    if (!last_pass) return;
+    if (last_pass) block_partials<1>(v, pn);
+}
+
+// end of Arnoldi step j from the ||w||^2 partials (one block)
+__global__ void k_gm_finish(GmState* st, const double* __restrict__ pn, int nb)
+{
+    if (st->cycle_done) return;
     double tot[1];
-    if (grid_sum<1>(v, partials, &st->counter, tot) && threadIdx.x == 0) gm_arnoldi_finish(st, sqrt(tot[0]));
+    sum_partials<1>(pn, nb, tot);
+    if (threadIdx.x == 0) gm_arnoldi_finish(st, sqrt(tot[0]));
 }
 
 // y = H^-1 beta (cublasDtrsv, upper, non-unit; linear_solver.cu:508-511)
@@ -338,7 +383,7 @@ __global__ void k_gm_solve_y(GmState* st)
     }
 }
 
-// x += V y (cublasDgemv with alpha = beta = 1; linear_solver.cu:513-514)
+// x += V y (cublasDgemv with alpha = beta = 1; linear_solver.cu:513-514); clears cycle_done for a restart
 template <typename T>
 __global__ void k_gm_update_x(int n, const T* __restrict__ V, long ld, T* __restrict__ x, const GmState* st)
 {
@@ -354,22 +399,25 @@ __global__ void k_gm_update_x(int n, const T* __restrict__ V, long ld, T* __rest
     }
 }
 
+__global__ void k_gm_rearm(GmState* st)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) st->cycle_done = 0;
+}
+
 // ------------------------------------------------------------------ refinement kernels
 // FP64 refinement state
 struct IrState {
     double thr, rz, scale;
     int first, converged, nan;
-    unsigned counter;
 };
 
-// r = b - A x (FP64), z = invD r (Jacobi) / r; totals (r,z) and (b, M^-1 b); gmres: ||z||^2, ||M^-1 b||^2
+// r = b - A x (FP64), z = invD r (Jacobi) / r; partials (r,z) and (b, M^-1 b); gmres: ||z||^2, ||M^-1 b||^2
 template <bool JAC, bool GM>
 __global__ __launch_bounds__(kBlock) void k_ir_residual(const int* __restrict__ tiles, const int* __restrict__ rowptr,
                                                         const int* __restrict__ col, const double* __restrict__ val,
                                                         const double* __restrict__ x, const double* __restrict__ b,
                                                         const double* __restrict__ invd, double* __restrict__ r,
-                                                        double* __restrict__ z, IrState* st, double tol, int tol_mode,
-                                                        double* partials)
+                                                        double* __restrict__ z, double* __restrict__ part)
 {
     __shared__ double prod[kTileNnz];
     int row, r1;
@@ -385,45 +433,39 @@ __global__ __launch_bounds__(kBlock) void k_ir_residual(const int* __restrict__ 
         v[0] = GM ? zz * zz : rr * zz;
         v[1] = GM ? mb * mb : bb * mb;
     }
-    double tot[2];
-    if (grid_sum<2>(v, partials, &st->counter, tot) && threadIdx.x == 0) {
-        if (st->first) {
-            const bool rel = tol_mode == PFEM_TOL_RELATIVE && tot[1] > 0.0;
-            st->thr = GM ? (rel ? tol * sqrt(tot[1]) : tol) : (rel ? tol * tol * tot[1] : tol * tol);
-            st->first = 0;
-        }
-        st->rz = tot[0];
-        const double meas = GM ? sqrt(tot[0]) : tot[0];
-        st->nan = tot[0] != tot[0];
-        st->converged = meas < st->thr ? 1 : 0;
-        st->scale = sqrt(tot[0]) > 0.0 ? sqrt(tot[0]) : 1.0;
-    }
+    block_partials<2>(v, part);
 }
 
-// general preconditioner: z = M^-1 r already applied; totals as above (t = M^-1 b, may be null)
+// general preconditioner: z = M^-1 r already applied; partials as above (t = M^-1 b, may be null)
 template <bool GM>
 __global__ void k_ir_norms(int n, const double* __restrict__ r, const double* __restrict__ z,
-                           const double* __restrict__ b, const double* __restrict__ t, IrState* st, double tol,
-                           int tol_mode, double* partials)
+                           const double* __restrict__ b, const double* __restrict__ t, double* __restrict__ part)
 {
     double v[2] = {0.0, 0.0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         v[0] += GM ? z[i] * z[i] : r[i] * z[i];
         if (t) v[1] += GM ? t[i] * t[i] : b[i] * t[i];
     }
+    block_partials<2>(v, part);
+}
+
+// refinement stopping test (the FP64 solver's criterion) and the scale of the next correction; one block
+template <bool GM>
+__global__ void k_ir_test(IrState* st, const double* __restrict__ part, int nb, double tol, int tol_mode)
+{
     double tot[2];
-    if (grid_sum<2>(v, partials, &st->counter, tot) && threadIdx.x == 0) {
-        if (st->first) {
-            const bool rel = tol_mode == PFEM_TOL_RELATIVE && tot[1] > 0.0;
-            st->thr = GM ? (rel ? tol * sqrt(tot[1]) : tol) : (rel ? tol * tol * tot[1] : tol * tol);
-            st->first = 0;
-        }
-        st->rz = tot[0];
-        const double meas = GM ? sqrt(tot[0]) : tot[0];
-        st->nan = tot[0] != tot[0];
-        st->converged = meas < st->thr ? 1 : 0;
-        st->scale = sqrt(tot[0]) > 0.0 ? sqrt(tot[0]) : 1.0;
+    sum_partials<2>(part, nb, tot);
+    if (threadIdx.x != 0) return;
+    if (st->first) {
+        const bool rel = tol_mode == PFEM_TOL_RELATIVE && tot[1] > 0.0;
+        st->thr = GM ? (rel ? tol * sqrt(tot[1]) : tol) : (rel ? tol * tol * tot[1] : tol * tol);
+        st->first = 0;
     }
+    st->rz = tot[0];
+    const double meas = GM ? sqrt(tot[0]) : tot[0];
+    st->nan = tot[0] != tot[0];
+    st->converged = meas < st->thr ? 1 : 0;
+    st->scale = sqrt(tot[0]) > 0.0 ? sqrt(tot[0]) : 1.0;
 }
 
 __global__ void k_ir_down(int n, float* __restrict__ d, const double* __restrict__ s, const IrState* st)
@@ -474,19 +516,20 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
     const int n = P.n;
     const unsigned g = vec_grid(n);
     const bool jac = A.kind == PFEM_PRECOND_JACOBI, none = A.kind == PFEM_PRECOND_NONE;
-    double* part = s->partials.p;
     CgState init{};
     init.tol = A.tol;
     init.tol_mode = A.tol_mode;
     init.maxit = A.maxit;
-    PFEM_HIP(hipMemcpyAsync(st, &init, offsetof(CgState, counter), hipMemcpyHostToDevice, c->stream));
+    PFEM_HIP(hipMemcpyAsync(st, &init, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
+    int nb0 = (int)g;   // grid of the kernel that produced the start partials
     if (jac || none) {
         if (A.x_zero) {
-            if (jac) hipLaunchKernelGGL((k_cg_init_zero<T, true>), dim3(g), dim3(kBlock), 0, c->stream, n, A.b, A.invd, A.x, A.r, A.z, A.p, st, part);
-            else hipLaunchKernelGGL((k_cg_init_zero<T, false>), dim3(g), dim3(kBlock), 0, c->stream, n, A.b, A.invd, A.x, A.r, A.z, A.p, st, part);
+            if (jac) hipLaunchKernelGGL((k_cg_init_zero<T, true>), dim3(g), dim3(kBlock), 0, c->stream, n, A.b, A.invd, A.x, A.r, A.z, A.p, s->pa);
+            else hipLaunchKernelGGL((k_cg_init_zero<T, false>), dim3(g), dim3(kBlock), 0, c->stream, n, A.b, A.invd, A.x, A.r, A.z, A.p, s->pa);
         } else {
-            if (jac) hipLaunchKernelGGL((k_cg_init<T, true>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.x, A.b, A.invd, A.r, A.z, A.p, st, part);
-            else hipLaunchKernelGGL((k_cg_init<T, false>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.x, A.b, A.invd, A.r, A.z, A.p, st, part);
+            nb0 = P.ntiles;
+            if (jac) hipLaunchKernelGGL((k_cg_init<T, true>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.x, A.b, A.invd, A.r, A.z, A.p, s->pa);
+            else hipLaunchKernelGGL((k_cg_init<T, false>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.x, A.b, A.invd, A.r, A.z, A.p, s->pa);
         }
     } else {
         if (A.x_zero) {
@@ -501,30 +544,33 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
             if (A.x_zero) t = A.z;
             else { papply<T>(c, A.pc, A.t, A.b); t = A.t; }
         }
-        hipLaunchKernelGGL((k_cg_start<T>), dim3(g), dim3(kBlock), 0, c->stream, n, A.r, A.z, A.p, A.b, t, st, part);
+        hipLaunchKernelGGL((k_cg_start<T>), dim3(g), dim3(kBlock), 0, c->stream, n, A.r, A.z, A.p, A.b, t, s->pa);
     }
+    hipLaunchKernelGGL(k_cg_begin, dim3(1), dim3(kBlock), 0, c->stream, st, (const double*)s->pa, nb0);
     PFEM_HIP(hipGetLastError());
     const int batch = s->poll;
-    long launched = 0;
+    const int na = P.ntiles, nb = (int)g;
+    int launched = 0;
     while (true) {
-        for (int k = 0; k < batch; ++k) {
-            hipLaunchKernelGGL((k_cg_spmv<T>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.p, A.q, st, part);
+        for (int j = 0; j < batch; ++j) {
+            const int k = launched + j;
+            hipLaunchKernelGGL((k_cg_spmv<T>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.p, A.q, st, s->pa);
             if (jac) {
-                hipLaunchKernelGGL((k_cg_update<T, true>), dim3(g), dim3(kBlock), 0, c->stream, n, A.x, A.r, A.p, A.q, A.z, A.invd, st, part);
+                hipLaunchKernelGGL((k_cg_update<T, true>), dim3(g), dim3(kBlock), 0, c->stream, n, A.x, A.r, A.p, A.q, A.z, A.invd, st, s->pa, na, s->pb, k);
             } else if (none) {
-                hipLaunchKernelGGL((k_cg_update<T, false>), dim3(g), dim3(kBlock), 0, c->stream, n, A.x, A.r, A.p, A.q, A.r, A.invd, st, part);
+                hipLaunchKernelGGL((k_cg_update<T, false>), dim3(g), dim3(kBlock), 0, c->stream, n, A.x, A.r, A.p, A.q, A.r, A.invd, st, s->pa, na, s->pb, k);
             } else {
-                hipLaunchKernelGGL((k_cg_update_xr<T>), dim3(g), dim3(kBlock), 0, c->stream, n, A.x, A.r, A.p, A.q, st);
+                hipLaunchKernelGGL((k_cg_update_xr<T>), dim3(g), dim3(kBlock), 0, c->stream, n, A.x, A.r, A.p, A.q, st, s->pa, na, k);
                 papply<T>(c, A.pc, A.z, A.r);
-                hipLaunchKernelGGL((k_cg_rz<T>), dim3(g), dim3(kBlock), 0, c->stream, n, A.r, A.z, st, part);
+                hipLaunchKernelGGL((k_cg_rz<T>), dim3(g), dim3(kBlock), 0, c->stream, n, A.r, A.z, st, s->pb);
             }
-            hipLaunchKernelGGL((k_cg_p<T>), dim3(g), dim3(kBlock), 0, c->stream, n, A.p, none ? A.r : A.z, st);
+            hipLaunchKernelGGL((k_cg_p<T>), dim3(g), dim3(kBlock), 0, c->stream, n, A.p, none ? A.r : A.z, st, s->pb, nb, k);
         }
         launched += batch;
         PFEM_HIP(hipGetLastError());
         PFEM_HIP(hipMemcpyAsync(hst, st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
         PFEM_HIP(hipStreamSynchronize(c->stream));
-        if (hst->done || launched > (long)A.maxit + batch) break;
+        if (hst->done || launched > A.maxit + batch) break;
     }
 }
 
@@ -538,7 +584,6 @@ static void gmres_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, GmState* 
     const int m = s->m;
     const long ld = n;
     T* V = A.V;
-    double* part = s->partials.p;
     GmState init{};
     init.tol = A.tol;
     init.tol_mode = A.tol_mode;
@@ -552,16 +597,18 @@ static void gmres_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, GmState* 
     init.sn = s->sn.p;
     init.y = s->y.p;
     init.hc = s->hc.p;
-    PFEM_HIP(hipMemcpyAsync(st, &init, offsetof(GmState, counter), hipMemcpyHostToDevice, c->stream));
+    PFEM_HIP(hipMemcpyAsync(st, &init, sizeof(GmState), hipMemcpyHostToDevice, c->stream));
     bool first = true;
     while (true) {
+        int nb0 = (int)g;
         if (jac || none) {
             if (first && A.x_zero) {
-                if (jac) hipLaunchKernelGGL((k_gm_init_zero<T, true>), dim3(g), dim3(kBlock), 0, c->stream, n, A.b, A.invd, A.x, V, st, part);
-                else hipLaunchKernelGGL((k_gm_init_zero<T, false>), dim3(g), dim3(kBlock), 0, c->stream, n, A.b, A.invd, A.x, V, st, part);
+                if (jac) hipLaunchKernelGGL((k_gm_init_zero<T, true>), dim3(g), dim3(kBlock), 0, c->stream, n, A.b, A.invd, A.x, V, s->pa);
+                else hipLaunchKernelGGL((k_gm_init_zero<T, false>), dim3(g), dim3(kBlock), 0, c->stream, n, A.b, A.invd, A.x, V, s->pa);
             } else {
-                if (jac) hipLaunchKernelGGL((k_gm_residual<T, true>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.x, A.b, A.invd, V, st, part);
-                else hipLaunchKernelGGL((k_gm_residual<T, false>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.x, A.b, A.invd, V, st, part);
+                nb0 = P.ntiles;
+                if (jac) hipLaunchKernelGGL((k_gm_residual<T, true>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.x, A.b, A.invd, V, s->pa);
+                else hipLaunchKernelGGL((k_gm_residual<T, false>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.x, A.b, A.invd, V, s->pa);
             }
         } else {
             if (first && A.x_zero) {
@@ -576,8 +623,9 @@ static void gmres_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, GmState* 
                 if (A.x_zero) t = V;
                 else { papply<T>(c, A.pc, A.t, A.b); t = A.t; }
             }
-            hipLaunchKernelGGL((k_gm_norms<T>), dim3(g), dim3(kBlock), 0, c->stream, n, V, t, st, part);
+            hipLaunchKernelGGL((k_gm_norms<T>), dim3(g), dim3(kBlock), 0, c->stream, n, V, t, s->pa);
         }
+        hipLaunchKernelGGL(k_gm_begin, dim3(1), dim3(kBlock), 0, c->stream, st, (const double*)s->pa, nb0);
         first = false;
         hipLaunchKernelGGL((k_gm_scale<T>), dim3(g), dim3(kBlock), 0, c->stream, n, V, st);
         for (int j0 = 0; j0 < m; j0 += s->poll) {
@@ -591,9 +639,11 @@ static void gmres_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, GmState* 
                 const int passes = s->reorth ? 2 : 1;
                 for (int pass = 0; pass < passes; ++pass) {
                     for (int i0 = 0; i0 <= j; i0 += 8)
-                        hipLaunchKernelGGL((k_gm_multidot<T>), dim3(g), dim3(kBlock), 0, c->stream, n, V, ld, i0, std::min(8, j + 1 - i0), w, st, part, pass);
-                    hipLaunchKernelGGL((k_gm_orth<T>), dim3(g), dim3(kBlock), 0, c->stream, n, V, ld, w, st, part, pass == passes - 1 ? 1 : 0);
+                        hipLaunchKernelGGL((k_gm_multidot<T>), dim3(g), dim3(kBlock), 0, c->stream, n, V, ld, i0, std::min(8, j + 1 - i0), w, st, s->pb);
+                    hipLaunchKernelGGL(k_gm_reduce_h, dim3(j + 1), dim3(kBlock), 0, c->stream, st, (const double*)s->pb, (int)g, pass);
+                    hipLaunchKernelGGL((k_gm_orth<T>), dim3(g), dim3(kBlock), 0, c->stream, n, V, ld, w, st, s->pn, pass == passes - 1 ? 1 : 0);
                 }
+                hipLaunchKernelGGL(k_gm_finish, dim3(1), dim3(kBlock), 0, c->stream, st, (const double*)s->pn, (int)g);
                 hipLaunchKernelGGL((k_gm_scale<T>), dim3(g), dim3(kBlock), 0, c->stream, n, w, st);
             }
             PFEM_HIP(hipGetLastError());
@@ -607,7 +657,7 @@ static void gmres_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, GmState* 
         PFEM_HIP(hipMemcpyAsync(hst, st, sizeof(GmState), hipMemcpyDeviceToHost, c->stream));
         PFEM_HIP(hipStreamSynchronize(c->stream));
         if (hst->converged || hst->nan || hst->it >= A.maxit || !s->restarted) break;
-        PFEM_HIP(hipMemsetAsync(&st->cycle_done, 0, sizeof(int), c->stream));
+        hipLaunchKernelGGL(k_gm_rearm, dim3(1), dim3(64), 0, c->stream, st);
     }
 }
 
@@ -672,7 +722,7 @@ void solve_mixed(Context* c, pfem_solver_t* s, pfem_csr_t* a, double* x, const d
     IrState* ist = reinterpret_cast<IrState*>(s->xr.p + 2 * (size_t)n);   // state lives after the two n-vectors
     IrState init{};
     init.first = 1;
-    PFEM_HIP(hipMemcpyAsync(ist, &init, offsetof(IrState, counter), hipMemcpyHostToDevice, c->stream));
+    PFEM_HIP(hipMemcpyAsync(ist, &init, sizeof(IrState), hipMemcpyHostToDevice, c->stream));
     double* r = s->xr.p;
     double* z = s->xr.p + n;
     int total_inner = 0, refinements = 0, conv = 0, nan = 0;
@@ -691,13 +741,16 @@ void solve_mixed(Context* c, pfem_solver_t* s, pfem_csr_t* a, double* x, const d
     A.r = s->z32.p; A.z = s->p32.p; A.p = s->q32.p; A.q = s->t32.p; A.t = nullptr; A.V = s->V32.p;
     IrState h{};
     while (true) {
+        int nb0 = (int)g;
         if (jac || none) {
+            nb0 = P.ntiles;
+            const double* invd = jac ? s->pc->invd.p : nullptr;
             if (gm) {
-                if (jac) hipLaunchKernelGGL((k_ir_residual<true, true>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, a->val.p, x, b, s->pc->invd.p, r, z, ist, s->tol, s->tol_mode, s->partials.p);
-                else hipLaunchKernelGGL((k_ir_residual<false, true>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, a->val.p, x, b, (const double*)nullptr, r, z, ist, s->tol, s->tol_mode, s->partials.p);
+                if (jac) hipLaunchKernelGGL((k_ir_residual<true, true>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, a->val.p, x, b, invd, r, z, s->pa);
+                else hipLaunchKernelGGL((k_ir_residual<false, true>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, a->val.p, x, b, invd, r, z, s->pa);
             } else {
-                if (jac) hipLaunchKernelGGL((k_ir_residual<true, false>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, a->val.p, x, b, s->pc->invd.p, r, z, ist, s->tol, s->tol_mode, s->partials.p);
-                else hipLaunchKernelGGL((k_ir_residual<false, false>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, a->val.p, x, b, (const double*)nullptr, r, z, ist, s->tol, s->tol_mode, s->partials.p);
+                if (jac) hipLaunchKernelGGL((k_ir_residual<true, false>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, a->val.p, x, b, invd, r, z, s->pa);
+                else hipLaunchKernelGGL((k_ir_residual<false, false>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, a->val.p, x, b, invd, r, z, s->pa);
             }
         } else {
             hipLaunchKernelGGL((k_residual<double>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, a->val.p, x, b, r);
@@ -707,9 +760,11 @@ void solve_mixed(Context* c, pfem_solver_t* s, pfem_csr_t* a, double* x, const d
                 precond_apply(c, s->pc, s->t.p, b);
                 t = s->t.p;
             }
-            if (gm) hipLaunchKernelGGL((k_ir_norms<true>), dim3(g), dim3(kBlock), 0, c->stream, n, r, z, b, t, ist, s->tol, s->tol_mode, s->partials.p);
-            else hipLaunchKernelGGL((k_ir_norms<false>), dim3(g), dim3(kBlock), 0, c->stream, n, r, z, b, t, ist, s->tol, s->tol_mode, s->partials.p);
+            if (gm) hipLaunchKernelGGL((k_ir_norms<true>), dim3(g), dim3(kBlock), 0, c->stream, n, r, z, b, t, s->pa);
+            else hipLaunchKernelGGL((k_ir_norms<false>), dim3(g), dim3(kBlock), 0, c->stream, n, r, z, b, t, s->pa);
         }
+        if (gm) hipLaunchKernelGGL(k_ir_test<true>, dim3(1), dim3(kBlock), 0, c->stream, ist, (const double*)s->pa, nb0, s->tol, s->tol_mode);
+        else hipLaunchKernelGGL(k_ir_test<false>, dim3(1), dim3(kBlock), 0, c->stream, ist, (const double*)s->pa, nb0, s->tol, s->tol_mode);
         PFEM_HIP(hipMemcpyAsync(s->h_cg, ist, sizeof(IrState), hipMemcpyDeviceToHost, c->stream));
         PFEM_HIP(hipStreamSynchronize(c->stream));
         std::memcpy(&h, s->h_cg, sizeof(IrState));
@@ -800,8 +855,12 @@ int pfem_solver_create(pfem_csr a, int kind, double tol, int max_iterations, int
         s->cg.alloc(1);
         PFEM_HIP(hipMemset(s->cg.p, 0, sizeof(CgState)));
         PFEM_HIP(hipHostMalloc((void**)&s->h_cg, std::max(sizeof(CgState), (size_t)128), hipHostMallocDefault));
-        const size_t np = 8 * (size_t)std::max<unsigned>(a->pat->ntiles, 1024u);
-        s->partials.alloc(np);
+        const size_t na = 2 * (size_t)std::max(a->pat->ntiles, kVecGrid);
+        const size_t nb = (size_t)std::max(2, s->m + 1) * kVecGrid;
+        s->partials.alloc(na + nb + kVecGrid);
+        s->pa = s->partials.p;
+        s->pb = s->pa + na;
+        s->pn = s->pb + nb;
         *out = s.release();
     });
 }

@@ -78,49 +78,48 @@ __global__ void k_ilu0_level(int cnt, const int* __restrict__ rows, const int* _
     if (f[diag[i]] == 0.0) atomicCAS(status, 0, i + 1);
 }
 
-// forward solve of one level, in place on y: y_i = (y_i - sum_{k<i} L_ik y_k) [/ L_ii for IC0]
+// IC0: copy L_ki into the upper slot (i,k) so that the backward sweep with L^T reads its row
+// contiguously instead of gathering through the mirror map
+__global__ void k_ic0_transpose(int n, const int* __restrict__ rowptr, const int* __restrict__ diag,
+                                const int* __restrict__ mirror, double* __restrict__ f)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int p = diag[i] + 1; p < rowptr[i + 1]; ++p) f[p] = f[mirror[p]];
+}
+
+// forward solve of one level: y_i = (b_i - sum_{k<i} L_ik y_k) [/ L_ii for IC0], b_i = src[perm[i]]
+// (perm null: src[i]); y may alias src when perm is null
 template <typename T, bool UNIT>
 __global__ void k_lower_level(int cnt, const int* __restrict__ rows, const int* __restrict__ rowptr,
                               const int* __restrict__ col, const int* __restrict__ diag, const T* __restrict__ f,
-                              T* y)
+                              const int* __restrict__ perm, const T* src, T* y)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= cnt) return;
     const int i = rows[t];
     const int di = diag[i];
-    T s = y[i];
+    T s = src[perm ? perm[i] : i];
     for (int p = rowptr[i]; p < di; ++p) s -= f[p] * y[col[p]];
     y[i] = UNIT ? s : s / f[di];
 }
 
-// backward solve of one level, in place: z_i = (z_i - sum_{k>i} U_ik z_k) / U_ii, with U_ik = f[mirror]
-// (IC0: L^T) or f[p] (ILU0: U)
-template <typename T, bool MIRROR>
+// backward solve of one level, in place on z: z_i = (z_i - sum_{k>i} U_ik z_k) / U_ii (IC0: U = L^T,
+// stored in the upper slots); with a permutation the result is also scattered to out[perm[i]]
+template <typename T>
 __global__ void k_upper_level(int cnt, const int* __restrict__ rows, const int* __restrict__ rowptr,
-                              const int* __restrict__ col, const int* __restrict__ diag,
-                              const int* __restrict__ mirror, const T* __restrict__ f, T* z)
+                              const int* __restrict__ col, const int* __restrict__ diag, const T* __restrict__ f,
+                              const int* __restrict__ perm, T* z, T* out)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= cnt) return;
     const int i = rows[t];
     const int di = diag[i];
     T s = z[i];
-    for (int p = di + 1; p < rowptr[i + 1]; ++p) s -= (MIRROR ? f[mirror[p]] : f[p]) * z[col[p]];
-    z[i] = s / f[di];
-}
-
-template <typename T>
-__global__ void k_permute_in(int n, const int* __restrict__ perm, const T* __restrict__ src, T* __restrict__ dst)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[perm[i]];
-}
-
-template <typename T>
-__global__ void k_permute_out(int n, const int* __restrict__ perm, const T* __restrict__ src, T* __restrict__ dst)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[perm[i]] = src[i];
+    for (int p = di + 1; p < rowptr[i + 1]; ++p) s -= f[p] * z[col[p]];
+    s = s / f[di];
+    z[i] = s;
+    if (perm) out[perm[i]] = s;
 }
 
 template <typename T>
@@ -266,36 +265,39 @@ void precond_initialize(Context* c, pfem_precond_t* pc, pfem_csr_t* a)
             hipLaunchKernelGGL(k_ilu0_level, dim3(blocks_for(cnt)), dim3(kBlock), 0, c->stream, cnt,
                                pc->lrows.p + pc->h_lptr[l], F.rowptr.p, F.col.p, F.diag.p, pc->fac.p, pc->status.p);
     }
+    if (pc->kind == PFEM_PRECOND_IC0)
+        hipLaunchKernelGGL(k_ic0_transpose, dim3(blocks_for(F.n)), dim3(kBlock), 0, c->stream, F.n, F.rowptr.p,
+                           F.diag.p, pc->mirror.p, pc->fac.p);
     if (pc->fac32.p)
         hipLaunchKernelGGL(k_f64_to_f32, dim3(blocks_for(F.nnz)), dim3(kBlock), 0, c->stream, (long)F.nnz,
                            pc->fac32.p, pc->fac.p);
     PFEM_HIP(hipGetLastError());
 }
 
+// Both triangular sweeps. Natural ordering: in place on out (src may equal out). Multicolour: the
+// forward sweep gathers src through the permutation into work, the backward sweep scatters into out.
 template <typename T>
-static void tri_solves(Context* c, pfem_precond_t* pc, T* y, const T* f)
+static void tri_solves(Context* c, pfem_precond_t* pc, const T* src, T* work, T* out, const T* f)
 {
     const Pattern& F = *pc->fpat;
     const bool ic = pc->kind == PFEM_PRECOND_IC0;
+    const int* perm = pc->ordering == PFEM_ORDER_NATURAL ? nullptr : pc->perm.p;
+    T* y = perm ? work : out;
     for (size_t l = 0; l + 1 < pc->h_lptr.size(); ++l) {
         const int cnt = pc->h_lptr[l + 1] - pc->h_lptr[l];
         const int* rows = pc->lrows.p + pc->h_lptr[l];
         if (ic)
             hipLaunchKernelGGL((k_lower_level<T, false>), dim3(blocks_for(cnt)), dim3(kBlock), 0, c->stream, cnt, rows,
-                               F.rowptr.p, F.col.p, F.diag.p, f, y);
+                               F.rowptr.p, F.col.p, F.diag.p, f, perm, src, y);
         else
             hipLaunchKernelGGL((k_lower_level<T, true>), dim3(blocks_for(cnt)), dim3(kBlock), 0, c->stream, cnt, rows,
-                               F.rowptr.p, F.col.p, F.diag.p, f, y);
+                               F.rowptr.p, F.col.p, F.diag.p, f, perm, src, y);
     }
     for (size_t l = 0; l + 1 < pc->h_uptr.size(); ++l) {
         const int cnt = pc->h_uptr[l + 1] - pc->h_uptr[l];
         const int* rows = pc->urows.p + pc->h_uptr[l];
-        if (ic)
-            hipLaunchKernelGGL((k_upper_level<T, true>), dim3(blocks_for(cnt)), dim3(kBlock), 0, c->stream, cnt, rows,
-                               F.rowptr.p, F.col.p, F.diag.p, pc->mirror.p, f, y);
-        else
-            hipLaunchKernelGGL((k_upper_level<T, false>), dim3(blocks_for(cnt)), dim3(kBlock), 0, c->stream, cnt, rows,
-                               F.rowptr.p, F.col.p, F.diag.p, pc->mirror.p, f, y);
+        hipLaunchKernelGGL((k_upper_level<T>), dim3(blocks_for(cnt)), dim3(kBlock), 0, c->stream, cnt, rows,
+                           F.rowptr.p, F.col.p, F.diag.p, f, perm, y, out);
     }
 }
 
@@ -307,17 +309,7 @@ void precond_apply(Context* c, pfem_precond_t* pc, double* z, const double* r)
         if (z != r) hipLaunchKernelGGL((k_copyT<double>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, (long)n, z, r);
         break;
     case PFEM_PRECOND_JACOBI: jacobi_apply(c, n, z, r, pc->invd.p); break;
-    default:
-        if (pc->ordering == PFEM_ORDER_NATURAL) {
-            if (z != r) hipLaunchKernelGGL((k_copyT<double>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, (long)n, z, r);
-            tri_solves<double>(c, pc, z, pc->fac.p);
-        } else {
-            hipLaunchKernelGGL((k_permute_in<double>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, pc->perm.p, r,
-                               pc->tmp.p);
-            tri_solves<double>(c, pc, pc->tmp.p, pc->fac.p);
-            hipLaunchKernelGGL((k_permute_out<double>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, pc->perm.p,
-                               pc->tmp.p, z);
-        }
+    default: tri_solves<double>(c, pc, r, pc->tmp.p, z, pc->fac.p);
     }
     PFEM_HIP(hipGetLastError());
 }
@@ -330,17 +322,7 @@ void precond_apply32(Context* c, pfem_precond_t* pc, float* z, const float* r)
         if (z != r) hipLaunchKernelGGL((k_copyT<float>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, (long)n, z, r);
         break;
     case PFEM_PRECOND_JACOBI: jacobi_apply32(c, n, z, r, pc->invd32.p); break;
-    default:
-        if (pc->ordering == PFEM_ORDER_NATURAL) {
-            if (z != r) hipLaunchKernelGGL((k_copyT<float>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, (long)n, z, r);
-            tri_solves<float>(c, pc, z, pc->fac32.p);
-        } else {
-            hipLaunchKernelGGL((k_permute_in<float>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, pc->perm.p, r,
-                               pc->tmp32.p);
-            tri_solves<float>(c, pc, pc->tmp32.p, pc->fac32.p);
-            hipLaunchKernelGGL((k_permute_out<float>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, pc->perm.p,
-                               pc->tmp32.p, z);
-        }
+    default: tri_solves<float>(c, pc, r, pc->tmp32.p, z, pc->fac32.p);
     }
     PFEM_HIP(hipGetLastError());
 }

@@ -8,14 +8,15 @@
 namespace pfem {
 
 constexpr int kMaxRestart = 1024;   // largest GMRES cycle (the reference's maxIterations default is 1000)
+constexpr int kVecGrid = 1024;      // grid of the vector kernels (grid-stride loops)
 
-// Device-resident PCG scalars. Fields before `counter` are (re)written by the host at each solve.
+// Device-resident PCG scalars, (re)written by the host at the start of each solve. gam[k & 1] holds
+// (r_k, z_k): iteration k reads one slot while its last kernel writes the other.
 struct CgState {
     double tol;
     int tol_mode, maxit;
-    double gamma, thr, pAp, alpha, beta, res;
+    double gam[2], thr, res;
     int it, done, converged, nan;
-    unsigned counter;   // grid-reduction arrival counter, kept at zero between kernels
 };
 
 // Device-resident GMRES(m) scalars and small arrays (H column-major with leading dimension m).
@@ -26,7 +27,6 @@ struct GmState {
     double *H, *beta, *cs, *sn, *y, *hc;
     double res, norm;
     int k, it, done, converged, cycle_done, nan, apply_update;
-    unsigned counter;
 };
 
 #ifdef __HIPCC__
@@ -123,7 +123,11 @@ struct pfem_solver_t {
     double inner_tol = 1e-6;
     int max_refine = 50;
     int n = 0;
-    pfem::DevArray<double> r, z, p, q, t, V, H, beta, cs, sn, y, hc, partials;
+    pfem::DevArray<double> r, z, p, q, t, V, H, beta, cs, sn, y, hc;
+    // per-block partial sums of the reductions: pa (SpMV-shaped producers, 2 values x tiles), pb (vector
+    // producers: 2 values, or one per GMRES coefficient, x kVecGrid), pn (||w||^2 of an Arnoldi step)
+    pfem::DevArray<double> partials;
+    double *pa = nullptr, *pb = nullptr, *pn = nullptr;
     pfem::DevArray<pfem::CgState> cg;
     pfem::DevArray<pfem::GmState> gm;
     pfem::CgState* h_cg = nullptr;   // pinned
