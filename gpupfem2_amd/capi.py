@@ -102,6 +102,7 @@ SIGNATURES = {
     "pfem_precond_initialize": (_i, [_vp, _vp]),
     "pfem_precond_apply": (_i, [_vp, _vp, _vp]),
     "pfem_precond_download": (_i, [_vp, _vp, C.POINTER(_i), _vp]),
+    "pfem_precond_set_sweep": (_i, [_vp, _i]),
     "pfem_solver_create": (_i, [_vp, _i, _d, _i, _i, _i, _vp, _pp]),
     "pfem_solver_set_precision": (_i, [_vp, _i, _d, _i]),
     "pfem_solver_destroy": (_i, [_vp]),
@@ -426,6 +427,10 @@ class Preconditioner(_Handle):
 
     def apply(self, dest, src):
         self.ctx.check(lib().pfem_precond_apply(self.h, dest.ptr, src.ptr))
+
+    def set_sweep(self, single_pass):
+        """IC0/ILU0 triangular solves as one launch each (True) or one launch per level (False)."""
+        self.ctx.check(lib().pfem_precond_set_sweep(self.h, 1 if single_pass else 0))
 
     def download(self):
         nl = C.c_int()

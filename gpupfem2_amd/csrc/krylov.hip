@@ -683,6 +683,7 @@ void cg_solve_fp64(Context* c, pfem_solver_t* s, pfem_csr_t* a, double* x, const
 {
     if (s->pc) precond_initialize(c, s->pc, a);
     cg_run<double>(c, s, args64(s, a, x, b), s->cg.p, s->h_cg);
+    precond_check(c, s->pc);
     if (out) {
         out->converged = s->h_cg->converged;
         out->iterations = s->h_cg->it;
@@ -696,6 +697,7 @@ void gmres_solve_fp64(Context* c, pfem_solver_t* s, pfem_csr_t* a, double* x, co
 {
     if (s->pc) precond_initialize(c, s->pc, a);
     gmres_run<double>(c, s, args64(s, a, x, b), s->gm.p, s->h_gm);
+    precond_check(c, s->pc);
     if (out) {
         out->converged = s->h_gm->converged;
         out->iterations = s->h_gm->it;
@@ -784,6 +786,7 @@ void solve_mixed(Context* c, pfem_solver_t* s, pfem_csr_t* a, double* x, const d
         hipLaunchKernelGGL(k_ir_up, dim3(g), dim3(kBlock), 0, c->stream, n, x, s->d32.p, ist);
         ++refinements;
     }
+    precond_check(c, s->pc);
     if (out) {
         out->converged = conv;
         out->iterations = total_inner;

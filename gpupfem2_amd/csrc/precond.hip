@@ -4,7 +4,16 @@
 // dependencies are complete form a level, one lane computes one row, exactly the row recurrence of the
 // sequential algorithm (so the factor is bit-identical to a sequential CPU factorization of the same
 // matrix). With PFEM_ORDER_MULTICOLOR the matrix is first permuted symmetrically by a greedy colouring,
-// so a level is a colour and an apply needs only a handful of launches.
+// so a level is a colour.
+//
+// Applies run each triangular solve as ONE launch (single-pass sweep): one workgroup per chunk of <= 256
+// rows of one level, chunks numbered level by level. A row publishes its result with an agent-scope
+// atomic store into a buffer that holds a sentinel NaN until then; a row that needs it reads it with an
+// ordinary load and, only while it still sees the sentinel, polls with agent-scope atomic loads. No fence
+// is involved (the value is its own flag), and a workgroup waits only on chunks with smaller indices, so
+// the grid drains in dispatch order. Every wait has a spin limit; exceeding it sets an error flag that
+// the host turns into PFEM_ENUMERIC instead of waiting for ever. The arithmetic per row is that of the
+// per-level kernels, so both forms give bit-identical results.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -122,6 +131,142 @@ __global__ void k_upper_level(int cnt, const int* __restrict__ rows, const int* 
     if (perm) out[perm[i]] = s;
 }
 
+// ---- single-pass sweeps
+template <typename T>
+struct Sentinel;
+template <>
+struct Sentinel<double> {
+    using U = unsigned long long;
+    static constexpr U bits = 0x7FFC0DEC0DEC0DECull;   // quiet NaN with a payload no arithmetic produces
+    static constexpr U nan = 0x7FF8000000000000ull;
+};
+template <>
+struct Sentinel<float> {
+    using U = unsigned int;
+    static constexpr U bits = 0x7FE0DEC5u;
+    static constexpr U nan = 0x7FC00000u;
+};
+
+constexpr int kSpinLimit = 1 << 16;   // polls of one wait (~0.1 s) before it is declared failed
+
+template <typename T>
+__device__ __forceinline__ T await_value(T* p, int* err, int limit)
+{
+    using U = typename Sentinel<T>::U;
+    constexpr U sent = Sentinel<T>::bits;
+    U* q = reinterpret_cast<U*>(p);
+    // an ordinary load sees either the sentinel or the published value: the location is written once
+    // per launch, and the previous launch's contents are not visible past the kernel boundary
+    U v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (v == sent) {
+        int spins = 0;
+        do {
+            __builtin_amdgcn_s_sleep(1);
+            v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } while (v == sent && ++spins < limit);
+        if (v == sent) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return __builtin_bit_cast(T, v);
+}
+
+template <typename T>
+__device__ __forceinline__ void publish(T* p, T v)
+{
+    using U = typename Sentinel<T>::U;
+    constexpr U sent = Sentinel<T>::bits, canonical = Sentinel<T>::nan;
+    U b = __builtin_bit_cast(U, v);
+    if (b == sent) b = canonical;
+    __hip_atomic_store(reinterpret_cast<U*>(p), b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T>
+__device__ __forceinline__ T sentinel_value()
+{
+    constexpr typename Sentinel<T>::U sent = Sentinel<T>::bits;
+    return __builtin_bit_cast(T, sent);
+}
+
+template <typename T>
+__global__ void k_fill_sentinel(long n, T* __restrict__ d)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = sentinel_value<T>();
+}
+
+// s - sum_{p0 <= p < p1} f[p] * buf[col[p]], subtracted in ascending p (the sequential order); the
+// column indices, factor values and first reads of up to kDeps dependencies are issued together, and
+// only dependencies that still read as the sentinel are polled
+constexpr int kDeps = 8;
+
+template <typename T>
+__device__ __forceinline__ T dep_sum(T s, int p0, int p1, const int* __restrict__ col, const T* __restrict__ f,
+                                     T* buf, int* err, int limit)
+{
+    using U = typename Sentinel<T>::U;
+    constexpr U sent = Sentinel<T>::bits;
+    for (int base = p0; base < p1; base += kDeps) {
+        const int m = min(kDeps, p1 - base);
+        int cv[kDeps];
+        T fv[kDeps];
+        U yv[kDeps];
+#pragma unroll
+        for (int k = 0; k < kDeps; ++k)
+            if (k < m) {
+                cv[k] = col[base + k];
+                fv[k] = f[base + k];
+            }
+#pragma unroll
+        for (int k = 0; k < kDeps; ++k)
+            if (k < m) yv[k] = __hip_atomic_load(reinterpret_cast<U*>(buf + cv[k]), __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+        for (int k = 0; k < kDeps; ++k)
+            if (k < m) {
+                if (yv[k] == sent) yv[k] = __builtin_bit_cast(U, await_value(buf + cv[k], err, limit));
+                s -= fv[k] * __builtin_bit_cast(T, yv[k]);
+            }
+    }
+    return s;
+}
+
+// forward solve, all levels: y_i = (b_i - sum_{k<i} L_ik y_k) [/ L_ii for IC0], b_i = src[perm[i]];
+// also re-arms sw for the backward solve. CONTIG: a chunk is a row range, else positions in rows.
+template <typename T, bool UNIT, bool CONTIG>
+__global__ __launch_bounds__(kBlock) void k_sweep_lower(const int* __restrict__ chunks, const int* __restrict__ rows,
+                                                        const int* __restrict__ rowptr, const int* __restrict__ col,
+                                                        const int* __restrict__ diag, const T* __restrict__ f,
+                                                        const int* __restrict__ perm, const T* src, T* sy, T* sw,
+                                                        int* err, int limit)
+{
+    const int c0 = chunks[2 * blockIdx.x], cnt = chunks[2 * blockIdx.x + 1];
+    if ((int)threadIdx.x >= cnt) return;
+    const int i = CONTIG ? c0 + (int)threadIdx.x : rows[c0 + threadIdx.x];
+    const int di = diag[i];
+    T s = dep_sum(src[perm ? perm[i] : i], rowptr[i], di, col, f, sy, err, limit);
+    publish(sy + i, UNIT ? s : s / f[di]);
+    sw[i] = sentinel_value<T>();
+}
+
+// backward solve, all levels: z_i = (y_i - sum_{k>i} U_ik z_k) / U_ii, out[perm[i]] = z_i; re-arms sy
+template <typename T, bool CONTIG>
+__global__ __launch_bounds__(kBlock) void k_sweep_upper(const int* __restrict__ chunks, const int* __restrict__ rows,
+                                                        const int* __restrict__ rowptr, const int* __restrict__ col,
+                                                        const int* __restrict__ diag, const T* __restrict__ f,
+                                                        const int* __restrict__ perm, T* sy, T* sw, T* out, int* err,
+                                                        int limit)
+{
+    const int c0 = chunks[2 * blockIdx.x], cnt = chunks[2 * blockIdx.x + 1];
+    if ((int)threadIdx.x >= cnt) return;
+    const int i = CONTIG ? c0 + (int)threadIdx.x : rows[c0 + threadIdx.x];
+    const int di = diag[i];
+    const T yi = sy[i];
+    sy[i] = sentinel_value<T>();
+    T s = dep_sum(yi, di + 1, rowptr[i + 1], col, f, sw, err, limit);
+    s = s / f[di];
+    publish(sw + i, s);
+    out[perm ? perm[i] : i] = s;
+}
+
 template <typename T>
 __global__ void k_copyT(long n, T* __restrict__ d, const T* __restrict__ s)
 {
@@ -141,6 +286,7 @@ static void build_factor_structure(Context* c, pfem_precond_t* pc)
     const Pattern& A = *pc->pat;
     const int n = A.n;
     std::vector<int> perm(n), iperm(n);
+    std::vector<int> cptr;   // multicolour: colour c occupies rows [cptr[c], cptr[c+1]) of the new order
     if (pc->ordering == PFEM_ORDER_MULTICOLOR) {
         // greedy colouring in natural order; new order = (colour, old index)
         std::vector<int> colour(n, -1);
@@ -159,6 +305,9 @@ static void build_factor_structure(Context* c, pfem_precond_t* pc)
         }
         std::iota(perm.begin(), perm.end(), 0);
         std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return colour[a] < colour[b]; });
+        cptr.assign(ncol + 1, 0);
+        for (int i = 0; i < n; ++i) cptr[colour[i] + 1]++;
+        for (int k = 0; k < ncol; ++k) cptr[k + 1] += cptr[k];
     } else {
         std::iota(perm.begin(), perm.end(), 0);
     }
@@ -181,7 +330,10 @@ static void build_factor_structure(Context* c, pfem_precond_t* pc)
     else pc->fpat = make_pattern(c, n, std::vector<int>(rowptr), std::vector<int>(col));
     const Pattern& F = *pc->fpat;
     for (int i = 0; i < n; ++i) PFEM_REQUIRE(F.h_diag[i] >= 0, "IC0/ILU0 needs every diagonal entry in the pattern");
-    // forward levels
+    // forward levels: longest-path levels of the lower triangle; in the multicolour order a colour is an
+    // independent set whose lower neighbours all have smaller colours, so the colours are the levels
+    // (contiguous row ranges, forward in colour order and backward in reverse colour order)
+    const bool colour_levels = !cptr.empty();
     std::vector<int> lev(n, 0);
     int nl = 0;
     for (int i = 0; i < n; ++i) {
@@ -189,6 +341,11 @@ static void build_factor_structure(Context* c, pfem_precond_t* pc)
         for (int k = F.h_rowptr[i]; k < F.h_diag[i]; ++k) l = std::max(l, lev[F.h_col[k]] + 1);
         lev[i] = l;
         nl = std::max(nl, l + 1);
+    }
+    if (colour_levels) {
+        nl = (int)cptr.size() - 1;
+        for (int k = 0; k < nl; ++k)
+            for (int i = cptr[k]; i < cptr[k + 1]; ++i) lev[i] = k;
     }
     std::vector<int> lptr(nl + 1, 0), lrows(n);
     for (int i = 0; i < n; ++i) lptr[lev[i] + 1]++;
@@ -205,6 +362,11 @@ static void build_factor_structure(Context* c, pfem_precond_t* pc)
         for (int k = F.h_diag[i] + 1; k < F.h_rowptr[i + 1]; ++k) l = std::max(l, ulev[F.h_col[k]] + 1);
         ulev[i] = l;
         nu = std::max(nu, l + 1);
+    }
+    if (colour_levels) {
+        nu = nl;
+        for (int k = 0; k < nl; ++k)
+            for (int i = cptr[k]; i < cptr[k + 1]; ++i) ulev[i] = nl - 1 - k;
     }
     std::vector<int> uptr(nu + 1, 0), urows(n);
     for (int i = 0; i < n; ++i) uptr[ulev[i] + 1]++;
@@ -236,10 +398,33 @@ static void build_factor_structure(Context* c, pfem_precond_t* pc)
     upload(c, pc->src_pos, src);
     upload(c, pc->perm, perm);
     upload(c, pc->iperm, iperm);
+    // chunk = {first position in rows, count}; with colour levels every level is a contiguous row range
+    // and a chunk stores {first row, count} instead
+    auto chunk_table = [&](const std::vector<int>& ptr, const std::vector<int>& rows) {
+        std::vector<int> ch;
+        for (size_t l = 0; l + 1 < ptr.size(); ++l)
+            for (int o = ptr[l]; o < ptr[l + 1]; o += kBlock) {
+                ch.push_back(colour_levels ? rows[o] : o);
+                ch.push_back(std::min(kBlock, ptr[l + 1] - o));
+            }
+        return ch;
+    };
+    const std::vector<int> lch = chunk_table(lptr, lrows), uch = chunk_table(uptr, urows);
+    pc->contiguous = colour_levels;
+    pc->nlchunks = (int)lch.size() / 2;
+    pc->nuchunks = (int)uch.size() / 2;
+    upload(c, pc->lchunks, lch);
+    upload(c, pc->uchunks, uch);
     pc->fac.alloc(F.nnz);
     pc->tmp.alloc(n);
+    pc->sy.alloc(n);
+    pc->sw.alloc(n);
+    hipLaunchKernelGGL((k_fill_sentinel<double>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, (long)n, pc->sy.p);
+    hipLaunchKernelGGL((k_fill_sentinel<double>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, (long)n, pc->sw.p);
     pc->status.alloc(1);
     zero_async(c, pc->status.p, sizeof(int));
+    pc->sweep_err.alloc(1);
+    zero_async(c, pc->sweep_err.p, sizeof(int));
 }
 
 void precond_initialize(Context* c, pfem_precond_t* pc, pfem_csr_t* a)
@@ -277,11 +462,27 @@ void precond_initialize(Context* c, pfem_precond_t* pc, pfem_csr_t* a)
 // Both triangular sweeps. Natural ordering: in place on out (src may equal out). Multicolour: the
 // forward sweep gathers src through the permutation into work, the backward sweep scatters into out.
 template <typename T>
-static void tri_solves(Context* c, pfem_precond_t* pc, const T* src, T* work, T* out, const T* f)
+static void tri_solves(Context* c, pfem_precond_t* pc, const T* src, T* work, T* out, const T* f, T* sy, T* sw)
 {
     const Pattern& F = *pc->fpat;
     const bool ic = pc->kind == PFEM_PRECOND_IC0;
     const int* perm = pc->ordering == PFEM_ORDER_NATURAL ? nullptr : pc->perm.p;
+    if (pc->sweep) {
+        const dim3 gl(pc->nlchunks), gu(pc->nuchunks), bl(kBlock);
+        const int* lc = pc->lchunks.p;
+        const int* uc = pc->uchunks.p;
+        int* err = pc->sweep_err.p;
+        if (pc->contiguous) {
+            if (ic) hipLaunchKernelGGL((k_sweep_lower<T, false, true>), gl, bl, 0, c->stream, lc, pc->lrows.p, F.rowptr.p, F.col.p, F.diag.p, f, perm, src, sy, sw, err, kSpinLimit);
+            else hipLaunchKernelGGL((k_sweep_lower<T, true, true>), gl, bl, 0, c->stream, lc, pc->lrows.p, F.rowptr.p, F.col.p, F.diag.p, f, perm, src, sy, sw, err, kSpinLimit);
+            hipLaunchKernelGGL((k_sweep_upper<T, true>), gu, bl, 0, c->stream, uc, pc->urows.p, F.rowptr.p, F.col.p, F.diag.p, f, perm, sy, sw, out, err, kSpinLimit);
+        } else {
+            if (ic) hipLaunchKernelGGL((k_sweep_lower<T, false, false>), gl, bl, 0, c->stream, lc, pc->lrows.p, F.rowptr.p, F.col.p, F.diag.p, f, perm, src, sy, sw, err, kSpinLimit);
+            else hipLaunchKernelGGL((k_sweep_lower<T, true, false>), gl, bl, 0, c->stream, lc, pc->lrows.p, F.rowptr.p, F.col.p, F.diag.p, f, perm, src, sy, sw, err, kSpinLimit);
+            hipLaunchKernelGGL((k_sweep_upper<T, false>), gu, bl, 0, c->stream, uc, pc->urows.p, F.rowptr.p, F.col.p, F.diag.p, f, perm, sy, sw, out, err, kSpinLimit);
+        }
+        return;
+    }
     T* y = perm ? work : out;
     for (size_t l = 0; l + 1 < pc->h_lptr.size(); ++l) {
         const int cnt = pc->h_lptr[l + 1] - pc->h_lptr[l];
@@ -309,7 +510,7 @@ void precond_apply(Context* c, pfem_precond_t* pc, double* z, const double* r)
         if (z != r) hipLaunchKernelGGL((k_copyT<double>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, (long)n, z, r);
         break;
     case PFEM_PRECOND_JACOBI: jacobi_apply(c, n, z, r, pc->invd.p); break;
-    default: tri_solves<double>(c, pc, r, pc->tmp.p, z, pc->fac.p);
+    default: tri_solves<double>(c, pc, r, pc->tmp.p, z, pc->fac.p, pc->sy.p, pc->sw.p);
     }
     PFEM_HIP(hipGetLastError());
 }
@@ -322,7 +523,7 @@ void precond_apply32(Context* c, pfem_precond_t* pc, float* z, const float* r)
         if (z != r) hipLaunchKernelGGL((k_copyT<float>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, (long)n, z, r);
         break;
     case PFEM_PRECOND_JACOBI: jacobi_apply32(c, n, z, r, pc->invd32.p); break;
-    default: tri_solves<float>(c, pc, r, pc->tmp32.p, z, pc->fac32.p);
+    default: tri_solves<float>(c, pc, r, pc->tmp32.p, z, pc->fac32.p, pc->sy32.p, pc->sw32.p);
     }
     PFEM_HIP(hipGetLastError());
 }
@@ -334,8 +535,32 @@ void precond_enable_fp32(Context* c, pfem_precond_t* pc)
     } else if ((pc->kind == PFEM_PRECOND_IC0 || pc->kind == PFEM_PRECOND_ILU0) && !pc->fac32.p) {
         pc->fac32.alloc(pc->fpat->nnz);
         pc->tmp32.alloc(pc->n);
+        pc->sy32.alloc(pc->n);
+        pc->sw32.alloc(pc->n);
+        const long n = pc->n;
+        hipLaunchKernelGGL((k_fill_sentinel<float>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, pc->sy32.p);
+        hipLaunchKernelGGL((k_fill_sentinel<float>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, pc->sw32.p);
+        PFEM_HIP(hipGetLastError());
     }
-    (void)c;
+}
+
+// Reports a single-pass sweep whose wait exceeded its spin limit (and re-arms the sweep buffers).
+void precond_check(Context* c, pfem_precond_t* pc)
+{
+    if (!pc || !pc->sweep_err.p) return;
+    int e = 0;
+    d2h_sync(c, &e, pc->sweep_err.p, sizeof(int));
+    if (!e) return;
+    const long n = pc->n;
+    zero_async(c, pc->sweep_err.p, sizeof(int));
+    hipLaunchKernelGGL((k_fill_sentinel<double>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, pc->sy.p);
+    hipLaunchKernelGGL((k_fill_sentinel<double>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, pc->sw.p);
+    if (pc->sy32.p) {
+        hipLaunchKernelGGL((k_fill_sentinel<float>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, pc->sy32.p);
+        hipLaunchKernelGGL((k_fill_sentinel<float>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, pc->sw32.p);
+    }
+    PFEM_HIP(hipStreamSynchronize(c->stream));
+    throw Error(PFEM_ENUMERIC, "triangular sweep: a wait exceeded its spin limit");
 }
 
 }  // namespace pfem
@@ -396,6 +621,15 @@ int pfem_precond_apply(pfem_precond pc, double* d_dest, const double* d_src)
     PFEM_TRY(pc ? pc->ctx : nullptr, {
         PFEM_REQUIRE(pc && d_dest && d_src, "null argument");
         precond_apply(C(pc->ctx), pc, d_dest, d_src);
+        precond_check(C(pc->ctx), pc);
+    });
+}
+
+int pfem_precond_set_sweep(pfem_precond pc, int single_pass)
+{
+    PFEM_TRY(pc ? pc->ctx : nullptr, {
+        PFEM_REQUIRE(pc, "null argument");
+        pc->sweep = single_pass ? 1 : 0;
     });
 }
 

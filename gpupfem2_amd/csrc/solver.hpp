@@ -109,6 +109,16 @@ struct pfem_precond_t {
     pfem::DevArray<float> tmp32, tmp32b;
     pfem::DevArray<int> status;           // first failing row (+1), 0 if none
     int nlevels = 0;
+    // single-pass sweeps: rows in chunks of <= 256 of one level, numbered level by level
+    // ({offset into lrows/urows, count} pairs); sy / sw hold the forward / backward results in the
+    // factor's ordering and read as a sentinel NaN until a row has published its value
+    int sweep = 1;                        // 1: one launch per triangular solve, 0: one launch per level
+    pfem::DevArray<int> lchunks, uchunks;
+    int nlchunks = 0, nuchunks = 0;
+    bool contiguous = false;              // levels are row ranges (multicolour): chunks hold row ranges
+    pfem::DevArray<double> sy, sw;
+    pfem::DevArray<float> sy32, sw32;
+    pfem::DevArray<int> sweep_err;        // set when a wait exceeded its spin limit
 };
 
 struct pfem_solver_t {
@@ -141,6 +151,7 @@ namespace pfem {
 void precond_initialize(Context* c, pfem_precond_t* pc, pfem_csr_t* a);
 void precond_apply(Context* c, pfem_precond_t* pc, double* z, const double* r);
 void precond_apply32(Context* c, pfem_precond_t* pc, float* z, const float* r);
+void precond_check(Context* c, pfem_precond_t* pc);
 void jacobi_setup(Context* c, const Pattern& P, const double* val, double* invd, float* invd32);
 void jacobi_apply(Context* c, int n, double* dest, const double* src, const double* invd);
 void jacobi_apply32(Context* c, int n, float* dest, const float* src, const float* invd);

@@ -149,6 +149,10 @@ int pfem_precond_initialize(pfem_precond pc, pfem_csr a);
 int pfem_precond_apply(pfem_precond pc, double* d_dest, const double* d_src);
 /* factor values (nnz doubles, in the matrix's own CSR layout and row order), levels, permutation */
 int pfem_precond_download(pfem_precond pc, double* factor, int* n_levels, int* perm);
+/* IC0/ILU0 apply as one launch per triangular solve (1, default) or one launch per level (0); both give
+ * bit-identical results. Replaces the analysis + cusparseSpSV_solve pair of applyPreconditioner
+ * (preconditioners.cu:116-133, :195-212). */
+int pfem_precond_set_sweep(pfem_precond pc, int single_pass);
 
 /* ------------------------------------------------------------------ Krylov solvers (linear_solver.cuh) */
 #define PFEM_SOLVER_CG 0
