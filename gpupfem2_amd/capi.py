@@ -17,7 +17,7 @@ PRECOND_NONE, PRECOND_JACOBI, PRECOND_IC0, PRECOND_ILU0 = 0, 1, 2, 3
 ORDER_NATURAL, ORDER_MULTICOLOR = 0, 1
 SOLVER_CG, SOLVER_GMRES = 0, 1
 TOL_ABSOLUTE, TOL_RELATIVE = 0, 1
-PRECISION_FP64, PRECISION_MIXED = 0, 1
+PRECISION_FP64, PRECISION_MIXED, PRECISION_MIXED_PRECOND = 0, 1, 2
 
 
 class SolveStats(C.Structure):

@@ -64,17 +64,29 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], double* __restri
     }
 }
 
-// Totals of the nb partials of each of the NV values; valid in every thread.
+// Totals of the nb partials of each of the NV values; valid in every thread. Lane t adds partials
+// t, t + 256, t + 512, ... in that order; the loads of 8 consecutive terms are issued together.
 template <int NV>
 __device__ __forceinline__ void sum_partials(const double* __restrict__ part, int nb, double (&tot)[NV])
 {
     __shared__ double res[NV];
+    constexpr int kU = 8;
     double v[NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) v[i] = 0.0;
-    for (int k = threadIdx.x; k < nb; k += kBlock) {
+    for (int k0 = threadIdx.x; k0 < nb; k0 += kU * kBlock) {
+        double t[NV][kU];
 #pragma unroll
-        for (int i = 0; i < NV; ++i) v[i] += part[(size_t)i * nb + k];
+        for (int u = 0; u < kU; ++u) {
+            const int k = k0 + u * kBlock;
+#pragma unroll
+            for (int i = 0; i < NV; ++i) t[i][u] = k < nb ? part[(size_t)i * nb + k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+#pragma unroll
+            for (int i = 0; i < NV; ++i) v[i] += t[i][u];
+        }
     }
     block_sum<NV>(v);
     if (threadIdx.x == 0) {

@@ -160,14 +160,16 @@ __global__ void k_cg_update(int n, T* __restrict__ x, T* __restrict__ r, const T
 
 template <typename T>
 __global__ void k_cg_update_xr(int n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
-                               const T* __restrict__ q, const CgState* st, const double* __restrict__ pa, int na, int k)
+                               const T* __restrict__ q, CgState* st, const double* __restrict__ pa, int na, int k)
 {
     if (st->done) return;
-    const T alpha = (T)cg_alpha(st, pa, na, k);
+    const double a = cg_alpha(st, pa, na, k);
+    const T alpha = (T)a;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         x[i] += alpha * p[i];
         r[i] = r[i] - alpha * q[i];
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha[k & 1] = a;
 }
 
 template <typename T>
@@ -500,14 +502,14 @@ struct KArgs {
     double tol;
     int tol_mode, maxit;
     T *r, *z, *p, *q, *t, *V;
+    bool pc_fp32;        // FP64 vectors, IC0/ILU0 factor applied in FP32
 };
 
 template <typename T>
-static void papply(Context* c, pfem_precond_t* pc, T* z, const T* r);
-template <>
-void papply<double>(Context* c, pfem_precond_t* pc, double* z, const double* r) { precond_apply(c, pc, z, r); }
-template <>
-void papply<float>(Context* c, pfem_precond_t* pc, float* z, const float* r) { precond_apply32(c, pc, z, r); }
+static void papply(Context* c, const KArgs<T>& A, T* z, const T* r)
+{
+    precond_apply_t<T>(c, A.pc, z, r, A.pc_fp32, DotFuse<T>{});
+}
 
 template <typename T>
 static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st, CgState* hst)
@@ -538,11 +540,11 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
         } else {
             hipLaunchKernelGGL((k_residual<T>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.x, A.b, A.r);
         }
-        papply<T>(c, A.pc, A.z, A.r);
+        papply<T>(c, A, A.z, A.r);
         const T* t = nullptr;
         if (A.tol_mode == PFEM_TOL_RELATIVE) {
             if (A.x_zero) t = A.z;
-            else { papply<T>(c, A.pc, A.t, A.b); t = A.t; }
+            else { papply<T>(c, A, A.t, A.b); t = A.t; }
         }
         hipLaunchKernelGGL((k_cg_start<T>), dim3(g), dim3(kBlock), 0, c->stream, n, A.r, A.z, A.p, A.b, t, s->pa);
     }
@@ -554,6 +556,7 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
     while (true) {
         for (int j = 0; j < batch; ++j) {
             const int k = launched + j;
+            int nbk = nb;
             hipLaunchKernelGGL((k_cg_spmv<T>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.p, A.q, st, s->pa);
             if (jac) {
                 hipLaunchKernelGGL((k_cg_update<T, true>), dim3(g), dim3(kBlock), 0, c->stream, n, A.x, A.r, A.p, A.q, A.z, A.invd, st, s->pa, na, s->pb, k);
@@ -561,10 +564,14 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
                 hipLaunchKernelGGL((k_cg_update<T, false>), dim3(g), dim3(kBlock), 0, c->stream, n, A.x, A.r, A.p, A.q, A.r, A.invd, st, s->pa, na, s->pb, k);
             } else {
                 hipLaunchKernelGGL((k_cg_update_xr<T>), dim3(g), dim3(kBlock), 0, c->stream, n, A.x, A.r, A.p, A.q, st, s->pa, na, k);
-                papply<T>(c, A.pc, A.z, A.r);
-                hipLaunchKernelGGL((k_cg_rz<T>), dim3(g), dim3(kBlock), 0, c->stream, n, A.r, A.z, st, s->pb);
+                // the triangular sweeps also produce the (r, z) partials when they can
+                nbk = precond_apply_t<T>(c, A.pc, A.z, A.r, A.pc_fp32, DotFuse<T>{s->pb, s->pb_cap});
+                if (!nbk) {
+                    hipLaunchKernelGGL((k_cg_rz<T>), dim3(g), dim3(kBlock), 0, c->stream, n, A.r, A.z, st, s->pb);
+                    nbk = nb;
+                }
             }
-            hipLaunchKernelGGL((k_cg_p<T>), dim3(g), dim3(kBlock), 0, c->stream, n, A.p, none ? A.r : A.z, st, s->pb, nb, k);
+            hipLaunchKernelGGL((k_cg_p<T>), dim3(g), dim3(kBlock), 0, c->stream, n, A.p, none ? A.r : A.z, st, s->pb, nbk, k);
         }
         launched += batch;
         PFEM_HIP(hipGetLastError());
@@ -617,11 +624,11 @@ static void gmres_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, GmState* 
             } else {
                 hipLaunchKernelGGL((k_residual<T>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.x, A.b, V);
             }
-            papply<T>(c, A.pc, V, V);
+            papply<T>(c, A, V, V);
             const T* t = nullptr;
             if (first && A.tol_mode == PFEM_TOL_RELATIVE) {
                 if (A.x_zero) t = V;
-                else { papply<T>(c, A.pc, A.t, A.b); t = A.t; }
+                else { papply<T>(c, A, A.t, A.b); t = A.t; }
             }
             hipLaunchKernelGGL((k_gm_norms<T>), dim3(g), dim3(kBlock), 0, c->stream, n, V, t, s->pa);
         }
@@ -635,7 +642,7 @@ static void gmres_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, GmState* 
                 T* w = V + (size_t)(j + 1) * ld;
                 if (jac) hipLaunchKernelGGL((k_gm_arnoldi_spmv<T, true>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, vj, A.invd, w, st);
                 else hipLaunchKernelGGL((k_gm_arnoldi_spmv<T, false>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, vj, A.invd, w, st);
-                if (!jac && !none) papply<T>(c, A.pc, w, w);
+                if (!jac && !none) papply<T>(c, A, w, w);
                 const int passes = s->reorth ? 2 : 1;
                 for (int pass = 0; pass < passes; ++pass) {
                     for (int i0 = 0; i0 <= j; i0 += 8)
@@ -664,6 +671,8 @@ static void gmres_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, GmState* 
 static KArgs<double> args64(pfem_solver_t* s, pfem_csr_t* a, double* x, const double* b)
 {
     KArgs<double> A{};
+    A.pc_fp32 = s->precision == PFEM_PRECISION_MIXED_PRECOND && s->pc &&
+                (s->pc->kind == PFEM_PRECOND_IC0 || s->pc->kind == PFEM_PRECOND_ILU0);
     A.P = a->pat.get();
     A.val = a->val.p;
     A.kind = s->pc ? s->pc->kind : PFEM_PRECOND_NONE;
@@ -859,11 +868,13 @@ int pfem_solver_create(pfem_csr a, int kind, double tol, int max_iterations, int
         PFEM_HIP(hipMemset(s->cg.p, 0, sizeof(CgState)));
         PFEM_HIP(hipHostMalloc((void**)&s->h_cg, std::max(sizeof(CgState), (size_t)128), hipHostMallocDefault));
         const size_t na = 2 * (size_t)std::max(a->pat->ntiles, kVecGrid);
-        const size_t nb = (size_t)std::max(2, s->m + 1) * kVecGrid;
+        // pb also takes the per-chunk (r, z) partials of the single-pass sweeps (n/256 + one per level)
+        const size_t nb = std::max<size_t>((size_t)std::max(2, s->m + 1) * kVecGrid, blocks_for(n) + 4096);
         s->partials.alloc(na + nb + kVecGrid);
         s->pa = s->partials.p;
         s->pb = s->pa + na;
         s->pn = s->pb + nb;
+        s->pb_cap = (int)nb;
         *out = s.release();
     });
 }
@@ -872,11 +883,19 @@ int pfem_solver_set_precision(pfem_solver s, int precision, double inner_tol, in
 {
     PFEM_TRY(s ? s->ctx : nullptr, {
         PFEM_REQUIRE(s, "null solver");
-        PFEM_REQUIRE(precision == PFEM_PRECISION_FP64 || precision == PFEM_PRECISION_MIXED, "unknown precision");
+        PFEM_REQUIRE(precision == PFEM_PRECISION_FP64 || precision == PFEM_PRECISION_MIXED ||
+                         precision == PFEM_PRECISION_MIXED_PRECOND,
+                     "unknown precision");
+        if (precision == PFEM_PRECISION_MIXED_PRECOND)
+            PFEM_REQUIRE(s->pc && (s->pc->kind == PFEM_PRECOND_IC0 || s->pc->kind == PFEM_PRECOND_ILU0),
+                         "PFEM_PRECISION_MIXED_PRECOND needs an IC0 or ILU0 preconditioner");
         s->precision = precision;
         if (inner_tol > 0) s->inner_tol = inner_tol;
         if (max_refinements > 0) s->max_refine = max_refinements;
-        if (precision == PFEM_PRECISION_MIXED) {
+        if (precision == PFEM_PRECISION_MIXED_PRECOND) {
+            precond_enable_fp32(C(s->ctx), s->pc);
+            s->pc->sweep = 1;
+        } else if (precision == PFEM_PRECISION_MIXED) {
             if (s->kind == PFEM_SOLVER_GMRES && !s->gm.p) {
                 s->gm.alloc(1);
                 PFEM_HIP(hipMemset(s->gm.p, 0, sizeof(GmState)));
@@ -890,7 +909,7 @@ int pfem_solver_set_precision(pfem_solver s, int precision, double inner_tol, in
 int pfem_time_kernel(int op, pfem_csr a, pfem_precond pc, int launches, double* avg_ms)
 {
     PFEM_TRY(a ? a->ctx : nullptr, {
-        PFEM_REQUIRE(a && avg_ms && launches > 0 && op >= 0 && op <= 3, "bad arguments");
+        PFEM_REQUIRE(a && avg_ms && launches > 0 && op >= 0 && op <= 4, "bad arguments");
         PFEM_REQUIRE(op < 2 || pc, "ops 2 and 3 need a preconditioner");
         Context* c = C(a->ctx);
         const Pattern& P = *a->pat;
@@ -904,7 +923,7 @@ int pfem_time_kernel(int op, pfem_csr a, pfem_precond pc, int launches, double* 
             hipLaunchKernelGGL(k_to32, dim3(blocks_for(P.nnz)), dim3(kBlock), 0, c->stream, (long)P.nnz, a->val32.p, a->val.p);
         }
         if (op >= 2) {
-            if (op == 3) precond_enable_fp32(c, pc);
+            if (op >= 3) precond_enable_fp32(c, pc);
             precond_initialize(c, pc, a);
         }
         auto launch = [&]() {
@@ -912,7 +931,8 @@ int pfem_time_kernel(int op, pfem_csr a, pfem_precond pc, int launches, double* 
             case 0: launch_spmv(c, P, a->val.p, x.p, y.p); break;
             case 1: launch_spmv32(c, P, a->val32.p, x32.p, y32.p); break;
             case 2: precond_apply(c, pc, y.p, x.p); break;
-            default: precond_apply32(c, pc, y32.p, x32.p); break;
+            case 3: precond_apply32(c, pc, y32.p, x32.p); break;
+            default: precond_apply_t<double>(c, pc, y.p, x.p, true, DotFuse<double>{}); break;
             }
         };
         launch();   // warm-up

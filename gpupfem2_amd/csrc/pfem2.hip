@@ -86,8 +86,12 @@ void make_solver(pfem_pfem2_t* s, pfem_csr a, int kind, int pc_kind, int orderin
     const pfem_step_config& c = s->cfg;
     check(pfem_precond_create(a, pc_kind, ordering, pc), s->ctx);
     check(pfem_solver_create(a, kind, c.tol, c.maxit, c.restart, c.tol_mode, *pc, sv), s->ctx);
-    if (c.precision == PFEM_PRECISION_MIXED)
-        check(pfem_solver_set_precision(*sv, PFEM_PRECISION_MIXED, c.inner_tol, 0), s->ctx);
+    if (c.precision == PFEM_PRECISION_MIXED) {
+        const bool factor = pc_kind == PFEM_PRECOND_IC0 || pc_kind == PFEM_PRECOND_ILU0;
+        check(pfem_solver_set_precision(*sv, factor ? PFEM_PRECISION_MIXED_PRECOND : PFEM_PRECISION_MIXED,
+                                        c.inner_tol, 0),
+              s->ctx);
+    }
 }
 
 void solve(pfem_pfem2_t* s, pfem_solver sv, pfem_csr a, double* x, const double* b)

@@ -198,8 +198,8 @@ __global__ void k_fill_sentinel(long n, T* __restrict__ d)
 // only dependencies that still read as the sentinel are polled
 constexpr int kDeps = 8;
 
-template <typename T>
-__device__ __forceinline__ T dep_sum(T s, int p0, int p1, const int* __restrict__ col, const T* __restrict__ f,
+template <typename T, typename TS>
+__device__ __forceinline__ T dep_sum(T s, int p0, int p1, const int* __restrict__ col, const TS* __restrict__ f,
                                      T* buf, int* err, int limit)
 {
     using U = typename Sentinel<T>::U;
@@ -213,7 +213,7 @@ __device__ __forceinline__ T dep_sum(T s, int p0, int p1, const int* __restrict_
         for (int k = 0; k < kDeps; ++k)
             if (k < m) {
                 cv[k] = col[base + k];
-                fv[k] = f[base + k];
+                fv[k] = (T)f[base + k];
             }
 #pragma unroll
         for (int k = 0; k < kDeps; ++k)
@@ -231,40 +231,70 @@ __device__ __forceinline__ T dep_sum(T s, int p0, int p1, const int* __restrict_
 
 // forward solve, all levels: y_i = (b_i - sum_{k<i} L_ik y_k) [/ L_ii for IC0], b_i = src[perm[i]];
 // also re-arms sw for the backward solve. CONTIG: a chunk is a row range, else positions in rows.
-template <typename T, bool UNIT, bool CONTIG>
+// TV is the type of the vectors, TF that of the factor and of the arithmetic (FP64 vectors with an FP32
+// factor: the mixed-precision preconditioner of an FP64 Krylov solve).
+template <typename TV, typename TS, typename TF, bool UNIT, bool CONTIG>
 __global__ __launch_bounds__(kBlock) void k_sweep_lower(const int* __restrict__ chunks, const int* __restrict__ rows,
                                                         const int* __restrict__ rowptr, const int* __restrict__ col,
-                                                        const int* __restrict__ diag, const T* __restrict__ f,
-                                                        const int* __restrict__ perm, const T* src, T* sy, T* sw,
+                                                        const int* __restrict__ diag, const TS* __restrict__ f,
+                                                        const int* __restrict__ perm, const TV* src, TF* sy, TF* sw,
                                                         int* err, int limit)
 {
     const int c0 = chunks[2 * blockIdx.x], cnt = chunks[2 * blockIdx.x + 1];
     if ((int)threadIdx.x >= cnt) return;
     const int i = CONTIG ? c0 + (int)threadIdx.x : rows[c0 + threadIdx.x];
     const int di = diag[i];
-    T s = dep_sum(src[perm ? perm[i] : i], rowptr[i], di, col, f, sy, err, limit);
-    publish(sy + i, UNIT ? s : s / f[di]);
-    sw[i] = sentinel_value<T>();
+    TF s = dep_sum((TF)src[perm ? perm[i] : i], rowptr[i], di, col, f, sy, err, limit);
+    publish(sy + i, UNIT ? s : s / (TF)f[di]);
+    sw[i] = sentinel_value<TF>();
 }
 
-// backward solve, all levels: z_i = (y_i - sum_{k>i} U_ik z_k) / U_ii, out[perm[i]] = z_i; re-arms sy
-template <typename T, bool CONTIG>
+// backward solve, all levels: z_i = (y_i - sum_{k>i} U_ik z_k) / U_ii, out[perm[i]] = z_i; re-arms sy.
+// part != null: the chunk's partials of (r, z) and, when qdot is given, of (q, z) (vectors in their own
+// order) go to part[chunk] and part[nchunks + chunk].
+template <typename TV, typename TS, typename TF, bool CONTIG>
 __global__ __launch_bounds__(kBlock) void k_sweep_upper(const int* __restrict__ chunks, const int* __restrict__ rows,
                                                         const int* __restrict__ rowptr, const int* __restrict__ col,
-                                                        const int* __restrict__ diag, const T* __restrict__ f,
-                                                        const int* __restrict__ perm, T* sy, T* sw, T* out, int* err,
+                                                        const int* __restrict__ diag, const TS* __restrict__ f,
+                                                        const int* __restrict__ perm, TF* sy, TF* sw, TV* out,
+                                                        const TV* __restrict__ rdot, double* __restrict__ part, int* err,
                                                         int limit)
 {
+    // (r, z) partial without a workgroup barrier after the sweep: each wave leaves its sum in LDS and the
+    // last wave to arrive adds the four in wave order (the order block_sum uses)
+    __shared__ double wsum[kWaves];
+    __shared__ unsigned arrived;
+    if (part) {
+        if (threadIdx.x == 0) arrived = 0u;
+        __syncthreads();
+    }
     const int c0 = chunks[2 * blockIdx.x], cnt = chunks[2 * blockIdx.x + 1];
-    if ((int)threadIdx.x >= cnt) return;
-    const int i = CONTIG ? c0 + (int)threadIdx.x : rows[c0 + threadIdx.x];
-    const int di = diag[i];
-    const T yi = sy[i];
-    sy[i] = sentinel_value<T>();
-    T s = dep_sum(yi, di + 1, rowptr[i + 1], col, f, sw, err, limit);
-    s = s / f[di];
-    publish(sw + i, s);
-    out[perm ? perm[i] : i] = s;
+    double v = 0.0;
+    if ((int)threadIdx.x < cnt) {
+        const int i = CONTIG ? c0 + (int)threadIdx.x : rows[c0 + threadIdx.x];
+        const int di = diag[i];
+        const TF yi = sy[i];
+        sy[i] = sentinel_value<TF>();
+        TF s = dep_sum(yi, di + 1, rowptr[i + 1], col, f, sw, err, limit);
+        s = s / (TF)f[di];
+        publish(sw + i, s);
+        const int o = perm ? perm[i] : i;
+        const TV z = (TV)s;
+        out[o] = z;
+        if (part) v = (double)(rdot[o] * z);
+    }
+    if (!part) return;
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) {
+        wsum[threadIdx.x >> 6] = v;
+        __threadfence_block();
+        if (atomicAdd(&arrived, 1u) == kWaves - 1) {
+            __threadfence_block();
+            double t = wsum[0];
+            for (int w = 1; w < kWaves; ++w) t += wsum[w];
+            part[blockIdx.x] = t;
+        }
+    }
 }
 
 template <typename T>
@@ -459,30 +489,14 @@ void precond_initialize(Context* c, pfem_precond_t* pc, pfem_csr_t* a)
     PFEM_HIP(hipGetLastError());
 }
 
-// Both triangular sweeps. Natural ordering: in place on out (src may equal out). Multicolour: the
-// forward sweep gathers src through the permutation into work, the backward sweep scatters into out.
+// Per-level launches (TV == TF only). Natural ordering: in place on out (src may equal out). Multicolour:
+// the forward pass gathers src through the permutation into work, the backward pass scatters into out.
 template <typename T>
-static void tri_solves(Context* c, pfem_precond_t* pc, const T* src, T* work, T* out, const T* f, T* sy, T* sw)
+static void level_solves(Context* c, pfem_precond_t* pc, const T* src, T* work, T* out, const T* f)
 {
     const Pattern& F = *pc->fpat;
     const bool ic = pc->kind == PFEM_PRECOND_IC0;
     const int* perm = pc->ordering == PFEM_ORDER_NATURAL ? nullptr : pc->perm.p;
-    if (pc->sweep) {
-        const dim3 gl(pc->nlchunks), gu(pc->nuchunks), bl(kBlock);
-        const int* lc = pc->lchunks.p;
-        const int* uc = pc->uchunks.p;
-        int* err = pc->sweep_err.p;
-        if (pc->contiguous) {
-            if (ic) hipLaunchKernelGGL((k_sweep_lower<T, false, true>), gl, bl, 0, c->stream, lc, pc->lrows.p, F.rowptr.p, F.col.p, F.diag.p, f, perm, src, sy, sw, err, kSpinLimit);
-            else hipLaunchKernelGGL((k_sweep_lower<T, true, true>), gl, bl, 0, c->stream, lc, pc->lrows.p, F.rowptr.p, F.col.p, F.diag.p, f, perm, src, sy, sw, err, kSpinLimit);
-            hipLaunchKernelGGL((k_sweep_upper<T, true>), gu, bl, 0, c->stream, uc, pc->urows.p, F.rowptr.p, F.col.p, F.diag.p, f, perm, sy, sw, out, err, kSpinLimit);
-        } else {
-            if (ic) hipLaunchKernelGGL((k_sweep_lower<T, false, false>), gl, bl, 0, c->stream, lc, pc->lrows.p, F.rowptr.p, F.col.p, F.diag.p, f, perm, src, sy, sw, err, kSpinLimit);
-            else hipLaunchKernelGGL((k_sweep_lower<T, true, false>), gl, bl, 0, c->stream, lc, pc->lrows.p, F.rowptr.p, F.col.p, F.diag.p, f, perm, src, sy, sw, err, kSpinLimit);
-            hipLaunchKernelGGL((k_sweep_upper<T, false>), gu, bl, 0, c->stream, uc, pc->urows.p, F.rowptr.p, F.col.p, F.diag.p, f, perm, sy, sw, out, err, kSpinLimit);
-        }
-        return;
-    }
     T* y = perm ? work : out;
     for (size_t l = 0; l + 1 < pc->h_lptr.size(); ++l) {
         const int cnt = pc->h_lptr[l + 1] - pc->h_lptr[l];
@@ -502,30 +516,77 @@ static void tri_solves(Context* c, pfem_precond_t* pc, const T* src, T* work, T*
     }
 }
 
-void precond_apply(Context* c, pfem_precond_t* pc, double* z, const double* r)
+// The two single-pass sweeps; part != null also produces the (r, z) [and (q, z)] partials, one per
+// backward chunk.
+template <typename TV, typename TS, typename TF>
+static void sweeps(Context* c, pfem_precond_t* pc, const TV* src, TV* out, const TS* f, TF* sy, TF* sw, double* part)
 {
+    const Pattern& F = *pc->fpat;
+    const bool ic = pc->kind == PFEM_PRECOND_IC0;
+    const int* perm = pc->ordering == PFEM_ORDER_NATURAL ? nullptr : pc->perm.p;
+    const dim3 gl(pc->nlchunks), gu(pc->nuchunks), bl(kBlock);
+    const int* lc = pc->lchunks.p;
+    const int* uc = pc->uchunks.p;
+    int* err = pc->sweep_err.p;
+    if (pc->contiguous) {
+        if (ic) hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, false, true>), gl, bl, 0, c->stream, lc, pc->lrows.p, F.rowptr.p, F.col.p, F.diag.p, f, perm, src, sy, sw, err, kSpinLimit);
+        else hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, true, true>), gl, bl, 0, c->stream, lc, pc->lrows.p, F.rowptr.p, F.col.p, F.diag.p, f, perm, src, sy, sw, err, kSpinLimit);
+        hipLaunchKernelGGL((k_sweep_upper<TV, TS, TF, true>), gu, bl, 0, c->stream, uc, pc->urows.p, F.rowptr.p, F.col.p, F.diag.p, f, perm, sy, sw, out, src, part, err, kSpinLimit);
+    } else {
+        if (ic) hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, false, false>), gl, bl, 0, c->stream, lc, pc->lrows.p, F.rowptr.p, F.col.p, F.diag.p, f, perm, src, sy, sw, err, kSpinLimit);
+        else hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, true, false>), gl, bl, 0, c->stream, lc, pc->lrows.p, F.rowptr.p, F.col.p, F.diag.p, f, perm, src, sy, sw, err, kSpinLimit);
+        hipLaunchKernelGGL((k_sweep_upper<TV, TS, TF, false>), gu, bl, 0, c->stream, uc, pc->urows.p, F.rowptr.p, F.col.p, F.diag.p, f, perm, sy, sw, out, src, part, err, kSpinLimit);
+    }
+}
+
+template <typename TV>
+int precond_apply_t(Context* c, pfem_precond_t* pc, TV* z, const TV* r, bool fp32, const DotFuse<TV>& fuse)
+{
+    constexpr bool f64 = sizeof(TV) == 8;
     const int n = pc->n;
+    // the partials need r intact, so an in-place apply (z == r) does not fuse them
+    const bool fuse_ok = fuse.part && z != r && pc->sweep && pc->nuchunks <= fuse.cap;
+    double* part = fuse_ok ? fuse.part : nullptr;
     switch (pc->kind) {
     case PFEM_PRECOND_NONE:
-        if (z != r) hipLaunchKernelGGL((k_copyT<double>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, (long)n, z, r);
+        if (z != r) hipLaunchKernelGGL((k_copyT<TV>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, (long)n, z, r);
         break;
-    case PFEM_PRECOND_JACOBI: jacobi_apply(c, n, z, r, pc->invd.p); break;
-    default: tri_solves<double>(c, pc, r, pc->tmp.p, z, pc->fac.p, pc->sy.p, pc->sw.p);
+    case PFEM_PRECOND_JACOBI:
+        if constexpr (f64) jacobi_apply(c, n, z, r, pc->invd.p);
+        else jacobi_apply32(c, n, z, r, pc->invd32.p);
+        break;
+    default:
+        if constexpr (f64) {
+            if (fp32) {
+                // factor stored in FP32, solves in FP64: an exact apply of the rounded factor
+                PFEM_REQUIRE(pc->fac32.p && pc->sweep, "FP32 factor not enabled (or per-level apply selected)");
+                sweeps<double, float, double>(c, pc, r, z, pc->fac32.p, pc->sy.p, pc->sw.p, part);
+            } else if (pc->sweep) {
+                sweeps<double, double, double>(c, pc, r, z, pc->fac.p, pc->sy.p, pc->sw.p, part);
+            } else {
+                level_solves<double>(c, pc, r, pc->tmp.p, z, pc->fac.p);
+            }
+        } else {
+            PFEM_REQUIRE(pc->fac32.p, "FP32 factor not enabled");
+            if (pc->sweep) sweeps<float, float, float>(c, pc, r, z, pc->fac32.p, pc->sy32.p, pc->sw32.p, part);
+            else level_solves<float>(c, pc, r, pc->tmp32.p, z, pc->fac32.p);
+        }
     }
     PFEM_HIP(hipGetLastError());
+    return part && (pc->kind == PFEM_PRECOND_IC0 || pc->kind == PFEM_PRECOND_ILU0) ? pc->nuchunks : 0;
+}
+
+template int precond_apply_t<double>(Context*, pfem_precond_t*, double*, const double*, bool, const DotFuse<double>&);
+template int precond_apply_t<float>(Context*, pfem_precond_t*, float*, const float*, bool, const DotFuse<float>&);
+
+void precond_apply(Context* c, pfem_precond_t* pc, double* z, const double* r)
+{
+    precond_apply_t<double>(c, pc, z, r, false, DotFuse<double>{});
 }
 
 void precond_apply32(Context* c, pfem_precond_t* pc, float* z, const float* r)
 {
-    const int n = pc->n;
-    switch (pc->kind) {
-    case PFEM_PRECOND_NONE:
-        if (z != r) hipLaunchKernelGGL((k_copyT<float>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, (long)n, z, r);
-        break;
-    case PFEM_PRECOND_JACOBI: jacobi_apply32(c, n, z, r, pc->invd32.p); break;
-    default: tri_solves<float>(c, pc, r, pc->tmp32.p, z, pc->fac32.p, pc->sy32.p, pc->sw32.p);
-    }
-    PFEM_HIP(hipGetLastError());
+    precond_apply_t<float>(c, pc, z, r, true, DotFuse<float>{});
 }
 
 void precond_enable_fp32(Context* c, pfem_precond_t* pc)

@@ -11,11 +11,12 @@ constexpr int kMaxRestart = 1024;   // largest GMRES cycle (the reference's maxI
 constexpr int kVecGrid = 1024;      // grid of the vector kernels (grid-stride loops)
 
 // Device-resident PCG scalars, (re)written by the host at the start of each solve. gam[k & 1] holds
-// (r_k, z_k): iteration k reads one slot while its last kernel writes the other.
+// (r_k, z_k): iteration k reads one slot while its last kernel writes the other; alpha[k & 1] is alpha_k
+// (recorded for diagnostics).
 struct CgState {
     double tol;
     int tol_mode, maxit;
-    double gam[2], thr, res;
+    double gam[2], alpha[2], thr, res;
     int it, done, converged, nan;
 };
 
@@ -138,6 +139,7 @@ struct pfem_solver_t {
     // producers: 2 values, or one per GMRES coefficient, x kVecGrid), pn (||w||^2 of an Arnoldi step)
     pfem::DevArray<double> partials;
     double *pa = nullptr, *pb = nullptr, *pn = nullptr;
+    int pb_cap = 0;
     pfem::DevArray<pfem::CgState> cg;
     pfem::DevArray<pfem::GmState> gm;
     pfem::CgState* h_cg = nullptr;   // pinned
@@ -152,6 +154,19 @@ void precond_initialize(Context* c, pfem_precond_t* pc, pfem_csr_t* a);
 void precond_apply(Context* c, pfem_precond_t* pc, double* z, const double* r);
 void precond_apply32(Context* c, pfem_precond_t* pc, float* z, const float* r);
 void precond_check(Context* c, pfem_precond_t* pc);
+// Dot product an IC0/ILU0 apply can produce on the way: per-chunk partials of (r, z), written to
+// part[0..m) (m = returned count, at most cap).
+template <typename TV>
+struct DotFuse {
+    double* part = nullptr;
+    int cap = 0;
+};
+// z = M^-1 r with vectors of type TV. fp32 (TV = double): IC0/ILU0 with the factor stored in FP32 and the
+// solves computed in FP64. Returns
+// the number of fused partials written (0 if none: single-pass sweeps, z != r and a large enough cap are
+// needed).
+template <typename TV>
+int precond_apply_t(Context* c, pfem_precond_t* pc, TV* z, const TV* r, bool fp32, const DotFuse<TV>& fuse);
 void jacobi_setup(Context* c, const Pattern& P, const double* val, double* invd, float* invd32);
 void jacobi_apply(Context* c, int n, double* dest, const double* src, const double* invd);
 void jacobi_apply32(Context* c, int n, float* dest, const float* src, const float* invd);

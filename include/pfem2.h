@@ -161,6 +161,10 @@ int pfem_precond_set_sweep(pfem_precond pc, int single_pass);
 #define PFEM_TOL_RELATIVE 1  /* CG (r, M^-1 r) < tol^2 (b, M^-1 b); GMRES |beta| < tol ||M^-1 b|| */
 #define PFEM_PRECISION_FP64 0
 #define PFEM_PRECISION_MIXED 1 /* FP32 inner Krylov + FP64 iterative refinement */
+/* FP64 Krylov whose IC0/ILU0 factor is stored in FP32 (the triangular solves compute in FP64). The
+ * refinement of MIXED contracts by about cond(A) * 2^-24 per pass and stalls on the pressure Poisson
+ * matrices, whose condition number is of that order. */
+#define PFEM_PRECISION_MIXED_PRECOND 2
 typedef struct {
     int converged;
     int iterations;        /* Krylov iterations (for MIXED: inner iterations summed) */
@@ -273,7 +277,9 @@ typedef struct {
     int dirichlet_mode;
     int body_id;               /* -1: no loads */
     double body_cx, body_cy, force_coeff;
-    int precision;             /* PFEM_PRECISION_FP64 or PFEM_PRECISION_MIXED */
+    int precision;             /* PFEM_PRECISION_FP64, or PFEM_PRECISION_MIXED: solves with Jacobi/no
+                                  preconditioner run FP32-inner/FP64-refinement, IC0/ILU0 solves run
+                                  PFEM_PRECISION_MIXED_PRECOND */
     double inner_tol;          /* MIXED: relative tolerance of each FP32 inner solve */
     int pres_ordering;         /* PFEM_ORDER_* for an IC0/ILU0 pressure preconditioner */
 } pfem_step_config;
@@ -303,7 +309,8 @@ int pfem_pfem2_precond(pfem_pfem2 s, int which, pfem_precond* out);
 /* Average device time of one launch, measured with HIP events around `launches` back-to-back
  * launches on the context stream (no counterpart in the reference, whose GpuTimer times phases,
  * src/common/gpu_timer.cuh:14-98). op 0: FP64 SpMV of a; op 1: FP32 SpMV of a (values converted
- * first); op 2: FP64 apply of pc (initialised from a first); op 3: the FP32 apply of pc. */
+ * first); op 2: FP64 apply of pc (initialised from a first); op 3: the FP32 apply of pc; op 4: the apply
+ * to FP64 vectors with the FP32 factor (PFEM_PRECISION_MIXED_PRECOND). */
 int pfem_time_kernel(int op, pfem_csr a, pfem_precond pc, int launches, double* avg_ms);
 
 /* ------------------------------------------------------------------ export (src/data_export.cuh) */

@@ -325,6 +325,31 @@ def test_mixed_precision_refinement(ctx, oracle, kind, precond):
     assert rel_l2(x.download(), exact) < 1e-9
 
 
+@pytest.mark.parametrize("kind,precond,ordering", [(0, 2, 1), (0, 2, 0), (1, 3, 1)])
+def test_mixed_factor_precision(ctx, oracle, kind, precond, ordering):
+    """FP64 Krylov with the IC0/ILU0 factor in FP32 reaches the FP64 tolerance in about the FP64 solver's
+    iteration count."""
+    from gpupfem2_amd import capi
+    xy, m, a, rhs, rp, col, val = bc_system(ctx, oracle)
+    its = []
+    xs = []
+    for prec in (capi.PRECISION_FP64, capi.PRECISION_MIXED_PRECOND):
+        pc = capi.Preconditioner(a, precond, ordering)
+        s = capi.Solver(a, kind, 1e-12, 5000, 30, capi.TOL_RELATIVE, pc)
+        if prec != capi.PRECISION_FP64:
+            s.set_precision(prec, 1e-5, 20)
+        x = ctx.zeros(xy.shape[0])
+        st = s.solve(a, x, rhs)
+        assert st.converged and st.refinements == 0
+        its.append(st.iterations)
+        xs.append(x.download())
+    assert abs(its[1] - its[0]) <= max(3, its[0] // 10)
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    exact = spla.spsolve(sp.csr_matrix((val, col, rp)).tocsc(), rhs.download())
+    assert rel_l2(xs[1], exact) < 1e-9 and rel_l2(xs[0], exact) < 1e-9
+
+
 def test_solver_failure_paths(ctx, oracle):
     """maxIterations reached -> converged = 0 (the reference returns false); GMRES without restart leaves
     x untouched."""
