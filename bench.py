@@ -4,11 +4,16 @@ dt = 2e-4, two outer iterations, pressure PCG + IC0, prediction GMRES(30) + Jaco
 PCG + Jacobi, relative tolerance 1e-12), in the mixed-precision pipeline and in plain FP64.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
+    python bench.py --kernels      # only the pressure-solve kernels (for counter collection)
 
 One step = advection + relocation + refill + reorder of all particles, projection, the two outer
 iterations of prediction / pressure / correction (5 assemblies + 5 solves each), particle velocity
 correction and the drag/lift integral. The flow starts from rest; W warm-up steps, then K timed
 steps bracketed by device synchronisation. Prints one JSON line.
+
+Mixed precision: Jacobi-preconditioned solves run FP32 inner Krylov + FP64 refinement; the IC0
+pressure solve runs FP64 PCG with the IC0 factor stored in FP32 (FP32 refinement stalls on the
+pressure matrix, whose condition number is ~1/eps32).
 """
 import argparse
 import json
@@ -23,7 +28,7 @@ HBM_PEAK_GBS = 8000.0
 
 
 def gpu_run(ctx, case, precision, steps, warmup, inner_tol):
-    from gpupfem2_amd import capi, cases
+    from gpupfem2_amd import cases
     t0 = time.perf_counter()
     mesh, ph, st = cases.build_gpu(ctx, case, precision=precision, inner_tol=inner_tol)
     ctx.sync()
@@ -43,23 +48,70 @@ def gpu_run(ctx, case, precision, steps, warmup, inner_tol):
     return out, (mesh, ph, st)
 
 
-def roofline_spmv(ctx, st, precision):
-    """Pressure-matrix SpMV: the kernel every Krylov iteration of the dominant solve runs."""
+def csr_info(a):
     from gpupfem2_amd import capi
-    a = st.matrix_handle(2)
     n, nnz = capi.C.c_int(), capi.C.c_int()
     capi.lib().pfem_csr_info(a, capi.C.byref(n), capi.C.byref(nnz))
-    n, nnz = n.value, nnz.value
-    op = 1 if precision == 1 else 0
-    sv = 4 if op == 1 else 8
-    ms = capi.time_kernel(ctx, op, a, None, launches=200)
-    # algorithmic bytes: values + column indices + row offsets + x + y
-    bytes_ = nnz * (sv + 4) + 4 * (n + 1) + n * sv + n * sv
-    gbs = bytes_ / (ms * 1e-3) / 1e9
+    return n.value, nnz.value
+
+
+def ic0_apply_bytes(n, nnz, factor_bytes, vec_bytes=8):
+    """Algorithmic bytes of one IC0 apply (both triangular solves): the factor's two triangles with
+    their diagonals and column indices, the row offsets twice, and four vector passes (r read, y
+    written and read, z written)."""
+    return (nnz + n) * (factor_bytes + 4) + 4 * (n + 1) * 2 + 4 * n * vec_bytes
+
+
+def spmv_bytes(n, nnz, val_bytes, vec_bytes):
+    return nnz * (val_bytes + 4) + 4 * (n + 1) + 2 * n * vec_bytes
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "ic0_apply_cylinder2M.md")
+
+
+def measured_traffic(key):
+    """HBM bytes per launch from the committed counter summary (FETCH_SIZE + WRITE_SIZE runs of
+    `bench.py --kernels`, each scaled by what the counter reads for a 256 MiB copy); None if absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            for line in fh:
+                if line.startswith(key + " = "):
+                    return int(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
+def roofline_ic0(ctx, st, precision):
+    """The dominant kernels of a step: the two single-pass triangular sweeps of the pressure IC0 apply
+    (k_sweep_lower + k_sweep_upper, about 55% of the step), timed with HIP events on the context
+    stream over 200 applies."""
+    from gpupfem2_amd import capi
+    a, pc = st.matrix_handle(2), st.precond_handle(1)
+    n, nnz = csr_info(a)
+    op = 4 if precision == 1 else 2
+    ms = capi.time_kernel(ctx, op, a, pc, launches=200)
+    b = ic0_apply_bytes(n, nnz, 4 if op == 4 else 8)
+    gbs = b / (ms * 1e-3) / 1e9
+    traffic = measured_traffic("traffic_bytes_ic0_apply_fp32_factor" if op == 4 else "traffic_bytes_ic0_apply_fp64")
     return dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None,
-                kernel=f"k_spmv<{'float' if op else 'double'}> pressure matrix n={n} nnz={nnz}",
-                avg_us=round(ms * 1e3, 2), bytes_per_launch=bytes_)
+                frac=round(gbs / HBM_PEAK_GBS, 4), traffic=traffic,
+                traffic_source="profiles/ic0_apply_cylinder2M.md" if traffic else None,
+                kernel=(f"IC0 apply = k_sweep_lower + k_sweep_upper, pressure matrix n={n} nnz={nnz}, "
+                        f"factor {'FP32' if op == 4 else 'FP64'}, vectors FP64"),
+                avg_us=round(ms * 1e3, 2), bytes_per_launch=b)
+
+
+def roofline_spmv(ctx, st):
+    """FP64 SpMV of the pressure matrix (the SpMV of every pressure PCG iteration)."""
+    from gpupfem2_amd import capi
+    a = st.matrix_handle(2)
+    n, nnz = csr_info(a)
+    ms = capi.time_kernel(ctx, 0, a, None, launches=200)
+    b = spmv_bytes(n, nnz, 8, 8)
+    gbs = b / (ms * 1e-3) / 1e9
+    return dict(kernel=f"k_spmv<double> pressure matrix n={n} nnz={nnz}", achieved=round(gbs, 1),
+                frac=round(gbs / HBM_PEAK_GBS, 4), avg_us=round(ms * 1e3, 2), bytes_per_launch=b)
 
 
 def cpu_baseline(case, st, mesh, gpu_its):
@@ -106,9 +158,9 @@ def cpu_baseline(case, st, mesh, gpu_its):
         return max(it, 1e-9), max((tb - ta) - lo * it, 0.0)
 
     x = np.zeros(nv)
-    pres_it, pres_setup = per_iter(lambda k: O.pcg(rp, col, vp, x * 0, rpv, 1e-300, k, 1, "ic0"), 5, 25)
-    corr_it, corr_setup = per_iter(lambda k: O.pcg(rp, col, vc, x * 0, c0, 1e-300, k, 1, "jacobi"), 5, 25)
-    pred_it, pred_setup = per_iter(lambda k: O.gmres(rp, col, v0, x * 0, r0, 1e-300, k, 30, 1, "jacobi"), 5, 25)
+    pres_it, pres_setup = per_iter(lambda k: O.pcg(rp, col, vp, x * 0, rpv, 1e-300, k, 1, "ic0"), 10, 60)
+    corr_it, corr_setup = per_iter(lambda k: O.pcg(rp, col, vc, x * 0, c0, 1e-300, k, 1, "jacobi"), 10, 60)
+    pred_it, pred_setup = per_iter(lambda k: O.gmres(rp, col, v0, x * 0, r0, 1e-300, k, 30, 1, "jacobi"), 10, 60)
     its = np.asarray(gpu_its[-1]).reshape(case.outer_iterations, 5)
     t_solve = (its[:, 0:2].sum() * pred_it + 2 * case.outer_iterations * pred_setup +
                its[:, 2].sum() * pres_it + case.outer_iterations * pres_setup +
@@ -116,11 +168,46 @@ def cpu_baseline(case, st, mesh, gpu_its):
     t_step = t_part + case.outer_iterations * (2 * t_pred_asm + t_pres_asm + t_corr_asm) + t_solve
     return dict(value=round(1.0 / t_step, 6), unit="steps/s", cores=O.threads(), kind="port",
                 sample=(f"oracle (1 thread, FP64) on the full {case.tri.shape[0]}-triangle case: one particle "
-                        f"advect+project+correct ({t_part:.1f} s), one assembly of each system, and 20 "
+                        f"advect+project+correct ({t_part:.1f} s), one assembly of each system, and 50 "
                         f"Krylov iterations per solver type (IC0-PCG {pres_it * 1e3:.1f} ms/it, Jacobi-PCG "
                         f"{corr_it * 1e3:.1f} ms/it, GMRES(30) {pred_it * 1e3:.1f} ms/it), combined with the "
                         f"GPU FP64 step's iteration counts; measured in {time.perf_counter() - T0:.0f} s"),
                 seconds_per_step=round(t_step, 2))
+
+
+def kernels_only(h, launches):
+    """Pressure-system kernels alone: the cylinder mesh's P1 stiffness matrix (the pressure matrix:
+    it depends on the geometry only) with the pressure Dirichlet rows, its multicolour IC0 factor, then
+    `launches` IC0 applies (FP32-stored factor, then FP64), FP64 SpMVs, and device-to-device copies of
+    256 MiB (the calibration of the byte counters)."""
+    import numpy as np
+    from gpupfem2_amd import capi, cases
+    case = cases.cylinder_case(h=h)
+    ctx = capi.Context(0)
+    m = capi.Mesh2D(ctx, case.xy, case.tri)
+    a = capi.CSR(ctx, mesh=m)
+    rhs = ctx.zeros(case.xy.shape[0])
+    capi.assemble_poisson2d(m, a, rhs, coeff=1.0, rhs_kind=2, rhs_param=0.0)
+    capi.Dirichlet(ctx, case.xy.shape[0], case.bcp_node, case.bcp_val).apply(a, rhs, capi.DIRICHLET_MASK)
+    pc = capi.Preconditioner(a, capi.PRECOND_IC0, capi.ORDER_MULTICOLOR)
+    pc.initialize(a)
+    n, nnz = a.n, a.nnz
+    print(f"[kernels] n={n} nnz={nnz}", file=sys.stderr, flush=True)
+    res = {}
+    for op, name in ((4, "ic0_apply_fp32_factor"), (2, "ic0_apply_fp64"), (0, "spmv_fp64")):
+        ms = capi.time_kernel(ctx, op, a.h, pc.h if op else None, launches=launches)
+        res[name] = round(ms * 1e3, 2)
+        print(f"[kernels] {name}: {ms * 1e3:.2f} us", file=sys.stderr, flush=True)
+    mib = 256
+    src, dst = ctx.zeros(mib << 17), ctx.empty(mib << 17)
+    for _ in range(launches):
+        dst.copy_from(src)
+    ctx.sync()
+    print(f"[kernels] copied {launches} x {mib} MiB", file=sys.stderr, flush=True)
+    res["copy_bytes_read"] = mib << 20
+    res["ic0_bytes_fp32_factor"] = ic0_apply_bytes(n, nnz, 4)
+    res["ic0_bytes_fp64"] = ic0_apply_bytes(n, nnz, 8)
+    print(json.dumps(res))
 
 
 def main():
@@ -132,14 +219,19 @@ def main():
     ap.add_argument("--mode", choices=["both", "fp64", "mixed"], default="both")
     ap.add_argument("--inner-tol", type=float, default=1e-5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernels", action="store_true", help="pressure-solve kernels only (counter runs)")
+    ap.add_argument("--launches", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)")
+    if args.kernels:
+        kernels_only(args.h, args.launches)
+        return
     from gpupfem2_amd import capi, cases
     case = cases.cylinder_case(h=args.h)
     ctx = capi.Context(0)
     res = {}
-    roof = None
+    roof = roof_spmv = None
     cpu = None
     for prec in ([0, 1] if args.mode == "both" else [0] if args.mode == "fp64" else [1]):
         print(f"[bench] precision {'mixed' if prec else 'fp64'}: {case.tri.shape[0]} triangles, "
@@ -149,7 +241,10 @@ def main():
         print(f"[bench] {r['steps_per_s']:.3f} steps/s, iterations {r['iterations'][-1]}", file=sys.stderr,
               flush=True)
         if prec == (1 if args.mode != "fp64" else 0):
-            roof = roofline_spmv(ctx, st, prec)
+            roof = roofline_ic0(ctx, st, prec)
+            roof_spmv = roofline_spmv(ctx, st)
+            print(f"[bench] IC0 apply {roof['avg_us']} us ({roof['achieved']} GB/s), SpMV {roof_spmv['avg_us']} us",
+                  file=sys.stderr, flush=True)
         if prec == 0 and not args.no_cpu:
             print("[bench] cpu baseline (oracle)", file=sys.stderr, flush=True)
             cpu = cpu_baseline(case, st, mesh, r["iterations"])
@@ -166,17 +261,19 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32+fp64" if 1 in res else "f64",
+        "dtype": "f64+f32" if 1 in res else "f64",
         "data": "synthetic (generated mesh, Philox-seeded particles, flow from rest)",
         "config": {"workload": "cylinder Re=100, scheme B, dt=2e-4", "triangles": int(case.tri.shape[0]),
                    "nodes": int(case.xy.shape[0]), "particles_per_cell": case.ppc,
-                   "particles_after_steps": head["particles"], "parallelism": "1 GPU"},
+                   "particles_after_steps": head["particles"], "parallelism": "1 GPU",
+                   "precision": "mixed" if 1 in res else "fp64"},
         "fp64_steps_per_s": round(res[0]["steps_per_s"], 4) if 0 in res else None,
         "mixed_steps_per_s": round(res[1]["steps_per_s"], 4) if 1 in res else None,
         "mixed_over_fp64": round(res[1]["steps_per_s"] / res[0]["steps_per_s"], 3) if len(res) == 2 else None,
         "krylov_iterations_last_step": {("mixed" if k else "fp64"): v["iterations"][-1] for k, v in res.items()},
         "cd_cl_last_step": {("mixed" if k else "fp64"): v["cd_cl"] for k, v in res.items()},
         "roofline": roof,
+        "roofline_spmv": roof_spmv,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
