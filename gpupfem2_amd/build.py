@@ -1,7 +1,7 @@
 """Build recipe of libpfem2_hip.so (hipcc, gfx950) and of the CPU oracle.
 
-    python -m gpupfem2_amd.build          # library + oracle
-    python -m gpupfem2_amd.build --lib    # library only
+    python -m gpupfem2_amd.build          # library + C++ examples + oracle
+    python -m gpupfem2_amd.build --lib    # library + C++ examples
 
 Every .hip file under csrc/ is compiled separately (in parallel) into csrc/../_build/*.o and linked
 into gpupfem2_amd/libpfem2_hip.so. -ffp-contract=off keeps every floating-point expression rounded
@@ -62,6 +62,29 @@ def build_lib(jobs=None):
     return LIB
 
 
+EXAMPLES = os.path.join(HERE, "examples")
+CXX = os.environ.get("CXX", "g++")
+
+
+def build_examples():
+    """Host-only C++ drivers written against include/gpupfem2.hpp, linked with libpfem2_hip.so (found
+    next to the binary's directory at run time)."""
+    out = []
+    for src in sorted(glob.glob(os.path.join(EXAMPLES, "*.cpp"))):
+        exe = src[:-4]
+        cmd = [CXX, "-O2", "-std=c++17", "-Wall", "-Wextra", "-I" + os.path.join(ROOT, "include"), src,
+               "-L" + HERE, "-lpfem2_hip", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib", "-o", exe + ".tmp"]
+        p = subprocess.run(cmd, capture_output=True, text=True)
+        if p.returncode != 0:
+            print(p.stdout + p.stderr, file=sys.stderr)
+            raise RuntimeError(f"example build failed: {os.path.basename(src)}")
+        if p.stderr.strip():
+            print(p.stderr, file=sys.stderr)
+        os.replace(exe + ".tmp", exe)
+        out.append(exe)
+    return out
+
+
 def build_oracle():
     p = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], capture_output=True, text=True)
     if p.returncode != 0:
@@ -73,5 +96,6 @@ def build_oracle():
 if __name__ == "__main__":
     only_lib = "--lib" in sys.argv
     print(build_lib())
+    print("\n".join(build_examples()))
     if not only_lib:
         print(build_oracle())

@@ -103,6 +103,8 @@ SIGNATURES = {
     "pfem_precond_apply": (_i, [_vp, _vp, _vp]),
     "pfem_precond_download": (_i, [_vp, _vp, C.POINTER(_i), _vp]),
     "pfem_precond_set_sweep": (_i, [_vp, _i]),
+    "pfem_precond_create_custom": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
+    "pfem_mesh2d_device_ptr": (_i, [_vp, _i, _vp]),
     "pfem_solver_create": (_i, [_vp, _i, _d, _i, _i, _i, _vp, _pp]),
     "pfem_solver_set_precision": (_i, [_vp, _i, _d, _i]),
     "pfem_solver_destroy": (_i, [_vp]),

@@ -896,6 +896,8 @@ int pfem_solver_set_precision(pfem_solver s, int precision, double inner_tol, in
             precond_enable_fp32(C(s->ctx), s->pc);
             s->pc->sweep = 1;
         } else if (precision == PFEM_PRECISION_MIXED) {
+            PFEM_REQUIRE(!s->pc || s->pc->kind != PFEM_PRECOND_CUSTOM,
+                         "a custom preconditioner applies to FP64 vectors only");
             if (s->kind == PFEM_SOLVER_GMRES && !s->gm.p) {
                 s->gm.alloc(1);
                 PFEM_HIP(hipMemset(s->gm.p, 0, sizeof(GmState)));

@@ -479,6 +479,23 @@ int pfem_mesh2d_download(pfem_mesh2d m, int what, void* dst)
     });
 }
 
+int pfem_mesh2d_device_ptr(pfem_mesh2d m, int what, void** d_ptr)
+{
+    PFEM_TRY(m ? m->ctx : nullptr, {
+        PFEM_REQUIRE(m && d_ptr, "null argument");
+        switch (what) {
+        case 0: *d_ptr = m->xy.p; break;
+        case 1: *d_ptr = m->tri.p; break;
+        case 2: *d_ptr = m->area.p; break;
+        case 3: *d_ptr = m->invJ.p; break;
+        case 4: *d_ptr = m->nbr_off.p; break;
+        case 5: *d_ptr = m->nbr_idx.p; break;
+        case 6: *d_ptr = m->edge_ids.p; break;
+        default: throw Error(PFEM_EINVAL, "unknown mesh array");
+        }
+    });
+}
+
 int pfem_mesh2d_set_edge_ids(pfem_mesh2d m, const int* ids)
 {
     PFEM_TRY(m ? m->ctx : nullptr, {

@@ -461,6 +461,10 @@ void precond_initialize(Context* c, pfem_precond_t* pc, pfem_csr_t* a)
 {
     PFEM_REQUIRE(a->pat->n == pc->n, "preconditioner and matrix sizes differ");
     if (pc->kind == PFEM_PRECOND_NONE) return;
+    if (pc->kind == PFEM_PRECOND_CUSTOM) {
+        if (pc->custom_init) pc->custom_init(pc->custom_user, a);
+        return;
+    }
     if (pc->kind == PFEM_PRECOND_JACOBI) {
         jacobi_setup(c, *a->pat, a->val.p, pc->invd.p, pc->invd32.p);
         return;
@@ -555,6 +559,10 @@ int precond_apply_t(Context* c, pfem_precond_t* pc, TV* z, const TV* r, bool fp3
         if constexpr (f64) jacobi_apply(c, n, z, r, pc->invd.p);
         else jacobi_apply32(c, n, z, r, pc->invd32.p);
         break;
+    case PFEM_PRECOND_CUSTOM:
+        if constexpr (f64) pc->custom_apply(pc->custom_user, z, r);
+        else PFEM_REQUIRE(false, "a custom preconditioner applies to FP64 vectors only");
+        break;
     default:
         if constexpr (f64) {
             if (fp32) {
@@ -648,6 +656,22 @@ int pfem_precond_create(pfem_csr a, int kind, int ordering, pfem_precond* out)
             build_factor_structure(c, pc.get());
         }
         PFEM_HIP(hipStreamSynchronize(c->stream));
+        *out = pc.release();
+    });
+}
+
+int pfem_precond_create_custom(pfem_ctx ctx, int n, pfem_precond_init_fn init, pfem_precond_apply_fn apply,
+                               void* user, pfem_precond* out)
+{
+    PFEM_TRY(ctx, {
+        PFEM_REQUIRE(ctx && apply && out && n > 0, "bad arguments");
+        auto pc = std::make_unique<pfem_precond_t>();
+        pc->ctx = ctx;
+        pc->kind = PFEM_PRECOND_CUSTOM;
+        pc->n = n;
+        pc->custom_init = init;
+        pc->custom_apply = apply;
+        pc->custom_user = user;
         *out = pc.release();
     });
 }

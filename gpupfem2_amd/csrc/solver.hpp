@@ -91,6 +91,10 @@ __device__ inline void gm_arnoldi_finish(GmState* st, double hn)
 struct pfem_precond_t {
     pfem_ctx ctx = nullptr;
     int kind = PFEM_PRECOND_NONE;
+    // PFEM_PRECOND_CUSTOM: host callbacks that enqueue their work on the context stream
+    pfem_precond_init_fn custom_init = nullptr;
+    pfem_precond_apply_fn custom_apply = nullptr;
+    void* custom_user = nullptr;
     int ordering = PFEM_ORDER_NATURAL;
     int n = 0;
     std::shared_ptr<pfem::Pattern> pat;   // pattern of the matrices it is initialised from

@@ -68,6 +68,9 @@ int pfem_mesh2d_info(pfem_mesh2d m, int* nv, int* nt, int* nt_host, long* n_neig
  * 2 cell area (1 dbl), 3 inverse Jacobian (4 dbl, row-major), 4 neighbour offsets (nt+1 int),
  * 5 neighbour indices (int), 6 edge boundary IDs (3 int per cell), 7 host cells (3 int per host cell) */
 int pfem_mesh2d_download(pfem_mesh2d m, int what, void* dst);
+/* the same arrays in device memory (what 0..6), owned by the mesh: the deviceVector members behind the
+ * getters (getVertices().data, getCells().data, ...) */
+int pfem_mesh2d_device_ptr(pfem_mesh2d m, int what, void** d_ptr);
 /* kSetEdgeBoundaryIDs (cases/Cylinder2D/main.cu:35-70, cases/PoiseuilleFlow2D/main.cu:48-79), with the
  * geometry as parameters: edge midpoints within 2e-6 of x0 / x1 / y0 / y1 get id_x0 / id_x1 / id_y0 /
  * id_y1 (checked in that order); when circle_r > 0, midpoints with | |m - c| - r | < circle_tol get
@@ -135,6 +138,7 @@ int pfem_dirichlet_apply2(pfem_dirichlet bc0, pfem_dirichlet bc1, pfem_csr a, do
 #define PFEM_PRECOND_JACOBI 1
 #define PFEM_PRECOND_IC0 2
 #define PFEM_PRECOND_ILU0 3
+#define PFEM_PRECOND_CUSTOM 4   /* host callbacks (pfem_precond_create_custom) */
 /* ordering for IC0/ILU0: natural (exact reference factor, level-scheduled) or multicolour (the
  * factor of the symmetrically permuted matrix; few levels) */
 #define PFEM_ORDER_NATURAL 0
@@ -142,6 +146,15 @@ int pfem_dirichlet_apply2(pfem_dirichlet bc0, pfem_dirichlet bc1, pfem_csr a, do
 /* PreconditionerJacobi(n, LA) / PreconditionerIC(matrix, LA) / PreconditionerILU(matrix, LA)
  * (preconditioners.cuh:9-90); the pattern of `a` fixes the structure */
 int pfem_precond_create(pfem_csr a, int kind, int ordering, pfem_precond* out);
+/* The Preconditioner plugin point (preconditioners.cuh:9-26): a preconditioner defined by the caller.
+ * init(user, a) runs where Preconditioner::initialize would (the start of every solve,
+ * linear_solver.cu:306-307), apply(user, dest, src) where applyPreconditioner would; dest may equal src.
+ * Both run on the host and must enqueue their device work on the context stream (pfem_ctx_stream). FP64
+ * solves only. */
+typedef void (*pfem_precond_init_fn)(void* user, pfem_csr a);
+typedef void (*pfem_precond_apply_fn)(void* user, double* d_dest, const double* d_src);
+int pfem_precond_create_custom(pfem_ctx ctx, int n, pfem_precond_init_fn init, pfem_precond_apply_fn apply,
+                               void* user, pfem_precond* out);
 int pfem_precond_destroy(pfem_precond pc);
 /* Preconditioner::initialize(const SparseMatrixCSR&) */
 int pfem_precond_initialize(pfem_precond pc, pfem_csr a);
