@@ -31,7 +31,13 @@ def lib():
 
 
 def _p(a):
-    return None if a is None else a.ctypes.data_as(C.c_void_p)
+    """Pointer to an array's data for the C oracle. A strided input (a column slice, say) is copied to
+    contiguous memory first; the pointer keeps the copy alive. Outputs are always allocated contiguous."""
+    if a is None:
+        return None
+    if not a.flags.c_contiguous:
+        a = np.ascontiguousarray(a)
+    return a.ctypes.data_as(C.c_void_p)
 
 
 class SolveStats(C.Structure):
