@@ -89,16 +89,17 @@ def roofline_ic0(ctx, st, precision):
     from gpupfem2_amd import capi
     a, pc = st.matrix_handle(2), st.precond_handle(1)
     n, nnz = csr_info(a)
-    op = 4 if precision == 1 else 2
+    # the apply as the pressure PCG runs it: vectors in the multicolour factor's ordering
+    op = 6 if precision == 1 else 5
     ms = capi.time_kernel(ctx, op, a, pc, launches=200)
-    b = ic0_apply_bytes(n, nnz, 4 if op == 4 else 8)
+    b = ic0_apply_bytes(n, nnz, 4 if op == 6 else 8)
     gbs = b / (ms * 1e-3) / 1e9
-    traffic = measured_traffic("traffic_bytes_ic0_apply_fp32_factor" if op == 4 else "traffic_bytes_ic0_apply_fp64")
+    traffic = measured_traffic("traffic_bytes_ic0_apply_fp32_factor" if op == 6 else "traffic_bytes_ic0_apply_fp64")
     return dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(gbs / HBM_PEAK_GBS, 4), traffic=traffic,
                 traffic_source="profiles/ic0_apply_cylinder2M.md" if traffic else None,
                 kernel=(f"IC0 apply = k_sweep_lower + k_sweep_upper, pressure matrix n={n} nnz={nnz}, "
-                        f"factor {'FP32' if op == 4 else 'FP64'}, vectors FP64"),
+                        f"factor {'FP32' if op == 6 else 'FP64'}, vectors FP64 in factor order"),
                 avg_us=round(ms * 1e3, 2), bytes_per_launch=b)
 
 
@@ -178,7 +179,8 @@ def cpu_baseline(case, st, mesh, gpu_its):
 def kernels_only(h, launches):
     """Pressure-system kernels alone: the cylinder mesh's P1 stiffness matrix (the pressure matrix:
     it depends on the geometry only) with the pressure Dirichlet rows, its multicolour IC0 factor, then
-    `launches` IC0 applies (FP32-stored factor, then FP64), FP64 SpMVs, and device-to-device copies of
+    `launches` IC0 applies on vectors in the factor's ordering (FP32-stored factor, then FP64), FP64 SpMVs,
+    and device-to-device copies of
     256 MiB (the calibration of the byte counters)."""
     import numpy as np
     from gpupfem2_amd import capi, cases
@@ -194,7 +196,7 @@ def kernels_only(h, launches):
     n, nnz = a.n, a.nnz
     print(f"[kernels] n={n} nnz={nnz}", file=sys.stderr, flush=True)
     res = {}
-    for op, name in ((4, "ic0_apply_fp32_factor"), (2, "ic0_apply_fp64"), (0, "spmv_fp64")):
+    for op, name in ((6, "ic0_apply_fp32_factor"), (5, "ic0_apply_fp64"), (0, "spmv_fp64")):
         ms = capi.time_kernel(ctx, op, a.h, pc.h if op else None, launches=launches)
         res[name] = round(ms * 1e3, 2)
         print(f"[kernels] {name}: {ms * 1e3:.2f} us", file=sys.stderr, flush=True)

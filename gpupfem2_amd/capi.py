@@ -18,6 +18,7 @@ ORDER_NATURAL, ORDER_MULTICOLOR = 0, 1
 SOLVER_CG, SOLVER_GMRES = 0, 1
 TOL_ABSOLUTE, TOL_RELATIVE = 0, 1
 PRECISION_FP64, PRECISION_MIXED, PRECISION_MIXED_PRECOND = 0, 1, 2
+SOLVER_OPT_FACTOR_ORDER, SOLVER_OPT_POLL = 1, 2
 
 
 class SolveStats(C.Structure):
@@ -103,6 +104,7 @@ SIGNATURES = {
     "pfem_precond_apply": (_i, [_vp, _vp, _vp]),
     "pfem_precond_download": (_i, [_vp, _vp, C.POINTER(_i), _vp]),
     "pfem_precond_set_sweep": (_i, [_vp, _i]),
+    "pfem_solver_set_option": (_i, [_vp, _i, _i]),
     "pfem_precond_create_custom": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
     "pfem_mesh2d_device_ptr": (_i, [_vp, _i, _vp]),
     "pfem_solver_create": (_i, [_vp, _i, _d, _i, _i, _i, _vp, _pp]),
@@ -452,6 +454,10 @@ class Solver(_Handle):
                                                 precond.h if precond else None, C.byref(h)))
         self.h = h.value
         self.precond = precond
+
+    def set_option(self, option, value):
+        """SOLVER_OPT_FACTOR_ORDER (0/1) or SOLVER_OPT_POLL (iterations between convergence polls)."""
+        self.ctx.check(lib().pfem_solver_set_option(self.h, option, int(value)))
 
     def set_precision(self, precision, inner_tol=0.0, max_refinements=0):
         self.ctx.check(lib().pfem_solver_set_precision(self.h, precision, inner_tol, max_refinements))

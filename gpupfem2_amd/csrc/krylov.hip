@@ -488,6 +488,19 @@ __global__ void k_to32(long n, float* __restrict__ d, const double* __restrict__
     if (i < n) d[i] = (float)s[i];
 }
 
+// to and from a multicolour factor's ordering: d[i] = s[perm[i]] / d[perm[i]] = s[i]
+__global__ void k_perm_gather(int n, const int* __restrict__ perm, const double* __restrict__ s, double* __restrict__ d)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = s[perm[i]];
+}
+
+__global__ void k_perm_scatter(int n, const int* __restrict__ perm, const double* __restrict__ s, double* __restrict__ d)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[perm[i]] = s[i];
+}
+
 // ------------------------------------------------------------------ host drivers
 template <typename T>
 struct KArgs {
@@ -503,12 +516,13 @@ struct KArgs {
     int tol_mode, maxit;
     T *r, *z, *p, *q, *t, *V;
     bool pc_fp32;        // FP64 vectors, IC0/ILU0 factor applied in FP32
+    bool factor_order;   // the system is in the multicolour factor's ordering (P A P^T, P b, P x)
 };
 
 template <typename T>
 static void papply(Context* c, const KArgs<T>& A, T* z, const T* r)
 {
-    precond_apply_t<T>(c, A.pc, z, r, A.pc_fp32, DotFuse<T>{});
+    precond_apply_t<T>(c, A.pc, z, r, A.pc_fp32, ApplyOpts<T>{nullptr, 0, A.factor_order});
 }
 
 template <typename T>
@@ -565,7 +579,7 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
             } else {
                 hipLaunchKernelGGL((k_cg_update_xr<T>), dim3(g), dim3(kBlock), 0, c->stream, n, A.x, A.r, A.p, A.q, st, s->pa, na, k);
                 // the triangular sweeps also produce the (r, z) partials when they can
-                nbk = precond_apply_t<T>(c, A.pc, A.z, A.r, A.pc_fp32, DotFuse<T>{s->pb, s->pb_cap});
+                nbk = precond_apply_t<T>(c, A.pc, A.z, A.r, A.pc_fp32, ApplyOpts<T>{s->pb, s->pb_cap, A.factor_order});
                 if (!nbk) {
                     hipLaunchKernelGGL((k_cg_rz<T>), dim3(g), dim3(kBlock), 0, c->stream, n, A.r, A.z, st, s->pb);
                     nbk = nb;
@@ -688,10 +702,61 @@ static KArgs<double> args64(pfem_solver_t* s, pfem_csr_t* a, double* x, const do
     return A;
 }
 
+// A multicolour IC0/ILU0 with single-pass sweeps: the whole Krylov solve runs on P A P^T, P b, P x (the
+// factor's ordering), so the sweeps read and write contiguously and nothing is permuted per iteration.
+static bool factor_order_eligible(const pfem_solver_t* s)
+{
+    const pfem_precond_t* pc = s->pc;
+    return pc && (pc->kind == PFEM_PRECOND_IC0 || pc->kind == PFEM_PRECOND_ILU0) &&
+           pc->ordering != PFEM_ORDER_NATURAL && pc->sweep && pc->aval.p && s->factor_order;
+}
+
+static KArgs<double> args_factor_order(Context* c, pfem_solver_t* s, double* x, const double* b)
+{
+    pfem_precond_t* pc = s->pc;
+    const int n = pc->n;
+    if (!s->xp.p) {
+        s->xp.alloc(n);
+        s->bp.alloc(n);
+    }
+    hipLaunchKernelGGL(k_perm_gather, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, pc->perm.p, b, s->bp.p);
+    hipLaunchKernelGGL(k_perm_gather, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, pc->perm.p,
+                       (const double*)x, s->xp.p);
+    PFEM_HIP(hipGetLastError());
+    KArgs<double> A{};
+    A.P = pc->fpat.get();
+    A.val = pc->aval.p;
+    A.kind = pc->kind;
+    A.pc = pc;
+    A.x = s->xp.p;
+    A.b = s->bp.p;
+    A.tol = s->tol;
+    A.tol_mode = s->tol_mode;
+    A.maxit = s->maxit;
+    A.r = s->r.p; A.z = s->z.p; A.p = s->p.p; A.q = s->q.p; A.t = s->t.p; A.V = s->V.p;
+    A.pc_fp32 = s->precision == PFEM_PRECISION_MIXED_PRECOND;
+    A.factor_order = true;
+    PFEM_REQUIRE(2 * (size_t)A.P->ntiles <= (size_t)s->pa_cap, "partials too small for the permuted pattern");
+    return A;
+}
+
+static void scatter_back(Context* c, pfem_solver_t* s, double* x)
+{
+    const int n = s->pc->n;
+    hipLaunchKernelGGL(k_perm_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, s->pc->perm.p,
+                       (const double*)s->xp.p, x);
+    PFEM_HIP(hipGetLastError());
+}
+
 void cg_solve_fp64(Context* c, pfem_solver_t* s, pfem_csr_t* a, double* x, const double* b, pfem_solve_stats* out)
 {
     if (s->pc) precond_initialize(c, s->pc, a);
-    cg_run<double>(c, s, args64(s, a, x, b), s->cg.p, s->h_cg);
+    if (factor_order_eligible(s)) {
+        cg_run<double>(c, s, args_factor_order(c, s, x, b), s->cg.p, s->h_cg);
+        scatter_back(c, s, x);
+    } else {
+        cg_run<double>(c, s, args64(s, a, x, b), s->cg.p, s->h_cg);
+    }
     precond_check(c, s->pc);
     if (out) {
         out->converged = s->h_cg->converged;
@@ -705,7 +770,12 @@ void cg_solve_fp64(Context* c, pfem_solver_t* s, pfem_csr_t* a, double* x, const
 void gmres_solve_fp64(Context* c, pfem_solver_t* s, pfem_csr_t* a, double* x, const double* b, pfem_solve_stats* out)
 {
     if (s->pc) precond_initialize(c, s->pc, a);
-    gmres_run<double>(c, s, args64(s, a, x, b), s->gm.p, s->h_gm);
+    if (factor_order_eligible(s)) {
+        gmres_run<double>(c, s, args_factor_order(c, s, x, b), s->gm.p, s->h_gm);
+        scatter_back(c, s, x);
+    } else {
+        gmres_run<double>(c, s, args64(s, a, x, b), s->gm.p, s->h_gm);
+    }
     precond_check(c, s->pc);
     if (out) {
         out->converged = s->h_gm->converged;
@@ -867,7 +937,9 @@ int pfem_solver_create(pfem_csr a, int kind, double tol, int max_iterations, int
         s->cg.alloc(1);
         PFEM_HIP(hipMemset(s->cg.p, 0, sizeof(CgState)));
         PFEM_HIP(hipHostMalloc((void**)&s->h_cg, std::max(sizeof(CgState), (size_t)128), hipHostMallocDefault));
-        const size_t na = 2 * (size_t)std::max(a->pat->ntiles, kVecGrid);
+        int tiles = a->pat->ntiles;
+        if (precond && precond->fpat) tiles = std::max(tiles, precond->fpat->ntiles);
+        const size_t na = 2 * (size_t)std::max(tiles, kVecGrid);
         // pb also takes the per-chunk (r, z) partials of the single-pass sweeps (n/256 + one per level)
         const size_t nb = std::max<size_t>((size_t)std::max(2, s->m + 1) * kVecGrid, blocks_for(n) + 4096);
         s->partials.alloc(na + nb + kVecGrid);
@@ -875,6 +947,7 @@ int pfem_solver_create(pfem_csr a, int kind, double tol, int max_iterations, int
         s->pb = s->pa + na;
         s->pn = s->pb + nb;
         s->pb_cap = (int)nb;
+        s->pa_cap = (int)na;
         *out = s.release();
     });
 }
@@ -908,10 +981,27 @@ int pfem_solver_set_precision(pfem_solver s, int precision, double inner_tol, in
     });
 }
 
+int pfem_solver_set_option(pfem_solver s, int option, int value)
+{
+    PFEM_TRY(s ? s->ctx : nullptr, {
+        PFEM_REQUIRE(s, "null solver");
+        switch (option) {
+        case PFEM_SOLVER_OPT_FACTOR_ORDER: s->factor_order = value ? 1 : 0; break;
+        case PFEM_SOLVER_OPT_POLL:
+            PFEM_REQUIRE(value >= 1 && value <= 4096, "poll interval out of range");
+            s->poll = value;
+            break;
+        default: throw Error(PFEM_EINVAL, "unknown solver option");
+        }
+    });
+}
+
 int pfem_time_kernel(int op, pfem_csr a, pfem_precond pc, int launches, double* avg_ms)
 {
     PFEM_TRY(a ? a->ctx : nullptr, {
-        PFEM_REQUIRE(a && avg_ms && launches > 0 && op >= 0 && op <= 4, "bad arguments");
+        PFEM_REQUIRE(a && avg_ms && launches > 0 && op >= 0 && op <= 6, "bad arguments");
+        PFEM_REQUIRE(op < 5 || (pc && pc->ordering != PFEM_ORDER_NATURAL && pc->sweep),
+                     "ops 5 and 6 need a multicolour preconditioner with single-pass sweeps");
         PFEM_REQUIRE(op < 2 || pc, "ops 2 and 3 need a preconditioner");
         Context* c = C(a->ctx);
         const Pattern& P = *a->pat;
@@ -925,7 +1015,7 @@ int pfem_time_kernel(int op, pfem_csr a, pfem_precond pc, int launches, double* 
             hipLaunchKernelGGL(k_to32, dim3(blocks_for(P.nnz)), dim3(kBlock), 0, c->stream, (long)P.nnz, a->val32.p, a->val.p);
         }
         if (op >= 2) {
-            if (op >= 3) precond_enable_fp32(c, pc);
+            if (op == 3 || op == 4 || op == 6) precond_enable_fp32(c, pc);
             precond_initialize(c, pc, a);
         }
         auto launch = [&]() {
@@ -934,7 +1024,9 @@ int pfem_time_kernel(int op, pfem_csr a, pfem_precond pc, int launches, double* 
             case 1: launch_spmv32(c, P, a->val32.p, x32.p, y32.p); break;
             case 2: precond_apply(c, pc, y.p, x.p); break;
             case 3: precond_apply32(c, pc, y32.p, x32.p); break;
-            default: precond_apply_t<double>(c, pc, y.p, x.p, true, DotFuse<double>{}); break;
+            case 4: precond_apply_t<double>(c, pc, y.p, x.p, true, ApplyOpts<double>{}); break;
+            case 5: precond_apply_t<double>(c, pc, y.p, x.p, false, ApplyOpts<double>{nullptr, 0, true}); break;
+            default: precond_apply_t<double>(c, pc, y.p, x.p, true, ApplyOpts<double>{nullptr, 0, true}); break;
             }
         };
         launch();   // warm-up

@@ -475,6 +475,12 @@ void precond_initialize(Context* c, pfem_precond_t* pc, pfem_csr_t* a)
     const Pattern& F = *pc->fpat;
     hipLaunchKernelGGL(k_gather_values, dim3(blocks_for(F.nnz)), dim3(kBlock), 0, c->stream, (long)F.nnz,
                        pc->src_pos.p, a->val.p, pc->fac.p);
+    if (pc->ordering != PFEM_ORDER_NATURAL) {
+        // A on the permuted pattern, for Krylov iterations run in the factor's ordering
+        if (!pc->aval.p) pc->aval.alloc(F.nnz);
+        hipLaunchKernelGGL((k_copyT<double>), dim3(blocks_for(F.nnz)), dim3(kBlock), 0, c->stream, (long)F.nnz,
+                           pc->aval.p, (const double*)pc->fac.p);
+    }
     for (size_t l = 0; l + 1 < pc->h_lptr.size(); ++l) {
         const int cnt = pc->h_lptr[l + 1] - pc->h_lptr[l];
         if (pc->kind == PFEM_PRECOND_IC0)
@@ -523,11 +529,12 @@ static void level_solves(Context* c, pfem_precond_t* pc, const T* src, T* work, 
 // The two single-pass sweeps; part != null also produces the (r, z) [and (q, z)] partials, one per
 // backward chunk.
 template <typename TV, typename TS, typename TF>
-static void sweeps(Context* c, pfem_precond_t* pc, const TV* src, TV* out, const TS* f, TF* sy, TF* sw, double* part)
+static void sweeps(Context* c, pfem_precond_t* pc, const TV* src, TV* out, const TS* f, TF* sy, TF* sw, double* part,
+                   bool factor_order)
 {
     const Pattern& F = *pc->fpat;
     const bool ic = pc->kind == PFEM_PRECOND_IC0;
-    const int* perm = pc->ordering == PFEM_ORDER_NATURAL ? nullptr : pc->perm.p;
+    const int* perm = pc->ordering == PFEM_ORDER_NATURAL || factor_order ? nullptr : pc->perm.p;
     const dim3 gl(pc->nlchunks), gu(pc->nuchunks), bl(kBlock);
     const int* lc = pc->lchunks.p;
     const int* uc = pc->uchunks.p;
@@ -544,13 +551,15 @@ static void sweeps(Context* c, pfem_precond_t* pc, const TV* src, TV* out, const
 }
 
 template <typename TV>
-int precond_apply_t(Context* c, pfem_precond_t* pc, TV* z, const TV* r, bool fp32, const DotFuse<TV>& fuse)
+int precond_apply_t(Context* c, pfem_precond_t* pc, TV* z, const TV* r, bool fp32, const ApplyOpts<TV>& opt)
 {
     constexpr bool f64 = sizeof(TV) == 8;
     const int n = pc->n;
     // the partials need r intact, so an in-place apply (z == r) does not fuse them
-    const bool fuse_ok = fuse.part && z != r && pc->sweep && pc->nuchunks <= fuse.cap;
-    double* part = fuse_ok ? fuse.part : nullptr;
+    const bool fuse_ok = opt.part && z != r && pc->sweep && pc->nuchunks <= opt.cap;
+    double* part = fuse_ok ? opt.part : nullptr;
+    PFEM_REQUIRE(!opt.factor_order || pc->sweep, "factor-order apply needs the single-pass sweeps");
+    const bool fo = opt.factor_order;
     switch (pc->kind) {
     case PFEM_PRECOND_NONE:
         if (z != r) hipLaunchKernelGGL((k_copyT<TV>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, (long)n, z, r);
@@ -568,15 +577,15 @@ int precond_apply_t(Context* c, pfem_precond_t* pc, TV* z, const TV* r, bool fp3
             if (fp32) {
                 // factor stored in FP32, solves in FP64: an exact apply of the rounded factor
                 PFEM_REQUIRE(pc->fac32.p && pc->sweep, "FP32 factor not enabled (or per-level apply selected)");
-                sweeps<double, float, double>(c, pc, r, z, pc->fac32.p, pc->sy.p, pc->sw.p, part);
+                sweeps<double, float, double>(c, pc, r, z, pc->fac32.p, pc->sy.p, pc->sw.p, part, fo);
             } else if (pc->sweep) {
-                sweeps<double, double, double>(c, pc, r, z, pc->fac.p, pc->sy.p, pc->sw.p, part);
+                sweeps<double, double, double>(c, pc, r, z, pc->fac.p, pc->sy.p, pc->sw.p, part, fo);
             } else {
                 level_solves<double>(c, pc, r, pc->tmp.p, z, pc->fac.p);
             }
         } else {
             PFEM_REQUIRE(pc->fac32.p, "FP32 factor not enabled");
-            if (pc->sweep) sweeps<float, float, float>(c, pc, r, z, pc->fac32.p, pc->sy32.p, pc->sw32.p, part);
+            if (pc->sweep) sweeps<float, float, float>(c, pc, r, z, pc->fac32.p, pc->sy32.p, pc->sw32.p, part, fo);
             else level_solves<float>(c, pc, r, pc->tmp32.p, z, pc->fac32.p);
         }
     }
@@ -584,17 +593,17 @@ int precond_apply_t(Context* c, pfem_precond_t* pc, TV* z, const TV* r, bool fp3
     return part && (pc->kind == PFEM_PRECOND_IC0 || pc->kind == PFEM_PRECOND_ILU0) ? pc->nuchunks : 0;
 }
 
-template int precond_apply_t<double>(Context*, pfem_precond_t*, double*, const double*, bool, const DotFuse<double>&);
-template int precond_apply_t<float>(Context*, pfem_precond_t*, float*, const float*, bool, const DotFuse<float>&);
+template int precond_apply_t<double>(Context*, pfem_precond_t*, double*, const double*, bool, const ApplyOpts<double>&);
+template int precond_apply_t<float>(Context*, pfem_precond_t*, float*, const float*, bool, const ApplyOpts<float>&);
 
 void precond_apply(Context* c, pfem_precond_t* pc, double* z, const double* r)
 {
-    precond_apply_t<double>(c, pc, z, r, false, DotFuse<double>{});
+    precond_apply_t<double>(c, pc, z, r, false, ApplyOpts<double>{});
 }
 
 void precond_apply32(Context* c, pfem_precond_t* pc, float* z, const float* r)
 {
-    precond_apply_t<float>(c, pc, z, r, true, DotFuse<float>{});
+    precond_apply_t<float>(c, pc, z, r, true, ApplyOpts<float>{});
 }
 
 void precond_enable_fp32(Context* c, pfem_precond_t* pc)

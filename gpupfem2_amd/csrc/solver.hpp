@@ -110,6 +110,7 @@ struct pfem_precond_t {
     std::vector<int> h_lptr, h_uptr;      // level offsets into lrows / urows
     pfem::DevArray<double> fac;
     pfem::DevArray<float> fac32;
+    pfem::DevArray<double> aval;          // A's values on the permuted pattern (Krylov in factor order)
     pfem::DevArray<double> tmp, tmp2;
     pfem::DevArray<float> tmp32, tmp32b;
     pfem::DevArray<int> status;           // first failing row (+1), 0 if none
@@ -139,11 +140,13 @@ struct pfem_solver_t {
     int max_refine = 50;
     int n = 0;
     pfem::DevArray<double> r, z, p, q, t, V, H, beta, cs, sn, y, hc;
+    pfem::DevArray<double> xp, bp;   // x and b in a multicolour factor's ordering
     // per-block partial sums of the reductions: pa (SpMV-shaped producers, 2 values x tiles), pb (vector
     // producers: 2 values, or one per GMRES coefficient, x kVecGrid), pn (||w||^2 of an Arnoldi step)
     pfem::DevArray<double> partials;
     double *pa = nullptr, *pb = nullptr, *pn = nullptr;
-    int pb_cap = 0;
+    int pb_cap = 0, pa_cap = 0;
+    int factor_order = 1;   // run multicolour IC0/ILU0 solves in the factor's ordering
     pfem::DevArray<pfem::CgState> cg;
     pfem::DevArray<pfem::GmState> gm;
     pfem::CgState* h_cg = nullptr;   // pinned
@@ -158,19 +161,21 @@ void precond_initialize(Context* c, pfem_precond_t* pc, pfem_csr_t* a);
 void precond_apply(Context* c, pfem_precond_t* pc, double* z, const double* r);
 void precond_apply32(Context* c, pfem_precond_t* pc, float* z, const float* r);
 void precond_check(Context* c, pfem_precond_t* pc);
-// Dot product an IC0/ILU0 apply can produce on the way: per-chunk partials of (r, z), written to
-// part[0..m) (m = returned count, at most cap).
+// Options of an apply. part: the dot product an IC0/ILU0 apply can produce on the way, per-chunk
+// partials of (r, z) written to part[0..m) (m = returned count, at most cap). factor_order: r and z are
+// already in the factor's (multicolour) ordering, so the sweeps do not permute.
 template <typename TV>
-struct DotFuse {
+struct ApplyOpts {
     double* part = nullptr;
     int cap = 0;
+    bool factor_order = false;
 };
 // z = M^-1 r with vectors of type TV. fp32 (TV = double): IC0/ILU0 with the factor stored in FP32 and the
 // solves computed in FP64. Returns
 // the number of fused partials written (0 if none: single-pass sweeps, z != r and a large enough cap are
 // needed).
 template <typename TV>
-int precond_apply_t(Context* c, pfem_precond_t* pc, TV* z, const TV* r, bool fp32, const DotFuse<TV>& fuse);
+int precond_apply_t(Context* c, pfem_precond_t* pc, TV* z, const TV* r, bool fp32, const ApplyOpts<TV>& opt);
 void jacobi_setup(Context* c, const Pattern& P, const double* val, double* invd, float* invd32);
 void jacobi_apply(Context* c, int n, double* dest, const double* src, const double* invd);
 void jacobi_apply32(Context* c, int n, float* dest, const float* src, const float* invd);

@@ -191,6 +191,13 @@ typedef struct {
 int pfem_solver_create(pfem_csr a, int kind, double tol, int max_iterations, int restart, int tol_mode,
                        pfem_precond precond, pfem_solver* out);
 int pfem_solver_set_precision(pfem_solver s, int precision, double inner_tol, int max_refinements);
+/* Solver options (no counterpart in the reference). PFEM_SOLVER_OPT_FACTOR_ORDER (default 1): with a
+ * multicolour IC0/ILU0, run the FP64 Krylov solve on P A P^T, P b, P x (the factor's ordering; results
+ * differ from the natural-order solve only by reduction order). PFEM_SOLVER_OPT_POLL (default 8 for CG,
+ * 4 for GMRES): iterations launched between two host reads of the convergence flag. */
+#define PFEM_SOLVER_OPT_FACTOR_ORDER 1
+#define PFEM_SOLVER_OPT_POLL 2
+int pfem_solver_set_option(pfem_solver s, int option, int value);
 int pfem_solver_destroy(pfem_solver s);
 /* LinearSolver::solve(A, x, b) -> bool (linear_solver.cu:146-154, 301-376, 421-521): warm start from x,
  * the preconditioner is re-initialised from A first. stats may be NULL. */
@@ -323,7 +330,8 @@ int pfem_pfem2_precond(pfem_pfem2 s, int which, pfem_precond* out);
  * launches on the context stream (no counterpart in the reference, whose GpuTimer times phases,
  * src/common/gpu_timer.cuh:14-98). op 0: FP64 SpMV of a; op 1: FP32 SpMV of a (values converted
  * first); op 2: FP64 apply of pc (initialised from a first); op 3: the FP32 apply of pc; op 4: the apply
- * to FP64 vectors with the FP32 factor (PFEM_PRECISION_MIXED_PRECOND). */
+ * to FP64 vectors with the FP32 factor (PFEM_PRECISION_MIXED_PRECOND); ops 5 and 6: ops 2 and 4 on
+ * vectors already in a multicolour factor's ordering (as the Krylov solvers run them). */
 int pfem_time_kernel(int op, pfem_csr a, pfem_precond pc, int launches, double* avg_ms);
 
 /* ------------------------------------------------------------------ export (src/data_export.cuh) */

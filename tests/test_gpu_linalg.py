@@ -350,6 +350,28 @@ def test_mixed_factor_precision(ctx, oracle, kind, precond, ordering):
     assert rel_l2(xs[1], exact) < 1e-9 and rel_l2(xs[0], exact) < 1e-9
 
 
+@pytest.mark.parametrize("kind,precond,prec", [(0, 2, 0), (0, 2, 2), (1, 3, 0)])
+def test_factor_order_solve(ctx, oracle, kind, precond, prec):
+    """A Krylov solve with a multicolour IC0/ILU0 run in the factor's ordering (P A P^T, P b, P x) gives the
+    natural-order solve's solution (reduction order aside) and about its iteration count."""
+    from gpupfem2_amd import capi
+    xy, m, a, rhs, rp, col, val = bc_system(ctx, oracle)
+    out = []
+    for fo in (0, 1):
+        pc = capi.Preconditioner(a, precond, capi.ORDER_MULTICOLOR)
+        s = capi.Solver(a, kind, 1e-12, 5000, 30, capi.TOL_RELATIVE, pc)
+        if prec:
+            s.set_precision(prec, 1e-5, 20)
+        s.set_option(capi.SOLVER_OPT_FACTOR_ORDER, fo)
+        x = ctx.zeros(xy.shape[0])
+        x.upload(np.linspace(0.0, 0.1, xy.shape[0]))      # a warm start exercises the permutation of x
+        st = s.solve(a, x, rhs)
+        assert st.converged
+        out.append((st.iterations, x.download()))
+    assert abs(out[0][0] - out[1][0]) <= max(3, out[0][0] // 20)
+    assert rel_l2(out[1][1], out[0][1]) < 1e-10
+
+
 def test_solver_failure_paths(ctx, oracle):
     """maxIterations reached -> converged = 0 (the reference returns false); GMRES without restart leaves
     x untouched."""
