@@ -158,6 +158,14 @@ __global__ void k_cg_update(int n, T* __restrict__ x, T* __restrict__ r, const T
     block_partials<1>(v, pb);
 }
 
+// alpha_k alone (one block), for the iteration whose x/r update is fused into the forward sweep
+__global__ void k_cg_alpha(CgState* st, const double* __restrict__ pa, int na, int k)
+{
+    if (st->done) return;
+    const double a = cg_alpha(st, pa, na, k);
+    if (threadIdx.x == 0) st->alpha[k & 1] = a;
+}
+
 template <typename T>
 __global__ void k_cg_update_xr(int n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
                                const T* __restrict__ q, CgState* st, const double* __restrict__ pa, int na, int k)
@@ -532,6 +540,9 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
     const int n = P.n;
     const unsigned g = vec_grid(n);
     const bool jac = A.kind == PFEM_PRECOND_JACOBI, none = A.kind == PFEM_PRECOND_NONE;
+    // x/r update fused into the forward sweep: multicolour IC0/ILU0, single-pass sweeps, factor ordering
+    const bool fused = A.factor_order && A.pc && A.pc->sweep && A.pc->contiguous &&
+                       A.pc->nuchunks <= s->pb_cap && s->fuse_xr;
     CgState init{};
     init.tol = A.tol;
     init.tol_mode = A.tol_mode;
@@ -576,6 +587,18 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
                 hipLaunchKernelGGL((k_cg_update<T, true>), dim3(g), dim3(kBlock), 0, c->stream, n, A.x, A.r, A.p, A.q, A.z, A.invd, st, s->pa, na, s->pb, k);
             } else if (none) {
                 hipLaunchKernelGGL((k_cg_update<T, false>), dim3(g), dim3(kBlock), 0, c->stream, n, A.x, A.r, A.p, A.q, A.r, A.invd, st, s->pa, na, s->pb, k);
+            } else if (fused) {
+                // alpha, then the forward sweep applies x += alpha p, r -= alpha q row by row; the backward
+                // sweep produces the (r, z) partials
+                hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kBlock), 0, c->stream, st, (const double*)s->pa, na, k);
+                ApplyOpts<T> o{s->pb, s->pb_cap, true};
+                o.done = &st->done;
+                o.x = A.x;
+                o.p = A.p;
+                o.q = A.q;
+                o.alpha = &st->alpha[k & 1];
+                nbk = precond_apply_t<T>(c, A.pc, A.z, A.r, A.pc_fp32, o);
+                PFEM_REQUIRE(nbk > 0, "fused PCG iteration without its partials");
             } else {
                 hipLaunchKernelGGL((k_cg_update_xr<T>), dim3(g), dim3(kBlock), 0, c->stream, n, A.x, A.r, A.p, A.q, st, s->pa, na, k);
                 // the triangular sweeps also produce the (r, z) partials when they can
@@ -987,6 +1010,7 @@ int pfem_solver_set_option(pfem_solver s, int option, int value)
         PFEM_REQUIRE(s, "null solver");
         switch (option) {
         case PFEM_SOLVER_OPT_FACTOR_ORDER: s->factor_order = value ? 1 : 0; break;
+        case PFEM_SOLVER_OPT_FUSE: s->fuse_xr = value ? 1 : 0; break;
         case PFEM_SOLVER_OPT_POLL:
             PFEM_REQUIRE(value >= 1 && value <= 4096, "poll interval out of range");
             s->poll = value;

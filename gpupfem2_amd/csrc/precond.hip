@@ -229,61 +229,91 @@ __device__ __forceinline__ T dep_sum(T s, int p0, int p1, const int* __restrict_
     return s;
 }
 
+// Arguments of the single-pass sweeps (passed by value).
+template <typename TV, typename TS, typename TF>
+struct SweepArgs {
+    const int* chunks;   // {first row (or position in rows), count} per workgroup
+    const int* rows;     // rows by level (unused when the chunks are row ranges)
+    const int* rowptr;
+    const int* col;
+    const int* diag;
+    const TS* f;         // factor values
+    const int* perm;     // factor row -> vector index (null: vectors in the factor's ordering)
+    TF* sy;              // forward results (sentinel until published)
+    TF* sw;              // backward results
+    int* err;            // set when a wait exceeds its spin limit
+    int limit;
+    const int* done;     // the Krylov solve's done flag: both sweeps skip once it is set
+    const TV* src;       // forward: right-hand side b
+    // forward, fused PCG update (x != null, factor ordering): x += alpha p, r -= alpha q, then b = r
+    TV* x;
+    TV* r;
+    const TV* p;
+    const TV* q;
+    const double* alpha;
+    // backward: output and the optional (r, z) partials
+    TV* out;
+    const TV* rdot;
+    double* part;
+};
+
 // forward solve, all levels: y_i = (b_i - sum_{k<i} L_ik y_k) [/ L_ii for IC0], b_i = src[perm[i]];
 // also re-arms sw for the backward solve. CONTIG: a chunk is a row range, else positions in rows.
-// TV is the type of the vectors, TF that of the factor and of the arithmetic (FP64 vectors with an FP32
-// factor: the mixed-precision preconditioner of an FP64 Krylov solve).
-template <typename TV, typename TS, typename TF, bool UNIT, bool CONTIG>
-__global__ __launch_bounds__(kBlock) void k_sweep_lower(const int* __restrict__ chunks, const int* __restrict__ rows,
-                                                        const int* __restrict__ rowptr, const int* __restrict__ col,
-                                                        const int* __restrict__ diag, const TS* __restrict__ f,
-                                                        const int* __restrict__ perm, const TV* src, TF* sy, TF* sw,
-                                                        int* err, int limit)
+// TV is the type of the vectors, TS that of the stored factor, TF that of the arithmetic and of the
+// published values (FP64 vectors over an FP32-stored factor: the mixed-precision preconditioner).
+template <typename TV, typename TS, typename TF, bool UNIT, bool CONTIG, bool XR>
+__global__ __launch_bounds__(kBlock) void k_sweep_lower(SweepArgs<TV, TS, TF> a)
 {
-    const int c0 = chunks[2 * blockIdx.x], cnt = chunks[2 * blockIdx.x + 1];
+    if (a.done && *a.done) return;
+    const int c0 = a.chunks[2 * blockIdx.x], cnt = a.chunks[2 * blockIdx.x + 1];
     if ((int)threadIdx.x >= cnt) return;
-    const int i = CONTIG ? c0 + (int)threadIdx.x : rows[c0 + threadIdx.x];
-    const int di = diag[i];
-    TF s = dep_sum((TF)src[perm ? perm[i] : i], rowptr[i], di, col, f, sy, err, limit);
-    publish(sy + i, UNIT ? s : s / (TF)f[di]);
-    sw[i] = sentinel_value<TF>();
+    const int i = CONTIG ? c0 + (int)threadIdx.x : a.rows[c0 + threadIdx.x];
+    const int di = a.diag[i];
+    TV b;
+    if constexpr (XR) {
+        // updateXR (linear_solver.cu:68-77) for this row, as the separate vector kernel computes it
+        const TV al = (TV)*a.alpha;
+        a.x[i] += al * a.p[i];
+        b = a.r[i] - al * a.q[i];
+        a.r[i] = b;
+    } else {
+        b = a.src[a.perm ? a.perm[i] : i];
+    }
+    TF s = dep_sum((TF)b, a.rowptr[i], di, a.col, a.f, a.sy, a.err, a.limit);
+    publish(a.sy + i, UNIT ? s : s / (TF)a.f[di]);
+    a.sw[i] = sentinel_value<TF>();
 }
 
 // backward solve, all levels: z_i = (y_i - sum_{k>i} U_ik z_k) / U_ii, out[perm[i]] = z_i; re-arms sy.
-// part != null: the chunk's partials of (r, z) and, when qdot is given, of (q, z) (vectors in their own
-// order) go to part[chunk] and part[nchunks + chunk].
+// part != null: the chunk's partial of (r, z) (r = rdot, in the vectors' order) goes to part[chunk].
 template <typename TV, typename TS, typename TF, bool CONTIG>
-__global__ __launch_bounds__(kBlock) void k_sweep_upper(const int* __restrict__ chunks, const int* __restrict__ rows,
-                                                        const int* __restrict__ rowptr, const int* __restrict__ col,
-                                                        const int* __restrict__ diag, const TS* __restrict__ f,
-                                                        const int* __restrict__ perm, TF* sy, TF* sw, TV* out,
-                                                        const TV* __restrict__ rdot, double* __restrict__ part, int* err,
-                                                        int limit)
+__global__ __launch_bounds__(kBlock) void k_sweep_upper(SweepArgs<TV, TS, TF> a)
 {
     // (r, z) partial without a workgroup barrier after the sweep: each wave leaves its sum in LDS and the
     // last wave to arrive adds the four in wave order (the order block_sum uses)
     __shared__ double wsum[kWaves];
     __shared__ unsigned arrived;
-    if (part) {
+    if (a.done && *a.done) return;
+    if (a.part) {
         if (threadIdx.x == 0) arrived = 0u;
         __syncthreads();
     }
-    const int c0 = chunks[2 * blockIdx.x], cnt = chunks[2 * blockIdx.x + 1];
+    const int c0 = a.chunks[2 * blockIdx.x], cnt = a.chunks[2 * blockIdx.x + 1];
     double v = 0.0;
     if ((int)threadIdx.x < cnt) {
-        const int i = CONTIG ? c0 + (int)threadIdx.x : rows[c0 + threadIdx.x];
-        const int di = diag[i];
-        const TF yi = sy[i];
-        sy[i] = sentinel_value<TF>();
-        TF s = dep_sum(yi, di + 1, rowptr[i + 1], col, f, sw, err, limit);
-        s = s / (TF)f[di];
-        publish(sw + i, s);
-        const int o = perm ? perm[i] : i;
+        const int i = CONTIG ? c0 + (int)threadIdx.x : a.rows[c0 + threadIdx.x];
+        const int di = a.diag[i];
+        const TF yi = a.sy[i];
+        a.sy[i] = sentinel_value<TF>();
+        TF s = dep_sum(yi, di + 1, a.rowptr[i + 1], a.col, a.f, a.sw, a.err, a.limit);
+        s = s / (TF)a.f[di];
+        publish(a.sw + i, s);
+        const int o = a.perm ? a.perm[i] : i;
         const TV z = (TV)s;
-        out[o] = z;
-        if (part) v = (double)(rdot[o] * z);
+        a.out[o] = z;
+        if (a.part) v = (double)(a.rdot[o] * z);
     }
-    if (!part) return;
+    if (!a.part) return;
     v = wave_sum(v);
     if ((threadIdx.x & 63) == 0) {
         wsum[threadIdx.x >> 6] = v;
@@ -292,7 +322,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_upper(const int* __restrict__ 
             __threadfence_block();
             double t = wsum[0];
             for (int w = 1; w < kWaves; ++w) t += wsum[w];
-            part[blockIdx.x] = t;
+            a.part[blockIdx.x] = t;
         }
     }
 }
@@ -526,27 +556,57 @@ static void level_solves(Context* c, pfem_precond_t* pc, const T* src, T* work, 
     }
 }
 
-// The two single-pass sweeps; part != null also produces the (r, z) [and (q, z)] partials, one per
-// backward chunk.
+// The two single-pass sweeps; o.part also produces the (r, z) partials (one per backward chunk), o.x
+// fuses the PCG x/r update into the forward sweep.
 template <typename TV, typename TS, typename TF>
 static void sweeps(Context* c, pfem_precond_t* pc, const TV* src, TV* out, const TS* f, TF* sy, TF* sw, double* part,
-                   bool factor_order)
+                   const ApplyOpts<TV>& o)
 {
     const Pattern& F = *pc->fpat;
     const bool ic = pc->kind == PFEM_PRECOND_IC0;
-    const int* perm = pc->ordering == PFEM_ORDER_NATURAL || factor_order ? nullptr : pc->perm.p;
+    SweepArgs<TV, TS, TF> a{};
+    a.rows = nullptr;
+    a.rowptr = F.rowptr.p;
+    a.col = F.col.p;
+    a.diag = F.diag.p;
+    a.f = f;
+    a.perm = pc->ordering == PFEM_ORDER_NATURAL || o.factor_order ? nullptr : pc->perm.p;
+    a.sy = sy;
+    a.sw = sw;
+    a.err = pc->sweep_err.p;
+    a.limit = kSpinLimit;
+    a.done = o.done;
+    a.src = src;
+    a.x = o.x;
+    a.r = const_cast<TV*>(src);
+    a.p = o.p;
+    a.q = o.q;
+    a.alpha = o.alpha;
+    a.out = out;
+    a.rdot = src;
+    a.part = part;
+    const bool xr = o.x != nullptr;
+    PFEM_REQUIRE(!xr || (o.factor_order && pc->contiguous && o.p && o.q && o.alpha),
+                 "the fused x/r update needs a multicolour factor and vectors in its ordering");
     const dim3 gl(pc->nlchunks), gu(pc->nuchunks), bl(kBlock);
-    const int* lc = pc->lchunks.p;
-    const int* uc = pc->uchunks.p;
-    int* err = pc->sweep_err.p;
+    SweepArgs<TV, TS, TF> lo = a, up = a;
+    lo.chunks = pc->lchunks.p;
+    up.chunks = pc->uchunks.p;
     if (pc->contiguous) {
-        if (ic) hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, false, true>), gl, bl, 0, c->stream, lc, pc->lrows.p, F.rowptr.p, F.col.p, F.diag.p, f, perm, src, sy, sw, err, kSpinLimit);
-        else hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, true, true>), gl, bl, 0, c->stream, lc, pc->lrows.p, F.rowptr.p, F.col.p, F.diag.p, f, perm, src, sy, sw, err, kSpinLimit);
-        hipLaunchKernelGGL((k_sweep_upper<TV, TS, TF, true>), gu, bl, 0, c->stream, uc, pc->urows.p, F.rowptr.p, F.col.p, F.diag.p, f, perm, sy, sw, out, src, part, err, kSpinLimit);
+        if (xr) {
+            if (ic) hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, false, true, true>), gl, bl, 0, c->stream, lo);
+            else hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, true, true, true>), gl, bl, 0, c->stream, lo);
+        } else {
+            if (ic) hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, false, true, false>), gl, bl, 0, c->stream, lo);
+            else hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, true, true, false>), gl, bl, 0, c->stream, lo);
+        }
+        hipLaunchKernelGGL((k_sweep_upper<TV, TS, TF, true>), gu, bl, 0, c->stream, up);
     } else {
-        if (ic) hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, false, false>), gl, bl, 0, c->stream, lc, pc->lrows.p, F.rowptr.p, F.col.p, F.diag.p, f, perm, src, sy, sw, err, kSpinLimit);
-        else hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, true, false>), gl, bl, 0, c->stream, lc, pc->lrows.p, F.rowptr.p, F.col.p, F.diag.p, f, perm, src, sy, sw, err, kSpinLimit);
-        hipLaunchKernelGGL((k_sweep_upper<TV, TS, TF, false>), gu, bl, 0, c->stream, uc, pc->urows.p, F.rowptr.p, F.col.p, F.diag.p, f, perm, sy, sw, out, src, part, err, kSpinLimit);
+        lo.rows = pc->lrows.p;
+        up.rows = pc->urows.p;
+        if (ic) hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, false, false, false>), gl, bl, 0, c->stream, lo);
+        else hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, true, false, false>), gl, bl, 0, c->stream, lo);
+        hipLaunchKernelGGL((k_sweep_upper<TV, TS, TF, false>), gu, bl, 0, c->stream, up);
     }
 }
 
@@ -559,7 +619,6 @@ int precond_apply_t(Context* c, pfem_precond_t* pc, TV* z, const TV* r, bool fp3
     const bool fuse_ok = opt.part && z != r && pc->sweep && pc->nuchunks <= opt.cap;
     double* part = fuse_ok ? opt.part : nullptr;
     PFEM_REQUIRE(!opt.factor_order || pc->sweep, "factor-order apply needs the single-pass sweeps");
-    const bool fo = opt.factor_order;
     switch (pc->kind) {
     case PFEM_PRECOND_NONE:
         if (z != r) hipLaunchKernelGGL((k_copyT<TV>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, (long)n, z, r);
@@ -577,15 +636,15 @@ int precond_apply_t(Context* c, pfem_precond_t* pc, TV* z, const TV* r, bool fp3
             if (fp32) {
                 // factor stored in FP32, solves in FP64: an exact apply of the rounded factor
                 PFEM_REQUIRE(pc->fac32.p && pc->sweep, "FP32 factor not enabled (or per-level apply selected)");
-                sweeps<double, float, double>(c, pc, r, z, pc->fac32.p, pc->sy.p, pc->sw.p, part, fo);
+                sweeps<double, float, double>(c, pc, r, z, pc->fac32.p, pc->sy.p, pc->sw.p, part, opt);
             } else if (pc->sweep) {
-                sweeps<double, double, double>(c, pc, r, z, pc->fac.p, pc->sy.p, pc->sw.p, part, fo);
+                sweeps<double, double, double>(c, pc, r, z, pc->fac.p, pc->sy.p, pc->sw.p, part, opt);
             } else {
                 level_solves<double>(c, pc, r, pc->tmp.p, z, pc->fac.p);
             }
         } else {
             PFEM_REQUIRE(pc->fac32.p, "FP32 factor not enabled");
-            if (pc->sweep) sweeps<float, float, float>(c, pc, r, z, pc->fac32.p, pc->sy32.p, pc->sw32.p, part, fo);
+            if (pc->sweep) sweeps<float, float, float>(c, pc, r, z, pc->fac32.p, pc->sy32.p, pc->sw32.p, part, opt);
             else level_solves<float>(c, pc, r, pc->tmp32.p, z, pc->fac32.p);
         }
     }

@@ -147,6 +147,7 @@ struct pfem_solver_t {
     double *pa = nullptr, *pb = nullptr, *pn = nullptr;
     int pb_cap = 0, pa_cap = 0;
     int factor_order = 1;   // run multicolour IC0/ILU0 solves in the factor's ordering
+    int fuse_xr = 1;        // and fuse PCG's x/r update into the forward sweep
     pfem::DevArray<pfem::CgState> cg;
     pfem::DevArray<pfem::GmState> gm;
     pfem::CgState* h_cg = nullptr;   // pinned
@@ -169,6 +170,13 @@ struct ApplyOpts {
     double* part = nullptr;
     int cap = 0;
     bool factor_order = false;
+    // PCG fusion (single-pass sweeps in factor order): skip when *done; x += alpha p and r -= alpha q
+    // (r = the apply's input) before the forward sweep uses r
+    const int* done = nullptr;
+    TV* x = nullptr;
+    const TV* p = nullptr;
+    const TV* q = nullptr;
+    const double* alpha = nullptr;
 };
 // z = M^-1 r with vectors of type TV. fp32 (TV = double): IC0/ILU0 with the factor stored in FP32 and the
 // solves computed in FP64. Returns
