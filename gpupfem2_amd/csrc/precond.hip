@@ -341,6 +341,142 @@ __global__ void k_f64_to_f32(long n, float* __restrict__ d, const double* __rest
 }
 
 // ------------------------------------------------------------------ host setup
+// Reverse Cuthill-McKee order of the matrix graph: breadth-first from a pseudo-peripheral node of each
+// connected component (the last node reached by a BFS from the component's minimum-degree node),
+// neighbours visited by ascending degree (ties by index), the whole sequence reversed.
+static std::vector<int> rcm_order(const Pattern& A)
+{
+    const int n = A.n;
+    std::vector<int> deg(n);
+    for (int i = 0; i < n; ++i) deg[i] = A.h_rowptr[i + 1] - A.h_rowptr[i] - 1;
+    std::vector<int> order;
+    order.reserve(n);
+    std::vector<char> seen(n, 0);
+    std::vector<int> q, nb, last(n, -1);
+    auto bfs = [&](int root, std::vector<int>& out, int stamp) {
+        out.clear();
+        out.push_back(root);
+        last[root] = stamp;
+        for (size_t h = 0; h < out.size(); ++h) {
+            const int u = out[h];
+            nb.clear();
+            for (int k = A.h_rowptr[u]; k < A.h_rowptr[u + 1]; ++k) {
+                const int v = A.h_col[k];
+                if (v != u && last[v] != stamp && !seen[v]) {
+                    last[v] = stamp;
+                    nb.push_back(v);
+                }
+            }
+            std::sort(nb.begin(), nb.end(), [&](int a, int b) { return deg[a] != deg[b] ? deg[a] < deg[b] : a < b; });
+            out.insert(out.end(), nb.begin(), nb.end());
+        }
+    };
+    std::vector<int> by_degree(n);
+    std::iota(by_degree.begin(), by_degree.end(), 0);
+    std::stable_sort(by_degree.begin(), by_degree.end(), [&](int a, int b) { return deg[a] < deg[b]; });
+    int stamp = 0;
+    for (int s0 : by_degree) {
+        if (seen[s0]) continue;
+        bfs(s0, q, stamp++);
+        const int root = q.back();   // pseudo-peripheral
+        bfs(root, q, stamp++);
+        for (int v : q) seen[v] = 1;
+        order.insert(order.end(), q.begin(), q.end());
+    }
+    std::reverse(order.begin(), order.end());
+    return order;
+}
+
+// Reduces a proper colouring to K colours where a few stragglers exceed it (planar node graphs of
+// triangle meshes are 4-colourable, and greedy leaves a handful of nodes at colour 4). A straggler v
+// takes colour a after a Kempe interchange: a <-> b swapped in the union of the {a, b}-components of
+// v's a-coloured neighbours, allowed when that union holds none of v's b-coloured neighbours (every
+// such swap keeps the colouring proper). When no pair works, the chain through one neighbour is swapped
+// and the search repeats, a bounded number of times. Deterministic; stragglers that resist keep their
+// colour. Returns the colour count.
+static int reduce_colours(const Pattern& A, std::vector<int>& colour, int ncol, int K)
+{
+    const int n = A.n;
+    std::vector<int> bad;
+    for (int i = 0; i < n; ++i)
+        if (colour[i] >= K) bad.push_back(i);
+    if (bad.empty() || bad.size() > (size_t)n / 100 + 16) return ncol;
+    constexpr size_t kCap = 1u << 18;   // largest chain considered
+    std::vector<int> seen(n, -1), comp, nb;
+    std::vector<char> forbid(n, 0);
+    int stamp = 0;
+    // union of the {a, b}-components of the start nodes; false if it reaches a forbidden node or the cap
+    auto chain = [&](const std::vector<int>& starts, int a, int b) {
+        ++stamp;
+        comp.clear();
+        for (int u : starts)
+            if (seen[u] != stamp) {
+                seen[u] = stamp;
+                comp.push_back(u);
+            }
+        for (size_t h = 0; h < comp.size(); ++h) {
+            const int u = comp[h];
+            for (int k = A.h_rowptr[u]; k < A.h_rowptr[u + 1]; ++k) {
+                const int w = A.h_col[k];
+                if (w == u || seen[w] == stamp || (colour[w] != a && colour[w] != b)) continue;
+                if (forbid[w] || comp.size() >= kCap) return false;
+                seen[w] = stamp;
+                comp.push_back(w);
+            }
+        }
+        return true;
+    };
+    auto swap_chain = [&](int a, int b) {
+        for (int u : comp) colour[u] = colour[u] == a ? b : a;
+    };
+    auto neighbours = [&](int v) {
+        nb.clear();
+        for (int k = A.h_rowptr[v]; k < A.h_rowptr[v + 1]; ++k)
+            if (A.h_col[k] != v) nb.push_back(A.h_col[k]);
+    };
+    auto fix = [&](int v) {
+        neighbours(v);
+        for (int a = 0; a < K; ++a) {
+            std::vector<int> na;
+            for (int u : nb)
+                if (colour[u] == a) na.push_back(u);
+            if (na.empty()) {
+                colour[v] = a;
+                return true;
+            }
+            for (int b = 0; b < K; ++b) {
+                if (b == a) continue;
+                for (int u : nb)
+                    if (colour[u] == b) forbid[u] = 1;
+                const bool ok = chain(na, a, b);
+                for (int u : nb) forbid[u] = 0;
+                if (ok) {
+                    swap_chain(a, b);
+                    colour[v] = a;
+                    return true;
+                }
+            }
+        }
+        return false;
+    };
+    for (int v : bad) {
+        if (colour[v] < K || fix(v)) continue;
+        neighbours(v);
+        const std::vector<int> ring = nb;
+        for (int t = 0; t < 64 && colour[v] >= K; ++t) {
+            const int u = ring[t % ring.size()];
+            const int a = colour[u];
+            if (a >= K) continue;
+            const int b = (a + 1 + (t / (int)ring.size()) % (K - 1)) % K;
+            if (chain(std::vector<int>{u}, a, b)) swap_chain(a, b);
+            fix(v);
+        }
+    }
+    int m = 0;
+    for (int i = 0; i < n; ++i) m = std::max(m, colour[i] + 1);
+    return m;
+}
+
 static void build_factor_structure(Context* c, pfem_precond_t* pc)
 {
     const Pattern& A = *pc->pat;
@@ -348,11 +484,16 @@ static void build_factor_structure(Context* c, pfem_precond_t* pc)
     std::vector<int> perm(n), iperm(n);
     std::vector<int> cptr;   // multicolour: colour c occupies rows [cptr[c], cptr[c+1]) of the new order
     if (pc->ordering == PFEM_ORDER_MULTICOLOR) {
-        // greedy colouring in natural order; new order = (colour, old index)
+        // greedy colouring visiting the nodes in reverse Cuthill-McKee order, then Kempe-chain reduction of
+        // the few nodes above colour 3; new order = (colour, RCM position). On the cylinder channel this
+        // gives 4 colours where greedy in natural order gives 6: the colours are the sweeps' dependency
+        // levels, so two fewer waits per triangular solve.
+        const std::vector<int> order = rcm_order(A);
         std::vector<int> colour(n, -1);
         std::vector<int> mark;
         int ncol = 0;
-        for (int i = 0; i < n; ++i) {
+        for (int oi = 0; oi < n; ++oi) {
+            const int i = order[oi];
             mark.assign(ncol + 1, 0);
             for (int k = A.h_rowptr[i]; k < A.h_rowptr[i + 1]; ++k) {
                 const int j = A.h_col[k];
@@ -363,7 +504,11 @@ static void build_factor_structure(Context* c, pfem_precond_t* pc)
             colour[i] = cc;
             if (cc == ncol) ++ncol;
         }
-        std::iota(perm.begin(), perm.end(), 0);
+        ncol = reduce_colours(A, colour, ncol, 4);
+        for (int i = 0; i < n; ++i)
+            for (int k = A.h_rowptr[i]; k < A.h_rowptr[i + 1]; ++k)
+                PFEM_REQUIRE(A.h_col[k] == i || colour[A.h_col[k]] != colour[i], "colouring is not proper");
+        perm = order;
         std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return colour[a] < colour[b]; });
         cptr.assign(ncol + 1, 0);
         for (int i = 0; i < n; ++i) cptr[colour[i] + 1]++;
@@ -460,13 +605,42 @@ static void build_factor_structure(Context* c, pfem_precond_t* pc)
     upload(c, pc->iperm, iperm);
     // chunk = {first position in rows, count}; with colour levels every level is a contiguous row range
     // and a chunk stores {first row, count} instead
+    //
+    // Within a level the chunks are dealt to workgroup slots XCD by XCD: workgroup b runs on XCD b mod 8,
+    // and each XCD receives one contiguous band of the level's chunks. The rows of every colour are in
+    // RCM order, so band x of one colour and band x of the next cover the same region of the mesh, and a
+    // row's dependencies were published on its own XCD (its L2) rather than on any of the eight. The
+    // reordering stays inside the level, so every workgroup still waits only on lower-numbered ones.
     auto chunk_table = [&](const std::vector<int>& ptr, const std::vector<int>& rows) {
+        constexpr int kXcds = 8;
         std::vector<int> ch;
-        for (size_t l = 0; l + 1 < ptr.size(); ++l)
-            for (int o = ptr[l]; o < ptr[l + 1]; o += kBlock) {
-                ch.push_back(colour_levels ? rows[o] : o);
-                ch.push_back(std::min(kBlock, ptr[l + 1] - o));
+        for (size_t l = 0; l + 1 < ptr.size(); ++l) {
+            std::vector<std::pair<int, int>> level;
+            for (int o = ptr[l]; o < ptr[l + 1]; o += kBlock)
+                level.emplace_back(colour_levels ? rows[o] : o, std::min(kBlock, ptr[l + 1] - o));
+            const int m = (int)level.size();
+            const int base = (int)ch.size() / 2;   // workgroup index of the level's first chunk
+            std::vector<int> next(kXcds), end(kXcds);
+            for (int x = 0; x < kXcds; ++x) {
+                next[x] = (int)((long)m * x / kXcds);
+                end[x] = (int)((long)m * (x + 1) / kXcds);
             }
+            std::vector<char> used(m, 0);
+            int spill = 0;   // lowest chunk not yet dealt, for slots whose band is exhausted
+            for (int b = 0; b < m; ++b) {
+                const int x = (base + b) % kXcds;
+                int q = -1;
+                while (next[x] < end[x] && used[next[x]]) ++next[x];
+                if (next[x] < end[x]) q = next[x]++;
+                else {
+                    while (used[spill]) ++spill;
+                    q = spill;
+                }
+                used[q] = 1;
+                ch.push_back(level[q].first);
+                ch.push_back(level[q].second);
+            }
+        }
         return ch;
     };
     const std::vector<int> lch = chunk_table(lptr, lrows), uch = chunk_table(uptr, urows);
