@@ -75,7 +75,7 @@ struct Pattern {
     long nnz = 0;
     std::vector<int> h_rowptr, h_col;
     DevArray<int> rowptr, col;
-    DevArray<int> tiles;   // SpMV row tiles: first row of each tile, ntiles + 1 entries
+    DevArray<int> tiles;   // SpMV row tiles: {first row, end row} per workgroup (2 * ntiles entries)
     int ntiles = 0;
     DevArray<int> diag;    // position of the diagonal entry of each row
     std::vector<int> h_diag;

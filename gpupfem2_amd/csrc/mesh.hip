@@ -34,7 +34,9 @@ void zero_async(Context* c, void* d, size_t bytes)
 
 // ------------------------------------------------------------------ SpMV row tiles
 // Rows are cut into tiles of at most kBlock rows and at most kTileNnz non-zeros (a longer single row
-// gets a tile of its own); one workgroup streams one tile through LDS.
+// gets a tile of its own); one workgroup streams one tile through LDS. The table holds {first row, end
+// row} per workgroup, dealt so that the tiles of XCD x (workgroup b runs on XCD b mod 8) form the x-th
+// contiguous band of rows: neighbouring rows share their gathered vector entries in one L2.
 constexpr int kTileNnz = 2048;
 
 std::shared_ptr<Pattern> make_pattern(Context* c, int n, std::vector<int>&& rowptr, std::vector<int>&& col)
@@ -44,17 +46,43 @@ std::shared_ptr<Pattern> make_pattern(Context* c, int n, std::vector<int>&& rowp
     P->nnz = rowptr[n];
     P->h_rowptr = std::move(rowptr);
     P->h_col = std::move(col);
-    std::vector<int> tiles;
-    tiles.push_back(0);
+    std::vector<int> bounds;
+    bounds.push_back(0);
     int r = 0;
     while (r < n) {
         int r1 = r;
         const int base = P->h_rowptr[r];
         while (r1 < n && r1 - r < kBlock && (P->h_rowptr[r1 + 1] - base <= kTileNnz || r1 == r)) ++r1;
-        tiles.push_back(r1);
+        bounds.push_back(r1);
         r = r1;
     }
-    P->ntiles = (int)tiles.size() - 1;
+    const int m = (int)bounds.size() - 1;
+    P->ntiles = m;
+    constexpr int kXcds = 8;
+    std::vector<int> tiles;
+    tiles.reserve(2 * (size_t)m);
+    {
+        std::vector<int> next(kXcds), end(kXcds);
+        for (int x = 0; x < kXcds; ++x) {
+            next[x] = (int)((long)m * x / kXcds);
+            end[x] = (int)((long)m * (x + 1) / kXcds);
+        }
+        std::vector<char> used(m, 0);
+        int spill = 0;
+        for (int b = 0; b < m; ++b) {
+            const int x = b % kXcds;
+            int q;
+            while (next[x] < end[x] && used[next[x]]) ++next[x];
+            if (next[x] < end[x]) q = next[x]++;
+            else {
+                while (used[spill]) ++spill;
+                q = spill;
+            }
+            used[q] = 1;
+            tiles.push_back(bounds[q]);
+            tiles.push_back(bounds[q + 1]);
+        }
+    }
     P->h_diag.assign(n, -1);
     for (int i = 0; i < n; ++i)
         for (int k = P->h_rowptr[i]; k < P->h_rowptr[i + 1]; ++k)

@@ -12,16 +12,16 @@ namespace pfem {
 
 constexpr int kTileNnz = 2048;
 
-// Product of the calling lane's row with x (tile t = blockIdx.x). Returns the row index in r
-// (r >= r1 means the lane owns no row). prod is an LDS buffer of kTileNnz entries.
+// Product of the calling lane's row with x (tile t = blockIdx.x: rows [tiles[2t], tiles[2t+1])). Returns
+// the row index in r (r >= r1 means the lane owns no row). prod is an LDS buffer of kTileNnz entries.
 template <typename VT, typename XT>
 __device__ __forceinline__ XT tile_row_product(const int* __restrict__ tiles, const int* __restrict__ rowptr,
                                                const int* __restrict__ col, const VT* __restrict__ val,
                                                const XT* __restrict__ x, XT* prod, int& r, int& r1)
 {
     const int t = blockIdx.x;
-    const int r0 = tiles[t];
-    r1 = tiles[t + 1];
+    const int r0 = tiles[2 * t];
+    r1 = tiles[2 * t + 1];
     const int k0 = rowptr[r0];
     const int nk = rowptr[r1] - k0;
     r = r0 + (int)threadIdx.x;
