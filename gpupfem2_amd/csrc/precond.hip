@@ -234,10 +234,10 @@ template <typename TV, typename TS, typename TF>
 struct SweepArgs {
     const int* chunks;   // {first row (or position in rows), count} per workgroup
     const int* rows;     // rows by level (unused when the chunks are row ranges)
-    const int* rowptr;
+    const int* ptr;      // row offsets of this sweep's triangle in the split factor
     const int* col;
-    const int* diag;
-    const TS* f;         // factor values
+    const TS* f;         // split factor values (off-diagonal entries)
+    const TS* d;         // diagonal values
     const int* perm;     // factor row -> vector index (null: vectors in the factor's ordering)
     TF* sy;              // forward results (sentinel until published)
     TF* sw;              // backward results
@@ -268,7 +268,6 @@ __global__ __launch_bounds__(kBlock) void k_sweep_lower(SweepArgs<TV, TS, TF> a)
     const int c0 = a.chunks[2 * blockIdx.x], cnt = a.chunks[2 * blockIdx.x + 1];
     if ((int)threadIdx.x >= cnt) return;
     const int i = CONTIG ? c0 + (int)threadIdx.x : a.rows[c0 + threadIdx.x];
-    const int di = a.diag[i];
     TV b;
     if constexpr (XR) {
         // updateXR (linear_solver.cu:68-77) for this row, as the separate vector kernel computes it
@@ -279,8 +278,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_lower(SweepArgs<TV, TS, TF> a)
     } else {
         b = a.src[a.perm ? a.perm[i] : i];
     }
-    TF s = dep_sum((TF)b, a.rowptr[i], di, a.col, a.f, a.sy, a.err, a.limit);
-    publish(a.sy + i, UNIT ? s : s / (TF)a.f[di]);
+    TF s = dep_sum((TF)b, a.ptr[i], a.ptr[i + 1], a.col, a.f, a.sy, a.err, a.limit);
+    publish(a.sy + i, UNIT ? s : s / (TF)a.d[i]);
     a.sw[i] = sentinel_value<TF>();
 }
 
@@ -302,11 +301,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep_upper(SweepArgs<TV, TS, TF> a)
     double v = 0.0;
     if ((int)threadIdx.x < cnt) {
         const int i = CONTIG ? c0 + (int)threadIdx.x : a.rows[c0 + threadIdx.x];
-        const int di = a.diag[i];
         const TF yi = a.sy[i];
         a.sy[i] = sentinel_value<TF>();
-        TF s = dep_sum(yi, di + 1, a.rowptr[i + 1], a.col, a.f, a.sw, a.err, a.limit);
-        s = s / (TF)a.f[di];
+        TF s = dep_sum(yi, a.ptr[i], a.ptr[i + 1], a.col, a.f, a.sw, a.err, a.limit);
+        s = s / (TF)a.d[i];
         publish(a.sw + i, s);
         const int o = a.perm ? a.perm[i] : i;
         const TV z = (TV)s;
@@ -649,6 +647,31 @@ static void build_factor_structure(Context* c, pfem_precond_t* pc)
     pc->nuchunks = (int)uch.size() / 2;
     upload(c, pc->lchunks, lch);
     upload(c, pc->uchunks, uch);
+    {
+        // split copy for the sweeps: a forward (backward) sweep then fetches only the lower (upper) triangle
+        std::vector<int> sptr(2 * ((size_t)n + 1)), scol, spos;
+        scol.reserve(F.nnz);
+        spos.reserve(F.nnz);
+        for (int t = 0; t < 2; ++t)
+            for (int i = 0; i < n; ++i) {
+                sptr[(size_t)t * (n + 1) + i] = (int)scol.size();
+                const int k0 = t == 0 ? F.h_rowptr[i] : F.h_diag[i] + 1;
+                const int k1 = t == 0 ? F.h_diag[i] : F.h_rowptr[i + 1];
+                for (int k = k0; k < k1; ++k) {
+                    scol.push_back(F.h_col[k]);
+                    spos.push_back(k);
+                }
+                if (i == n - 1) sptr[(size_t)t * (n + 1) + n] = (int)scol.size();
+            }
+        pc->split_diag = (long)scol.size();
+        for (int i = 0; i < n; ++i) spos.push_back(F.h_diag[i]);
+        PFEM_REQUIRE((long)spos.size() == F.nnz, "factor pattern with a repeated diagonal entry");
+        if (scol.empty()) scol.push_back(0);   // keep the device array non-null for a diagonal-only factor
+        upload(c, pc->split_ptr, sptr);
+        upload(c, pc->split_col, scol);
+        upload(c, pc->split_pos, spos);
+        pc->split.alloc(F.nnz);
+    }
     pc->fac.alloc(F.nnz);
     pc->tmp.alloc(n);
     pc->sy.alloc(n);
@@ -697,9 +720,14 @@ void precond_initialize(Context* c, pfem_precond_t* pc, pfem_csr_t* a)
     if (pc->kind == PFEM_PRECOND_IC0)
         hipLaunchKernelGGL(k_ic0_transpose, dim3(blocks_for(F.n)), dim3(kBlock), 0, c->stream, F.n, F.rowptr.p,
                            F.diag.p, pc->mirror.p, pc->fac.p);
-    if (pc->fac32.p)
+    hipLaunchKernelGGL(k_gather_values, dim3(blocks_for(F.nnz)), dim3(kBlock), 0, c->stream, (long)F.nnz,
+                       pc->split_pos.p, pc->fac.p, pc->split.p);
+    if (pc->fac32.p) {
         hipLaunchKernelGGL(k_f64_to_f32, dim3(blocks_for(F.nnz)), dim3(kBlock), 0, c->stream, (long)F.nnz,
                            pc->fac32.p, pc->fac.p);
+        hipLaunchKernelGGL(k_f64_to_f32, dim3(blocks_for(F.nnz)), dim3(kBlock), 0, c->stream, (long)F.nnz,
+                           pc->split32.p, pc->split.p);
+    }
     PFEM_HIP(hipGetLastError());
 }
 
@@ -740,10 +768,9 @@ static void sweeps(Context* c, pfem_precond_t* pc, const TV* src, TV* out, const
     const bool ic = pc->kind == PFEM_PRECOND_IC0;
     SweepArgs<TV, TS, TF> a{};
     a.rows = nullptr;
-    a.rowptr = F.rowptr.p;
-    a.col = F.col.p;
-    a.diag = F.diag.p;
+    a.col = pc->split_col.p;
     a.f = f;
+    a.d = f + pc->split_diag;
     a.perm = pc->ordering == PFEM_ORDER_NATURAL || o.factor_order ? nullptr : pc->perm.p;
     a.sy = sy;
     a.sw = sw;
@@ -766,6 +793,8 @@ static void sweeps(Context* c, pfem_precond_t* pc, const TV* src, TV* out, const
     SweepArgs<TV, TS, TF> lo = a, up = a;
     lo.chunks = pc->lchunks.p;
     up.chunks = pc->uchunks.p;
+    lo.ptr = pc->split_ptr.p;
+    up.ptr = pc->split_ptr.p + (F.n + 1);
     if (pc->contiguous) {
         if (xr) {
             if (ic) hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, false, true, true>), gl, bl, 0, c->stream, lo);
@@ -810,15 +839,15 @@ int precond_apply_t(Context* c, pfem_precond_t* pc, TV* z, const TV* r, bool fp3
             if (fp32) {
                 // factor stored in FP32, solves in FP64: an exact apply of the rounded factor
                 PFEM_REQUIRE(pc->fac32.p && pc->sweep, "FP32 factor not enabled (or per-level apply selected)");
-                sweeps<double, float, double>(c, pc, r, z, pc->fac32.p, pc->sy.p, pc->sw.p, part, opt);
+                sweeps<double, float, double>(c, pc, r, z, pc->split32.p, pc->sy.p, pc->sw.p, part, opt);
             } else if (pc->sweep) {
-                sweeps<double, double, double>(c, pc, r, z, pc->fac.p, pc->sy.p, pc->sw.p, part, opt);
+                sweeps<double, double, double>(c, pc, r, z, pc->split.p, pc->sy.p, pc->sw.p, part, opt);
             } else {
                 level_solves<double>(c, pc, r, pc->tmp.p, z, pc->fac.p);
             }
         } else {
             PFEM_REQUIRE(pc->fac32.p, "FP32 factor not enabled");
-            if (pc->sweep) sweeps<float, float, float>(c, pc, r, z, pc->fac32.p, pc->sy32.p, pc->sw32.p, part, opt);
+            if (pc->sweep) sweeps<float, float, float>(c, pc, r, z, pc->split32.p, pc->sy32.p, pc->sw32.p, part, opt);
             else level_solves<float>(c, pc, r, pc->tmp32.p, z, pc->fac32.p);
         }
     }
@@ -845,6 +874,7 @@ void precond_enable_fp32(Context* c, pfem_precond_t* pc)
         pc->invd32.alloc(pc->n);
     } else if ((pc->kind == PFEM_PRECOND_IC0 || pc->kind == PFEM_PRECOND_ILU0) && !pc->fac32.p) {
         pc->fac32.alloc(pc->fpat->nnz);
+        pc->split32.alloc(pc->fpat->nnz);
         pc->tmp32.alloc(pc->n);
         pc->sy32.alloc(pc->n);
         pc->sw32.alloc(pc->n);

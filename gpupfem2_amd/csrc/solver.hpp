@@ -125,6 +125,13 @@ struct pfem_precond_t {
     pfem::DevArray<double> sy, sw;
     pfem::DevArray<float> sy32, sw32;
     pfem::DevArray<int> sweep_err;        // set when a wait exceeded its spin limit
+    // the sweeps' copy of the factor: all strictly lower entries (row by row), then all strictly upper
+    // entries, then the n diagonal values, each row's entries in the factor's order; split[k] takes
+    // fac[split_pos[k]]. split_ptr holds the lower row offsets (n + 1), then the upper ones (n + 1).
+    pfem::DevArray<int> split_pos, split_ptr, split_col;
+    pfem::DevArray<double> split;
+    pfem::DevArray<float> split32;
+    long split_diag = 0;                  // offset of the diagonal values in split
 };
 
 struct pfem_solver_t {
