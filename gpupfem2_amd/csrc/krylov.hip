@@ -15,6 +15,7 @@
 #include "internal.hpp"
 #include "solver.hpp"
 #include "spmv.hpp"
+#include "sync.hpp"
 
 namespace pfem {
 
@@ -130,6 +131,42 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(const int* __restrict__ tile
     block_partials<1>(v, part);
 }
 
+// q = A p with alpha_k = gamma_k / (p, q) finished by the launch's last workgroup from the published
+// partials (the sum cg_alpha forms), for the fused PCG iteration: the same values as k_cg_spmv followed
+// by k_cg_alpha, one launch fewer.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_alpha(const int* __restrict__ tiles, const int* __restrict__ rowptr,
+                                                          const int* __restrict__ col, const T* __restrict__ val,
+                                                          const T* __restrict__ p, T* __restrict__ q, CgState* st,
+                                                          double* part, int k, int* err)
+{
+    __shared__ T prod[kTileNnz];
+    if (st->done) return;
+    int row, r1;
+    const T ap = tile_row_product<T, T>(tiles, rowptr, col, val, p, prod, row, r1);
+    double v[1] = {0.0};
+    if (row < r1) {
+        q[row] = ap;
+        v[0] = (double)(p[row] * ap);
+    }
+    block_sum<1>(v);
+    publish_partial(v[0], part);
+    if (blockIdx.x == gridDim.x - 1) {
+        const double pq = collect_partials(part, gridDim.x, err, kSpinLimit);
+        if (threadIdx.x == 0) st->alpha[k & 1] = st->gam[k & 1] / pq;
+    }
+}
+
+// p = z + beta p with beta_{k+1} = gamma_{k+1} / gamma_k already recorded (by the backward sweep's last
+// workgroup): k_cg_p's update without its partial sum.
+template <typename T>
+__global__ void k_cg_p_recorded(int n, T* __restrict__ p, const T* __restrict__ z, const CgState* st, int k)
+{
+    if (st->done) return;
+    const T beta = (T)(st->gam[(k + 1) & 1] / st->gam[k & 1]);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = z[i] + beta * p[i];
+}
+
 // alpha_k = gamma_k / (p, q) from the SpMV partials
 __device__ __forceinline__ double cg_alpha(const CgState* st, const double* pa, int na, int k)
 {
@@ -203,20 +240,7 @@ __global__ void k_cg_p(int n, T* __restrict__ p, const T* __restrict__ z, CgStat
     const double gnew = g[0];
     const T beta = (T)(gnew / st->gam[k & 1]);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = z[i] + beta * p[i];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        st->gam[(k + 1) & 1] = gnew;
-        st->it = k + 1;
-        st->res = sqrt(fabs(gnew));
-        if (gnew != gnew) {
-            st->nan = 1;
-            st->done = 1;
-        } else if (gnew < st->thr) {
-            st->converged = 1;
-            st->done = 1;
-        } else if (k + 1 >= st->maxit) {
-            st->done = 1;
-        }
-    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) cg_record(st, gnew, k);
 }
 
 // ------------------------------------------------------------------ GMRES kernels
@@ -542,7 +566,10 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
     const bool jac = A.kind == PFEM_PRECOND_JACOBI, none = A.kind == PFEM_PRECOND_NONE;
     // x/r update fused into the forward sweep: multicolour IC0/ILU0, single-pass sweeps, factor ordering
     const bool fused = A.factor_order && A.pc && A.pc->sweep && A.pc->contiguous &&
-                       A.pc->nuchunks <= s->pb_cap && s->fuse_xr;
+                       A.pc->nuchunks <= s->pb_cap && s->fuse >= 1;
+    // four launches per iteration, the reductions finished inside the launches that produce them: SpMV
+    // (+ alpha), forward sweep (+ x/r), backward sweep (+ beta and the stopping test), p update
+    const bool fused2 = fused && s->fuse >= 2 && P.ntiles <= s->pa_cap;
     CgState init{};
     init.tol = A.tol;
     init.tol_mode = A.tol_mode;
@@ -574,14 +601,39 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
         hipLaunchKernelGGL((k_cg_start<T>), dim3(g), dim3(kBlock), 0, c->stream, n, A.r, A.z, A.p, A.b, t, s->pa);
     }
     hipLaunchKernelGGL(k_cg_begin, dim3(1), dim3(kBlock), 0, c->stream, st, (const double*)s->pa, nb0);
+    if (fused2) {
+        hipLaunchKernelGGL((k_fill_sentinel<double>), dim3(blocks_for(P.ntiles)), dim3(kBlock), 0, c->stream,
+                           (long)P.ntiles, s->pa);
+        hipLaunchKernelGGL((k_fill_sentinel<double>), dim3(blocks_for(A.pc->nuchunks)), dim3(kBlock), 0, c->stream,
+                           (long)A.pc->nuchunks, s->pb);
+    }
     PFEM_HIP(hipGetLastError());
     const int batch = s->poll;
     const int na = P.ntiles, nb = (int)g;
     int launched = 0;
+    // the state of batch b is copied to slot b & 1 and checked after batch b + 1 is queued, so the stream
+    // never drains while the host waits; a converged solve runs at most one batch of no-op launches
+    int pend = -1, pend_launched = 0, slot = 0;
     while (true) {
         for (int j = 0; j < batch; ++j) {
             const int k = launched + j;
             int nbk = nb;
+            if (fused2) {
+                hipLaunchKernelGGL((k_cg_spmv_alpha<T>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p,
+                                   P.rowptr.p, P.col.p, A.val, A.p, A.q, st, s->pa, k, A.pc->sweep_err.p);
+                ApplyOpts<T> o{s->pb, s->pb_cap, true};
+                o.done = &st->done;
+                o.x = A.x;
+                o.p = A.p;
+                o.q = A.q;
+                o.alpha = &st->alpha[k & 1];
+                o.cg = st;
+                o.k = k;
+                PFEM_REQUIRE(precond_apply_t<T>(c, A.pc, A.z, A.r, A.pc_fp32, o) > 0,
+                             "fused PCG iteration without its partials");
+                hipLaunchKernelGGL((k_cg_p_recorded<T>), dim3(g), dim3(kBlock), 0, c->stream, n, A.p, A.z, st, k);
+                continue;
+            }
             hipLaunchKernelGGL((k_cg_spmv<T>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.p, A.q, st, s->pa);
             if (jac) {
                 hipLaunchKernelGGL((k_cg_update<T, true>), dim3(g), dim3(kBlock), 0, c->stream, n, A.x, A.r, A.p, A.q, A.z, A.invd, st, s->pa, na, s->pb, k);
@@ -612,10 +664,18 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
         }
         launched += batch;
         PFEM_HIP(hipGetLastError());
-        PFEM_HIP(hipMemcpyAsync(hst, st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
-        PFEM_HIP(hipStreamSynchronize(c->stream));
-        if (hst->done || launched > A.maxit + batch) break;
+        PFEM_HIP(hipMemcpyAsync(hst + slot, st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+        PFEM_HIP(hipEventRecord(s->ev[slot], c->stream));
+        if (pend >= 0) {
+            PFEM_HIP(hipEventSynchronize(s->ev[pend]));
+            if (hst[pend].done || pend_launched > A.maxit + batch) break;
+        }
+        pend = slot;
+        pend_launched = launched;
+        slot ^= 1;
     }
+    PFEM_HIP(hipMemcpyAsync(hst, st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+    PFEM_HIP(hipStreamSynchronize(c->stream));
 }
 
 template <typename T>
@@ -959,7 +1019,8 @@ int pfem_solver_create(pfem_csr a, int kind, double tol, int max_iterations, int
         s->t.alloc(n);
         s->cg.alloc(1);
         PFEM_HIP(hipMemset(s->cg.p, 0, sizeof(CgState)));
-        PFEM_HIP(hipHostMalloc((void**)&s->h_cg, std::max(sizeof(CgState), (size_t)128), hipHostMallocDefault));
+        PFEM_HIP(hipHostMalloc((void**)&s->h_cg, 2 * std::max(sizeof(CgState), (size_t)128), hipHostMallocDefault));
+        for (hipEvent_t& e : s->ev) PFEM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         int tiles = a->pat->ntiles;
         if (precond && precond->fpat) tiles = std::max(tiles, precond->fpat->ntiles);
         const size_t na = 2 * (size_t)std::max(tiles, kVecGrid);
@@ -1010,7 +1071,10 @@ int pfem_solver_set_option(pfem_solver s, int option, int value)
         PFEM_REQUIRE(s, "null solver");
         switch (option) {
         case PFEM_SOLVER_OPT_FACTOR_ORDER: s->factor_order = value ? 1 : 0; break;
-        case PFEM_SOLVER_OPT_FUSE: s->fuse_xr = value ? 1 : 0; break;
+        case PFEM_SOLVER_OPT_FUSE:
+            PFEM_REQUIRE(value >= 0 && value <= 2, "fusion level out of range");
+            s->fuse = value;
+            break;
         case PFEM_SOLVER_OPT_POLL:
             PFEM_REQUIRE(value >= 1 && value <= 4096, "poll interval out of range");
             s->poll = value;
@@ -1075,6 +1139,8 @@ int pfem_solver_destroy(pfem_solver s)
     (void)hipStreamSynchronize(s->ctx->stream);
     if (s->h_cg) (void)hipHostFree(s->h_cg);
     if (s->h_gm) (void)hipHostFree(s->h_gm);
+    for (hipEvent_t e : s->ev)
+        if (e) (void)hipEventDestroy(e);
     delete s;
     return PFEM_OK;
 }

@@ -22,6 +22,7 @@
 #include "device.hpp"
 #include "internal.hpp"
 #include "solver.hpp"
+#include "sync.hpp"
 
 namespace pfem {
 
@@ -132,67 +133,6 @@ __global__ void k_upper_level(int cnt, const int* __restrict__ rows, const int* 
 }
 
 // ---- single-pass sweeps
-template <typename T>
-struct Sentinel;
-template <>
-struct Sentinel<double> {
-    using U = unsigned long long;
-    static constexpr U bits = 0x7FFC0DEC0DEC0DECull;   // quiet NaN with a payload no arithmetic produces
-    static constexpr U nan = 0x7FF8000000000000ull;
-};
-template <>
-struct Sentinel<float> {
-    using U = unsigned int;
-    static constexpr U bits = 0x7FE0DEC5u;
-    static constexpr U nan = 0x7FC00000u;
-};
-
-constexpr int kSpinLimit = 1 << 16;   // polls of one wait (~0.1 s) before it is declared failed
-
-template <typename T>
-__device__ __forceinline__ T await_value(T* p, int* err, int limit)
-{
-    using U = typename Sentinel<T>::U;
-    constexpr U sent = Sentinel<T>::bits;
-    U* q = reinterpret_cast<U*>(p);
-    // an ordinary load sees either the sentinel or the published value: the location is written once
-    // per launch, and the previous launch's contents are not visible past the kernel boundary
-    U v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (v == sent) {
-        int spins = 0;
-        do {
-            __builtin_amdgcn_s_sleep(1);
-            v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } while (v == sent && ++spins < limit);
-        if (v == sent) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    return __builtin_bit_cast(T, v);
-}
-
-template <typename T>
-__device__ __forceinline__ void publish(T* p, T v)
-{
-    using U = typename Sentinel<T>::U;
-    constexpr U sent = Sentinel<T>::bits, canonical = Sentinel<T>::nan;
-    U b = __builtin_bit_cast(U, v);
-    if (b == sent) b = canonical;
-    __hip_atomic_store(reinterpret_cast<U*>(p), b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-template <typename T>
-__device__ __forceinline__ T sentinel_value()
-{
-    constexpr typename Sentinel<T>::U sent = Sentinel<T>::bits;
-    return __builtin_bit_cast(T, sent);
-}
-
-template <typename T>
-__global__ void k_fill_sentinel(long n, T* __restrict__ d)
-{
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) d[i] = sentinel_value<T>();
-}
-
 // s - sum_{p0 <= p < p1} f[p] * buf[col[p]], subtracted in ascending p (the sequential order); the
 // column indices, factor values and first reads of up to kDeps dependencies are issued together, and
 // only dependencies that still read as the sentinel are polled
@@ -251,10 +191,13 @@ struct SweepArgs {
     const TV* p;
     const TV* q;
     const double* alpha;
-    // backward: output and the optional (r, z) partials
+    // backward: output and the optional (r, z) partials; with cg, the partials are published into
+    // sentinel-armed slots and the last workgroup ends PCG iteration k (cg_record)
     TV* out;
     const TV* rdot;
     double* part;
+    CgState* cg;
+    int k;
 };
 
 // forward solve, all levels: y_i = (b_i - sum_{k<i} L_ik y_k) [/ L_ii for IC0], b_i = src[perm[i]];
@@ -320,8 +263,15 @@ __global__ __launch_bounds__(kBlock) void k_sweep_upper(SweepArgs<TV, TS, TF> a)
             __threadfence_block();
             double t = wsum[0];
             for (int w = 1; w < kWaves; ++w) t += wsum[w];
-            a.part[blockIdx.x] = t;
+            if (a.cg) publish(a.part + blockIdx.x, t);
+            else a.part[blockIdx.x] = t;
         }
+    }
+    if (a.cg && blockIdx.x == gridDim.x - 1) {
+        // gamma_{k+1} = (r, z) and the stopping test, which the separate p-update kernel does otherwise
+        __syncthreads();
+        const double g = collect_partials(a.part, gridDim.x, a.err, a.limit);
+        if (threadIdx.x == 0) cg_record(a.cg, g, a.k);
     }
 }
 
@@ -786,6 +736,8 @@ static void sweeps(Context* c, pfem_precond_t* pc, const TV* src, TV* out, const
     a.out = out;
     a.rdot = src;
     a.part = part;
+    a.cg = part ? o.cg : nullptr;
+    a.k = o.k;
     const bool xr = o.x != nullptr;
     PFEM_REQUIRE(!xr || (o.factor_order && pc->contiguous && o.p && o.q && o.alpha),
                  "the fused x/r update needs a multicolour factor and vectors in its ordering");

@@ -1,6 +1,7 @@
 // Krylov solver and preconditioner state (internal).
 #pragma once
 
+#include <cmath>
 #include <cstddef>
 
 #include "internal.hpp"
@@ -19,6 +20,24 @@ struct CgState {
     double gam[2], alpha[2], thr, res;
     int it, done, converged, nan;
 };
+
+// End of PCG iteration k, given gnew = (r_{k+1}, z_{k+1}): gamma_{k+1}, the iteration count and the
+// stopping test (linear_solver.cu:360); one thread.
+__device__ __forceinline__ void cg_record(CgState* st, double gnew, int k)
+{
+    st->gam[(k + 1) & 1] = gnew;
+    st->it = k + 1;
+    st->res = sqrt(fabs(gnew));
+    if (gnew != gnew) {
+        st->nan = 1;
+        st->done = 1;
+    } else if (gnew < st->thr) {
+        st->converged = 1;
+        st->done = 1;
+    } else if (k + 1 >= st->maxit) {
+        st->done = 1;
+    }
+}
 
 // Device-resident GMRES(m) scalars and small arrays (H column-major with leading dimension m).
 struct GmState {
@@ -154,10 +173,13 @@ struct pfem_solver_t {
     double *pa = nullptr, *pb = nullptr, *pn = nullptr;
     int pb_cap = 0, pa_cap = 0;
     int factor_order = 1;   // run multicolour IC0/ILU0 solves in the factor's ordering
-    int fuse_xr = 1;        // and fuse PCG's x/r update into the forward sweep
+    // PCG kernel fusion in the factor's ordering: 0 none; 1 x/r update inside the forward sweep; 2 also
+    // alpha and beta finished by the last workgroups of the SpMV and of the backward sweep
+    int fuse = 2;
+    hipEvent_t ev[2] = {nullptr, nullptr};   // pipelined polls of the device state
     pfem::DevArray<pfem::CgState> cg;
     pfem::DevArray<pfem::GmState> gm;
-    pfem::CgState* h_cg = nullptr;   // pinned
+    pfem::CgState* h_cg = nullptr;   // pinned, two slots
     pfem::GmState* h_gm = nullptr;   // pinned
     // mixed precision (FP32 inner CG, FP64 refinement)
     pfem::DevArray<float> r32, z32, p32, q32, d32, t32, V32;
@@ -184,6 +206,10 @@ struct ApplyOpts {
     const TV* p = nullptr;
     const TV* q = nullptr;
     const double* alpha = nullptr;
+    // with part: the backward sweep publishes its partials (sentinel-armed slots) and its last workgroup
+    // adds them and ends PCG iteration k in cg (cg_record); nothing further is needed for beta
+    CgState* cg = nullptr;
+    int k = 0;
 };
 // z = M^-1 r with vectors of type TV. fp32 (TV = double): IC0/ILU0 with the factor stored in FP32 and the
 // solves computed in FP64. Returns

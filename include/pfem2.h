@@ -197,7 +197,9 @@ int pfem_solver_set_precision(pfem_solver s, int precision, double inner_tol, in
  * 4 for GMRES): iterations launched between two host reads of the convergence flag. */
 #define PFEM_SOLVER_OPT_FACTOR_ORDER 1
 #define PFEM_SOLVER_OPT_POLL 2
-#define PFEM_SOLVER_OPT_FUSE 3   /* default 1: in factor order, PCG's x/r update runs inside the forward sweep */
+#define PFEM_SOLVER_OPT_FUSE 3   /* PCG kernel fusion in factor order, default 2: 0 none; 1 the x/r update runs
+                                    inside the forward sweep; 2 also alpha and beta are finished by the last
+                                    workgroups of the SpMV and of the backward sweep */
 int pfem_solver_set_option(pfem_solver s, int option, int value);
 int pfem_solver_destroy(pfem_solver s);
 /* LinearSolver::solve(A, x, b) -> bool (linear_solver.cu:146-154, 301-376, 421-521): warm start from x,

@@ -371,16 +371,21 @@ def test_factor_order_solve(ctx, oracle, kind, precond, prec):
     assert abs(out[0][0] - out[1][0]) <= max(3, out[0][0] // 20)
     assert rel_l2(out[1][1], out[0][1]) < 1e-10
     if kind == 0:
-        # PCG's x/r update fused into the forward sweep: the same operations, so the same bits
-        pc = capi.Preconditioner(a, precond, capi.ORDER_MULTICOLOR)
-        s = capi.Solver(a, kind, 1e-12, 5000, 30, capi.TOL_RELATIVE, pc)
-        if prec:
-            s.set_precision(prec, 1e-5, 20)
-        s.set_option(capi.SOLVER_OPT_FUSE, 0)
-        x = ctx.zeros(xy.shape[0])
-        x.upload(np.linspace(0.0, 0.1, xy.shape[0]))
-        st = s.solve(a, x, rhs)
-        assert st.iterations == out[1][0] and np.array_equal(x.download(), out[1][1])
+        # PCG fused (the default: alpha and beta finished by the last workgroups of the SpMV and of the
+        # backward sweep, the x/r update inside the forward sweep; level 1: only the x/r update) and
+        # unfused: the same operations, so the same bits; two solves in a row check that the partial
+        # slots are re-armed
+        for fuse in (0, 1, 2):
+            pc = capi.Preconditioner(a, precond, capi.ORDER_MULTICOLOR)
+            s = capi.Solver(a, kind, 1e-12, 5000, 30, capi.TOL_RELATIVE, pc)
+            if prec:
+                s.set_precision(prec, 1e-5, 20)
+            s.set_option(capi.SOLVER_OPT_FUSE, fuse)
+            for rep in range(2):
+                x = ctx.zeros(xy.shape[0])
+                x.upload(np.linspace(0.0, 0.1, xy.shape[0]))
+                st = s.solve(a, x, rhs)
+                assert st.iterations == out[1][0] and np.array_equal(x.download(), out[1][1]), (fuse, rep)
 
 
 def test_solver_failure_paths(ctx, oracle):
