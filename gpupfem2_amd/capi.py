@@ -111,6 +111,8 @@ SIGNATURES = {
     "pfem_solver_set_precision": (_i, [_vp, _i, _d, _i]),
     "pfem_solver_destroy": (_i, [_vp]),
     "pfem_solver_solve": (_i, [_vp, _vp, _vp, _vp, C.POINTER(SolveStats)]),
+    "pfem_solver_solve_multi": (_i, [_vp, _vp, _i, _vp, _vp, C.POINTER(SolveStats)]),
+    "pfem_solver_solve_batched": (_i, [_vp, _vp, _i, _vp, _vp, _vp, C.POINTER(SolveStats), _vp]),
     "pfem_quadrature": (_i, [_i, _i, _vp]),
     "pfem_assemble_poisson2d": (_i, [_vp, _vp, _vp, _d, _i, _d, _i]),
     "pfem_assemble_poisson3d": (_i, [_vp, _vp, _vp, _d, _i, _d, _i]),
@@ -456,7 +458,8 @@ class Solver(_Handle):
         self.precond = precond
 
     def set_option(self, option, value):
-        """SOLVER_OPT_FACTOR_ORDER (0/1) or SOLVER_OPT_POLL (iterations between convergence polls)."""
+        """SOLVER_OPT_FACTOR_ORDER (0/1), SOLVER_OPT_POLL (iterations between convergence polls) or
+        SOLVER_OPT_FUSE (0/1/2)."""
         self.ctx.check(lib().pfem_solver_set_option(self.h, option, int(value)))
 
     def set_precision(self, precision, inner_tol=0.0, max_refinements=0):
@@ -466,6 +469,22 @@ class Solver(_Handle):
         st = SolveStats()
         self.ctx.check(lib().pfem_solver_solve(self.h, a.h, x.ptr, b.ptr, C.byref(st)))
         return st
+
+    def solve_batched(self, a, sys_rows, x, b):
+        """Independent systems of a block-diagonal matrix (system k: rows sys_rows[k]:sys_rows[k+1]);
+        returns (stats list, per-system time to solution in ms)."""
+        rows = np.ascontiguousarray(sys_rows, np.int32)
+        nsys = rows.size - 1
+        st = (SolveStats * nsys)()
+        tms = np.zeros(nsys, np.float64)
+        self.ctx.check(lib().pfem_solver_solve_batched(self.h, a.h, nsys, _ptr(rows), x.ptr, b.ptr, st, _ptr(tms)))
+        return list(st), tms
+
+    def solve_multi(self, a, X, B, nrhs):
+        """A X = B for nrhs columns (device arrays of nrhs * n values, column j at offset j * n)."""
+        st = (SolveStats * nrhs)()
+        self.ctx.check(lib().pfem_solver_solve_multi(self.h, a.h, int(nrhs), X.ptr, B.ptr, st))
+        return list(st)
 
 
 def quadrature(kind, index):

@@ -155,6 +155,79 @@ def poisson_case(n=64):
     return xy, tri, nodes, np.zeros(nodes.size)
 
 
+def poisson3d_multi_case(n=128, nrhs=8, seed=5):
+    """3D Poisson on the unit box: n^3 cubes, 6 Kuhn tetrahedra each, u = sin(pi x) sin(pi y) sin(pi z)
+    (f = 3 pi^2 u), u = 0 on every face; nrhs right-hand sides, column j = s_j * (assembled RHS) plus
+    N(0, 1e-3) noise on the interior nodes, s_j seeded in [0.5, 2]."""
+    xyz, tet = meshgen.box3d(n)
+    on_b = np.zeros(xyz.shape[0], bool)
+    for d in range(3):
+        on_b |= (np.abs(xyz[:, d]) < TOL_BC) | (np.abs(xyz[:, d] - 1.0) < TOL_BC)
+    nodes = np.nonzero(on_b)[0].astype(np.int32)
+    rng = np.random.default_rng(seed)
+    scales = rng.uniform(0.5, 2.0, nrhs)
+    noise = rng.standard_normal((nrhs, xyz.shape[0])) * 1e-3
+    noise[:, on_b] = 0.0
+    return dict(xyz=xyz, tet=tet, bnodes=nodes, scales=scales, noise=noise, n=n, nrhs=nrhs)
+
+
+def build_poisson3d_multi(ctx, case, quad=0):
+    """Device system of a poisson3d_multi_case: CSR matrix with the Dirichlet rows applied and the nrhs
+    right-hand sides as one column-major device array. Returns (mesh, csr, B, base_rhs)."""
+    from . import capi
+    mesh = capi.Mesh3D(ctx, case["xyz"], case["tet"])
+    a = capi.CSR(ctx, mesh=mesh)
+    nv = case["xyz"].shape[0]
+    rhs = ctx.zeros(nv)
+    capi.assemble_poisson3d(mesh, a, rhs, coeff=1.0, rhs_kind=1, rhs_param=3 * math.pi ** 2, quad=quad)
+    bc = capi.Dirichlet(ctx, nv, case["bnodes"], np.zeros(case["bnodes"].size))
+    bc.apply(a, rhs, MASK)
+    b0 = rhs.download()
+    B = (case["scales"][:, None] * b0[None, :] + case["noise"]).ravel()
+    return mesh, a, ctx.to_device(B), b0
+
+
+def batched_poisson_case(nmesh=256, tmin=10_000, tmax=500_000, seed=11, jitter=0.2):
+    """Independent pressure-Poisson systems: nmesh jittered structured triangulations of [0, 2] x [0, 1]
+    with triangle counts drawn log-uniform in [tmin, tmax]; each has homogeneous Dirichlet nodes on its
+    left edge and a right-hand side of i.i.d. N(0, 1) nodal values with the mean removed (zero on the
+    Dirichlet nodes). The meshes are stored as one triangulation with disjoint vertex sets, so their
+    matrix is block diagonal; sys_rows[k] is the first node of mesh k."""
+    rng = np.random.default_rng(seed)
+    target = np.exp(rng.uniform(np.log(tmin), np.log(tmax), nmesh))
+    xys, tris, bnodes, rhs, rows = [], [], [], [], [0]
+    for k in range(nmesh):
+        nx = max(2, int(round(math.sqrt(target[k]))))
+        ny = max(1, int(round(target[k] / (2 * nx))))
+        xy, tri = meshgen.rectangle(0.0, 2.0, 0.0, 1.0, nx, ny, jitter=jitter, seed=seed * 1000 + k)
+        off = rows[-1]
+        left = np.nonzero(np.abs(xy[:, 0]) < TOL_BC)[0]
+        b = rng.standard_normal(xy.shape[0])
+        b -= b.mean()
+        b[left] = 0.0
+        xys.append(xy)
+        tris.append(tri + off)
+        bnodes.append(left + off)
+        rhs.append(b)
+        rows.append(off + xy.shape[0])
+    return dict(xy=np.concatenate(xys), tri=np.concatenate(tris).astype(np.int32),
+                bnodes=np.concatenate(bnodes).astype(np.int32), rhs=np.concatenate(rhs),
+                sys_rows=np.asarray(rows, np.int32), ntri=np.array([t.shape[0] for t in tris]))
+
+
+def build_batched_poisson(ctx, case):
+    """Device system of a batched_poisson_case: the block-diagonal stiffness matrix with the Dirichlet
+    rows applied and the right-hand side. Returns (mesh, csr, b)."""
+    from . import capi
+    mesh = capi.Mesh2D(ctx, case["xy"], case["tri"], fill_neighbors=False)
+    a = capi.CSR(ctx, mesh=mesh)
+    nv = case["xy"].shape[0]
+    scratch = ctx.zeros(nv)
+    capi.assemble_poisson2d(mesh, a, scratch, coeff=1.0, rhs_kind=2, rhs_param=0.0)
+    capi.Dirichlet(ctx, nv, case["bnodes"], np.zeros(case["bnodes"].size)).apply(a, scratch, MASK)
+    return mesh, a, ctx.to_device(case["rhs"])
+
+
 def step_config(case, precision=0, inner_tol=1e-6, pres_ordering=1):
     """pfem_step_config (include/pfem2.h) for a case."""
     from .capi import StepConfig, make_params

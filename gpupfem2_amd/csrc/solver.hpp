@@ -8,6 +8,15 @@
 
 namespace pfem {
 
+struct MultiWork;   // multi-RHS PCG work space (multi.hip)
+struct MultiWorkDeleter {
+    void operator()(MultiWork* w) const;
+};
+struct BatchWork;   // batched-systems PCG work space (batched.hip)
+struct BatchWorkDeleter {
+    void operator()(BatchWork* w) const;
+};
+
 constexpr int kMaxRestart = 1024;   // largest GMRES cycle (the reference's maxIterations default is 1000)
 constexpr int kVecGrid = 1024;      // grid of the vector kernels (grid-stride loops)
 
@@ -177,6 +186,8 @@ struct pfem_solver_t {
     // alpha and beta finished by the last workgroups of the SpMV and of the backward sweep
     int fuse = 2;
     hipEvent_t ev[2] = {nullptr, nullptr};   // pipelined polls of the device state
+    std::unique_ptr<pfem::MultiWork, pfem::MultiWorkDeleter> multi;   // pfem_solver_solve_multi
+    std::unique_ptr<pfem::BatchWork, pfem::BatchWorkDeleter> batch;   // pfem_solver_solve_batched
     pfem::DevArray<pfem::CgState> cg;
     pfem::DevArray<pfem::GmState> gm;
     pfem::CgState* h_cg = nullptr;   // pinned, two slots

@@ -205,6 +205,24 @@ int pfem_solver_destroy(pfem_solver s);
 /* LinearSolver::solve(A, x, b) -> bool (linear_solver.cu:146-154, 301-376, 421-521): warm start from x,
  * the preconditioner is re-initialised from A first. stats may be NULL. */
 int pfem_solver_solve(pfem_solver s, pfem_csr a, double* d_x, const double* d_b, pfem_solve_stats* stats);
+/* Multi-right-hand-side PCG (no counterpart in the reference, whose SolverCG::solve, linear_solver.cu:301-376,
+ * takes one right-hand side per call): A X = B for nrhs (1..8) columns sharing the matrix, column j of X and
+ * B at d_X + j*n and d_B + j*n (X is also the initial guess). Each column runs the solver's PCG recurrence
+ * and stopping test with its own scalars and stops on its own; the matrix is streamed once per iteration
+ * for all columns. The solver must be PCG with a Jacobi preconditioner or none, in PFEM_PRECISION_FP64 or
+ * PFEM_PRECISION_MIXED (FP32 inner multi-RHS CG, FP64 refinement per column). stats: nrhs entries (or NULL). */
+int pfem_solver_solve_multi(pfem_solver s, pfem_csr a, int nrhs, double* d_X, const double* d_B,
+                            pfem_solve_stats* stats);
+/* Batched independent PCG solves (no counterpart in the reference, which solves one system per
+ * SolverCG::solve call, linear_solver.cu:301-376): a is block diagonal, system k owning rows and columns
+ * [h_sys_rows[k], h_sys_rows[k+1]) (h_sys_rows: nsys + 1 host ints from 0 to n; checked). Each system runs
+ * the solver's PCG recurrence and stopping test with its own scalars and stops on its own; all systems
+ * share every launch. Jacobi preconditioner or none; PFEM_PRECISION_FP64 or PFEM_PRECISION_MIXED (FP32
+ * inner batched CG, FP64 refinement per system). d_x is the initial guess (FP64 path) and the solution.
+ * stats: nsys entries (or NULL); h_time_ms (or NULL): per-system time to solution on the device clock,
+ * from the start of the call's first Krylov or refinement pass to the pass where the system stopped. */
+int pfem_solver_solve_batched(pfem_solver s, pfem_csr a, int nsys, const int* h_sys_rows, double* d_x,
+                              const double* d_b, pfem_solve_stats* stats, double* h_time_ms);
 
 /* ------------------------------------------------------------------ quadrature */
 /* createEdgeQuadratureFormula / createFaceQuadratureFormula / createCellQuadratureFormula
