@@ -18,7 +18,7 @@ ORDER_NATURAL, ORDER_MULTICOLOR = 0, 1
 SOLVER_CG, SOLVER_GMRES = 0, 1
 TOL_ABSOLUTE, TOL_RELATIVE = 0, 1
 PRECISION_FP64, PRECISION_MIXED, PRECISION_MIXED_PRECOND = 0, 1, 2
-SOLVER_OPT_FACTOR_ORDER, SOLVER_OPT_POLL, SOLVER_OPT_FUSE = 1, 2, 3
+SOLVER_OPT_FACTOR_ORDER, SOLVER_OPT_POLL, SOLVER_OPT_FUSE, SOLVER_OPT_REPLACE = 1, 2, 3, 4
 
 
 class SolveStats(C.Structure):

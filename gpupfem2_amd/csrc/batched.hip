@@ -8,9 +8,13 @@
 // every vector kernel works tile by tile, so a workgroup knows its system from the tile and a finished
 // system's tiles return at once. Per-system dot products: one partial per tile, added by one workgroup
 // per system over the system's contiguous tile range, in tile order (deterministic). Per-system time to
-// solution is taken from the device wall clock when the system's stopping test passes.
+// solution is taken from the device wall clock when the system stops.
 //
-// PFEM_PRECISION_MIXED: FP32 inner batched CG on the FP32 copy of A, FP64 iterative refinement per system.
+// PFEM_PRECISION_MIXED: PCG with reliable updates (residual replacement) per system: FP32 vectors and the
+// FP32 copy of A in the iterations, the solution accumulated in FP64; after a tenfold drop of a system's
+// recursive residual (by default), or when it passes the stopping test, the system spends one slot on
+// x += x_s (FP64), r = b - A x with the FP64 matrix, the stopping test on that true residual, and
+// continues its search direction from it. The Krylov recurrence is never restarted.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -24,25 +28,29 @@
 namespace pfem {
 
 struct BSys {
-    double gam[2], alpha, beta, thr, res, irthr, irrz, scale;
-    int it, done, converged, nan, irconv, irnan, iractive, pad;
+    double gam[2], alpha, beta, thr, res;
+    double rz, rzmax;     // reliable updates: gamma now and at the last replacement
+    int it, done, converged, nan;
+    int mode, nrep;       // 0 iterating, 1 replacement due, 2 direction update after one; replacements
     unsigned long long t0, tdone;
 };
 
-// Tile t: rows [tiles[2t], tiles[2t+1]) of system tsys[t]; a system's tiles are contiguous,
-// [stile[s], stile[s+1]).
+
+// Systems are laid out by position, largest first (psys[q] = system at position q), so that the systems
+// still iterating late in a batch form a short prefix and the grids can shrink to it. Tile t: rows
+// [tiles[2t], tiles[2t+1]) of system tsys[t]; the tiles of position q are [stile[q], stile[q+1]).
 struct BatchLayout {
     const Pattern* pat = nullptr;
-    std::vector<int> rows;
+    std::vector<int> rows, order, h_stile;
     int nsys = 0, ntiles = 0;
-    DevArray<int> tiles, tsys, stile;
+    DevArray<int> tiles, tsys, stile, psys;
 };
 
 struct BatchWork {
     BatchLayout L;
     int n = 0;
-    DevArray<double> r, p, q, xr;       // FP64 vectors (xr: refinement residual)
-    DevArray<float> x32, b32, r32, p32, q32;
+    DevArray<double> r, p, q;           // FP64 PCG vectors
+    DevArray<float> x32, r32, p32, q32; // reliable updates: x_s, r_s, p_s, q_s
     DevArray<double> part;              // 2 values x tiles
     DevArray<BSys> sys;
     BSys* h = nullptr;                  // pinned copy of the per-system states
@@ -83,8 +91,8 @@ __global__ __launch_bounds__(kBlock) void k_b_init(const int* __restrict__ tiles
     block_partials<2>(v, part);
 }
 
-// Sum of the tile partials of system blockIdx.x (series i at part + i * ntiles), in tile order; valid in
-// every thread.
+// Sum of the tile partials of the system at position blockIdx.x (series i at part + i * ntiles), in tile
+// order; valid in every thread.
 template <int NV>
 __device__ __forceinline__ void system_sum(const double* __restrict__ part, int ntiles, const int* __restrict__ stile,
                                            double (&tot)[NV])
@@ -108,12 +116,12 @@ __device__ __forceinline__ void system_sum(const double* __restrict__ part, int 
 
 // gamma_0, threshold and start time per system; one workgroup per system
 __global__ void k_b_begin(BSys* __restrict__ sys, const double* __restrict__ part, int ntiles,
-                          const int* __restrict__ stile, double tol, int tol_mode, int outer)
+                          const int* __restrict__ stile, const int* __restrict__ psys, double tol, int tol_mode)
 {
     double tot[2];
     system_sum<2>(part, ntiles, stile, tot);
     if (threadIdx.x != 0) return;
-    BSys& S = sys[blockIdx.x];
+    BSys& S = sys[psys[blockIdx.x]];
     const double rz = tot[0], bmb = tot[1];
     S.gam[0] = rz;
     S.thr = tol_mode == PFEM_TOL_RELATIVE && bmb > 0.0 ? tol * tol * bmb : tol * tol;
@@ -122,11 +130,8 @@ __global__ void k_b_begin(BSys* __restrict__ sys, const double* __restrict__ par
     S.converged = rz < S.thr ? 1 : 0;
     S.done = S.converged | S.nan;
     S.res = sqrt(fabs(rz));
-    if (outer) {
-        // an outer (FP64) solve times itself; an inner pass leaves the refinement's clock alone
-        S.t0 = wall_clock64();
-        S.tdone = S.t0;
-    }
+    S.t0 = wall_clock64();
+    S.tdone = S.t0;
 }
 
 // q = A p; partial (p, q)
@@ -150,9 +155,9 @@ __global__ __launch_bounds__(kBlock) void k_b_spmv(const int* __restrict__ tiles
 
 // alpha per system; one workgroup per system
 __global__ void k_b_alpha(BSys* __restrict__ sys, const double* __restrict__ part, int ntiles,
-                          const int* __restrict__ stile, int k)
+                          const int* __restrict__ stile, const int* __restrict__ psys, int k)
 {
-    BSys& S = sys[blockIdx.x];
+    BSys& S = sys[psys[blockIdx.x]];
     if (S.done) return;
     double tot[1];
     system_sum<1>(part, ntiles, stile, tot);
@@ -183,9 +188,9 @@ __global__ __launch_bounds__(kBlock) void k_b_update(const int* __restrict__ til
 
 // gamma_{k+1}, beta, the stopping test and the time of convergence per system; one workgroup per system
 __global__ void k_b_beta(BSys* __restrict__ sys, const double* __restrict__ part, int ntiles,
-                         const int* __restrict__ stile, int k, int maxit, int outer)
+                         const int* __restrict__ stile, const int* __restrict__ psys, int k, int maxit)
 {
-    BSys& S = sys[blockIdx.x];
+    BSys& S = sys[psys[blockIdx.x]];
     if (S.done) return;
     double tot[1];
     system_sum<1>(part, ntiles, stile, tot);
@@ -204,7 +209,7 @@ __global__ void k_b_beta(BSys* __restrict__ sys, const double* __restrict__ part
     } else if (k + 1 >= maxit) {
         S.done = 1;
     }
-    if (S.done && outer) S.tdone = wall_clock64();
+    if (S.done) S.tdone = wall_clock64();
 }
 
 // p = M^-1 r + beta p (updateP, linear_solver.cu:79-87)
@@ -223,25 +228,27 @@ __global__ __launch_bounds__(kBlock) void k_b_p(const int* __restrict__ tiles, c
     }
 }
 
-// ---- refinement (FP64 outer loop per system)
-// r = b - A x in FP64 for the systems still refining; partials (r, M^-1 r), (b, M^-1 b)
+// ---- reliable-update PCG (PFEM_PRECISION_MIXED)
+// r = b - A x (FP64), r_s = (float) r, p_s = D_s r_s, x_s = 0; partials (r, D r), (b, D b) in FP64
 template <bool JAC>
-__global__ __launch_bounds__(kBlock) void k_b_ir_residual(const int* __restrict__ tiles, const int* __restrict__ tsys,
-                                                          const int* __restrict__ rowptr, const int* __restrict__ col,
-                                                          const double* __restrict__ val, const double* __restrict__ x,
-                                                          const double* __restrict__ b, const double* __restrict__ invd,
-                                                          double* __restrict__ r, const BSys* __restrict__ sys,
-                                                          double* __restrict__ part)
+__global__ __launch_bounds__(kBlock) void k_br_init(const int* __restrict__ tiles, const int* __restrict__ rowptr,
+                                                    const int* __restrict__ col, const double* __restrict__ val,
+                                                    const double* __restrict__ x, const double* __restrict__ b,
+                                                    const double* __restrict__ invd, const float* __restrict__ invd32,
+                                                    float* __restrict__ rs, float* __restrict__ ps, float* __restrict__ xs,
+                                                    double* __restrict__ part)
 {
     __shared__ double prod[kTileNnz];
-    if (!sys[tsys[blockIdx.x]].iractive) return;
     int row, r1;
     const double ax = tile_row_product<double, double>(tiles, rowptr, col, val, x, prod, row, r1);
     double v[2] = {0.0, 0.0};
     if (row < r1) {
         const double bb = b[row];
         const double rr = bb - ax;
-        r[row] = rr;
+        const float r32 = (float)rr;
+        rs[row] = r32;
+        ps[row] = JAC ? r32 * invd32[row] : r32;
+        xs[row] = 0.0f;
         const double d = JAC ? invd[row] : 1.0;
         v[0] = rr * (rr * d);
         v[1] = bb * (bb * d);
@@ -249,57 +256,170 @@ __global__ __launch_bounds__(kBlock) void k_b_ir_residual(const int* __restrict_
     block_partials<2>(v, part);
 }
 
-// per-system refinement test (the FP64 solver's criterion) and the scale of the next correction; the
-// first pass sets the thresholds and the start times
-__global__ void k_b_ir_test(BSys* __restrict__ sys, const double* __restrict__ part, int ntiles,
-                            const int* __restrict__ stile, double tol, int tol_mode, int first)
+__global__ void k_br_begin(BSys* __restrict__ sys, const double* __restrict__ part, int ntiles,
+                           const int* __restrict__ stile, const int* __restrict__ psys, double tol, int tol_mode)
 {
-    BSys& S = sys[blockIdx.x];
-    if (!first && !S.iractive) return;
     double tot[2];
     system_sum<2>(part, ntiles, stile, tot);
     if (threadIdx.x != 0) return;
-    if (first) {
-        const bool rel = tol_mode == PFEM_TOL_RELATIVE && tot[1] > 0.0;
-        S.irthr = rel ? tol * tol * tot[1] : tol * tol;
-        S.t0 = wall_clock64();
+    BSys& S = sys[psys[blockIdx.x]];
+    const double rz = tot[0], bmb = tot[1];
+    S.thr = tol_mode == PFEM_TOL_RELATIVE && bmb > 0.0 ? tol * tol * bmb : tol * tol;
+    S.rz = S.rzmax = rz;
+    S.res = sqrt(fabs(rz));
+    S.it = 0;
+    S.mode = 0;
+    S.nrep = 0;
+    S.nan = rz != rz;
+    S.converged = rz < S.thr ? 1 : 0;
+    S.done = S.converged | S.nan;
+    S.t0 = wall_clock64();
+    S.tdone = S.t0;
+}
+
+// tile of an iterating system: q_s = A_s p_s, partial (p_s, q_s); of a system due a replacement:
+// r = b - A x with the FP64 matrix, r_s = (float) r, partial (r, D r)
+template <bool JAC>
+__global__ __launch_bounds__(kBlock) void k_br_spmv(const int* __restrict__ tiles, const int* __restrict__ tsys,
+                                                    const int* __restrict__ rowptr, const int* __restrict__ col,
+                                                    const float* __restrict__ val32, const double* __restrict__ val,
+                                                    const float* __restrict__ ps, float* __restrict__ qs,
+                                                    const double* __restrict__ x, const double* __restrict__ b,
+                                                    const double* __restrict__ invd, float* __restrict__ rs,
+                                                    const BSys* __restrict__ sys, double* __restrict__ part)
+{
+    __shared__ double prod[kTileNnz];
+    const BSys& S = sys[tsys[blockIdx.x]];
+    if (S.done) return;
+    int row, r1;
+    double v[1] = {0.0};
+    if (S.mode == 1) {
+        const double ax = tile_row_product<double, double>(tiles, rowptr, col, val, x, prod, row, r1);
+        if (row < r1) {
+            const double rr = b[row] - ax;
+            rs[row] = (float)rr;
+            v[0] = rr * (rr * (JAC ? invd[row] : 1.0));
+        }
+    } else {
+        const float ap = tile_row_product<float, float>(tiles, rowptr, col, val32, ps, reinterpret_cast<float*>(prod),
+                                                        row, r1);
+        if (row < r1) {
+            qs[row] = ap;
+            v[0] = (double)(ps[row] * ap);
+        }
     }
-    S.irrz = tot[0];
-    S.irnan = tot[0] != tot[0];
-    S.irconv = tot[0] < S.irthr ? 1 : 0;
-    S.iractive = !(S.irconv || S.irnan);
-    S.scale = S.iractive ? (sqrt(tot[0]) > 0.0 ? sqrt(tot[0]) : 1.0) : 0.0;
-    if (!S.iractive) S.tdone = wall_clock64();
+    block_partials<1>(v, part);
 }
 
-__global__ __launch_bounds__(kBlock) void k_b_ir_down(const int* __restrict__ tiles, const int* __restrict__ tsys,
-                                                      float* __restrict__ d, const double* __restrict__ r,
-                                                      const BSys* __restrict__ sys)
+// alpha of an iterating system; for a replaced one the true gamma: the stopping test, then beta
+__global__ void k_br_alpha(BSys* __restrict__ sys, const double* __restrict__ part, int ntiles,
+                           const int* __restrict__ stile, const int* __restrict__ psys)
 {
-    const double sc = sys[tsys[blockIdx.x]].scale;
-    const int row = tiles[2 * blockIdx.x] + (int)threadIdx.x, r1 = tiles[2 * blockIdx.x + 1];
-    if (row < r1) d[row] = sc > 0.0 ? (float)(r[row] / sc) : 0.0f;
+    BSys& S = sys[psys[blockIdx.x]];
+    if (S.done) return;
+    double tot[1];
+    system_sum<1>(part, ntiles, stile, tot);
+    if (threadIdx.x != 0) return;
+    const double g = tot[0];
+    if (S.mode == 1) {
+        S.nrep += 1;
+        S.res = sqrt(fabs(g));
+        if (g != g) {
+            S.nan = 1;
+            S.done = 1;
+        } else if (g < S.thr) {
+            S.converged = 1;
+            S.done = 1;
+        } else {
+            S.beta = g / S.rz;
+            S.rz = S.rzmax = g;
+            S.mode = 2;
+        }
+        if (S.done) S.tdone = wall_clock64();
+    } else {
+        S.mode = 0;
+        S.alpha = S.rz / g;
+    }
 }
 
-__global__ __launch_bounds__(kBlock) void k_b_ir_up(const int* __restrict__ tiles, const int* __restrict__ tsys,
-                                                    double* __restrict__ x, const float* __restrict__ d,
-                                                    const BSys* __restrict__ sys)
+// x_s += alpha p_s, r_s -= alpha q_s (FP32); partial (r_s, D_s r_s)
+template <bool JAC>
+__global__ __launch_bounds__(kBlock) void k_br_update(const int* __restrict__ tiles, const int* __restrict__ tsys,
+                                                      float* __restrict__ xs, float* __restrict__ rs,
+                                                      const float* __restrict__ ps, const float* __restrict__ qs,
+                                                      const float* __restrict__ invd32, const BSys* __restrict__ sys,
+                                                      double* __restrict__ part)
 {
-    const double sc = sys[tsys[blockIdx.x]].scale;
+    const BSys& S = sys[tsys[blockIdx.x]];
+    if (S.done || S.mode != 0) return;
+    const float alpha = (float)S.alpha;
     const int row = tiles[2 * blockIdx.x] + (int)threadIdx.x, r1 = tiles[2 * blockIdx.x + 1];
-    if (row < r1 && sc > 0.0) x[row] += sc * (double)d[row];
+    double v[1] = {0.0};
+    if (row < r1) {
+        xs[row] += alpha * ps[row];
+        const float rr = rs[row] - alpha * qs[row];
+        rs[row] = rr;
+        v[0] = (double)(rr * (JAC ? rr * invd32[row] : rr));
+    }
+    block_partials<1>(v, part);
+}
+
+// gamma of the recursive residual: the iteration count, then beta or a replacement
+__global__ void k_br_beta(BSys* __restrict__ sys, const double* __restrict__ part, int ntiles,
+                          const int* __restrict__ stile, const int* __restrict__ psys, int maxit, double drop)
+{
+    BSys& S = sys[psys[blockIdx.x]];
+    if (S.done || S.mode != 0) return;
+    double tot[1];
+    system_sum<1>(part, ntiles, stile, tot);
+    if (threadIdx.x != 0) return;
+    const double g = tot[0];
+    S.it += 1;
+    if (g != g) {
+        S.nan = 1;
+        S.done = 1;
+    } else if (S.it >= maxit) {
+        S.done = 1;
+    } else if (g < S.thr || g < drop * S.rzmax) {
+        S.mode = 1;
+    } else {
+        S.beta = g / S.rz;
+        S.rz = g;
+    }
+    if (S.done) S.tdone = wall_clock64();
+}
+
+// p_s = D_s r_s + beta p_s; a system due a replacement first folds x_s into x (FP64)
+template <bool JAC>
+__global__ __launch_bounds__(kBlock) void k_br_p(const int* __restrict__ tiles, const int* __restrict__ tsys,
+                                                 float* __restrict__ ps, const float* __restrict__ rs,
+                                                 const float* __restrict__ invd32, double* __restrict__ x,
+                                                 float* __restrict__ xs, const BSys* __restrict__ sys)
+{
+    const BSys& S = sys[tsys[blockIdx.x]];
+    if (S.done) return;
+    const int row = tiles[2 * blockIdx.x] + (int)threadIdx.x, r1 = tiles[2 * blockIdx.x + 1];
+    if (row >= r1) return;
+    if (S.mode == 1) {
+        x[row] += (double)xs[row];
+        xs[row] = 0.0f;
+    } else {
+        const float zz = JAC ? rs[row] * invd32[row] : rs[row];
+        ps[row] = zz + (float)S.beta * ps[row];
+    }
+}
+
+// x += x_s everywhere (zero for a system whose last replacement folded it in)
+__global__ void k_br_fold(long n, double* __restrict__ x, const float* __restrict__ xs)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += (double)xs[i];
 }
 
 __global__ void k_b_to32(long n, float* __restrict__ d, const double* __restrict__ s)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) d[i] = (float)s[i];
-}
-
-__global__ void k_b_arm(int nsys, BSys* __restrict__ sys)
-{
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < nsys) sys[s].iractive = 1;
 }
 
 // ---- host side
@@ -314,9 +434,14 @@ static void build_layout(Context* c, const Pattern& P, const std::vector<int>& r
                 PFEM_REQUIRE(P.h_col[k] >= rows[s] && P.h_col[k] < rows[s + 1],
                              "the matrix is not block diagonal over the given systems");
     }
+    std::vector<int> order(nsys);
+    for (int q = 0; q < nsys; ++q) order[q] = q;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int u, int v) { return rows[u + 1] - rows[u] > rows[v + 1] - rows[v]; });
     std::vector<int> tiles, tsys, stile(nsys + 1, 0);
-    for (int s = 0; s < nsys; ++s) {
-        stile[s] = (int)tsys.size();
+    for (int q = 0; q < nsys; ++q) {
+        const int s = order[q];
+        stile[q] = (int)tsys.size();
         int r = rows[s];
         while (r < rows[s + 1]) {
             int r1 = r;
@@ -331,28 +456,32 @@ static void build_layout(Context* c, const Pattern& P, const std::vector<int>& r
     stile[nsys] = (int)tsys.size();
     L.pat = &P;
     L.rows = rows;
+    L.order = order;
+    L.h_stile = stile;
     L.nsys = nsys;
     L.ntiles = (int)tsys.size();
     upload(c, L.tiles, tiles);
     upload(c, L.tsys, tsys);
     upload(c, L.stile, stile);
+    upload(c, L.psys, order);
 }
 
-// One batched CG pass (T = double: the FP64 solve; T = float: the inner solve from x = 0)
+// Batched FP64 PCG over all systems (x: initial guess and solution)
 template <typename T, typename VT>
 static void b_cg_run(Context* c, pfem_solver_t* s, BatchWork& w, const VT* val, const T* invd, T* x, const T* b, T* r,
-                     T* p, T* q, int x_zero, double tol, int tol_mode, int maxit, bool outer)
+                     T* p, T* q, int x_zero, double tol, int tol_mode, int maxit)
 {
     const BatchLayout& L = w.L;
     const Pattern& P = *L.pat;
     const bool jac = invd != nullptr;
-    const dim3 gt(L.ntiles), gs(L.nsys), bl(kBlock);
+    dim3 gt(L.ntiles), gs(L.nsys);
+    const dim3 bl(kBlock);
     double* pa = w.part.p;
     if (x_zero) PFEM_HIP(hipMemsetAsync(x, 0, sizeof(T) * (size_t)w.n, c->stream));
     if (jac) hipLaunchKernelGGL((k_b_init<T, VT, true>), gt, bl, 0, c->stream, L.tiles.p, P.rowptr.p, P.col.p, val, x, b, invd, r, p, pa, x_zero);
     else hipLaunchKernelGGL((k_b_init<T, VT, false>), gt, bl, 0, c->stream, L.tiles.p, P.rowptr.p, P.col.p, val, x, b, invd, r, p, pa, x_zero);
-    hipLaunchKernelGGL(k_b_begin, gs, bl, 0, c->stream, w.sys.p, (const double*)pa, L.ntiles, L.stile.p, tol, tol_mode,
-                       outer ? 1 : 0);
+    hipLaunchKernelGGL(k_b_begin, gs, bl, 0, c->stream, w.sys.p, (const double*)pa, L.ntiles, L.stile.p, L.psys.p, tol,
+                       tol_mode);
     PFEM_HIP(hipGetLastError());
     const int batch = s->poll;
     int launched = 0;
@@ -360,11 +489,11 @@ static void b_cg_run(Context* c, pfem_solver_t* s, BatchWork& w, const VT* val, 
         for (int j = 0; j < batch; ++j) {
             const int k = launched + j;
             hipLaunchKernelGGL((k_b_spmv<T, VT>), gt, bl, 0, c->stream, L.tiles.p, L.tsys.p, P.rowptr.p, P.col.p, val, p, q, w.sys.p, pa);
-            hipLaunchKernelGGL(k_b_alpha, gs, bl, 0, c->stream, w.sys.p, (const double*)pa, L.ntiles, L.stile.p, k);
+            hipLaunchKernelGGL(k_b_alpha, gs, bl, 0, c->stream, w.sys.p, (const double*)pa, L.ntiles, L.stile.p, L.psys.p, k);
             if (jac) hipLaunchKernelGGL((k_b_update<T, true>), gt, bl, 0, c->stream, L.tiles.p, L.tsys.p, x, r, p, q, invd, w.sys.p, pa);
             else hipLaunchKernelGGL((k_b_update<T, false>), gt, bl, 0, c->stream, L.tiles.p, L.tsys.p, x, r, p, q, invd, w.sys.p, pa);
-            hipLaunchKernelGGL(k_b_beta, gs, bl, 0, c->stream, w.sys.p, (const double*)pa, L.ntiles, L.stile.p, k, maxit,
-                               outer ? 1 : 0);
+            hipLaunchKernelGGL(k_b_beta, gs, bl, 0, c->stream, w.sys.p, (const double*)pa, L.ntiles, L.stile.p, L.psys.p, k,
+                               maxit);
             if (jac) hipLaunchKernelGGL((k_b_p<T, true>), gt, bl, 0, c->stream, L.tiles.p, L.tsys.p, p, r, invd, w.sys.p);
             else hipLaunchKernelGGL((k_b_p<T, false>), gt, bl, 0, c->stream, L.tiles.p, L.tsys.p, p, r, invd, w.sys.p);
         }
@@ -372,10 +501,69 @@ static void b_cg_run(Context* c, pfem_solver_t* s, BatchWork& w, const VT* val, 
         PFEM_HIP(hipGetLastError());
         PFEM_HIP(hipMemcpyAsync(w.h, w.sys.p, sizeof(BSys) * L.nsys, hipMemcpyDeviceToHost, c->stream));
         PFEM_HIP(hipStreamSynchronize(c->stream));
-        bool all = true;
-        for (int q2 = 0; q2 < L.nsys && all; ++q2) all = w.h[q2].done != 0;
-        if (all || launched > maxit + batch) break;
+        // shrink the grids to the prefix of positions that still iterate
+        int last = -1;
+        for (int q2 = 0; q2 < L.nsys; ++q2)
+            if (!w.h[L.order[q2]].done) last = q2;
+        if (last < 0 || launched > maxit + batch) break;
+        gt = dim3(L.h_stile[last + 1]);
+        gs = dim3(last + 1);
     }
+}
+
+// Reliable-update PCG over all systems (x: FP64 solution, b: FP64 right-hand side)
+static void b_ru_run(Context* c, pfem_solver_t* s, BatchWork& w, pfem_csr_t* a, double* x, const double* b, bool jac)
+{
+    const BatchLayout& L = w.L;
+    const Pattern& P = *L.pat;
+    const int n = w.n;
+    if (!w.x32.p) {
+        w.x32.alloc(n); w.r32.alloc(n); w.p32.alloc(n); w.q32.alloc(n);
+    }
+    if (!a->val32.p) a->val32.alloc(P.nnz);
+    hipLaunchKernelGGL(k_b_to32, dim3(blocks_for(P.nnz)), dim3(kBlock), 0, c->stream, (long)P.nnz, a->val32.p, a->val.p);
+    const double* invd = jac ? s->pc->invd.p : nullptr;
+    const float* invd32 = jac ? s->pc->invd32.p : nullptr;
+    PFEM_REQUIRE(!jac || invd32, "FP32 Jacobi diagonal missing");
+    float *xs = w.x32.p, *rs = w.r32.p, *ps = w.p32.p, *qs = w.q32.p;
+    const float* val32 = a->val32.p;
+    const double* val = a->val.p;
+    double* pa = w.part.p;
+    dim3 gt(L.ntiles), gs(L.nsys);
+    const dim3 bl(kBlock);
+    if (jac) hipLaunchKernelGGL((k_br_init<true>), gt, bl, 0, c->stream, L.tiles.p, P.rowptr.p, P.col.p, val, (const double*)x, b, invd, invd32, rs, ps, xs, pa);
+    else hipLaunchKernelGGL((k_br_init<false>), gt, bl, 0, c->stream, L.tiles.p, P.rowptr.p, P.col.p, val, (const double*)x, b, invd, invd32, rs, ps, xs, pa);
+    hipLaunchKernelGGL(k_br_begin, gs, bl, 0, c->stream, w.sys.p, (const double*)pa, L.ntiles, L.stile.p, L.psys.p, s->tol, s->tol_mode);
+    PFEM_HIP(hipGetLastError());
+    const int batch = s->poll;
+    const int slots = s->maxit + s->maxit / 4 + 64;
+    const double drop = std::pow(10.0, -s->replace_digits);
+    int launched = 0;
+    while (true) {
+        for (int j = 0; j < batch; ++j) {
+            if (jac) hipLaunchKernelGGL((k_br_spmv<true>), gt, bl, 0, c->stream, L.tiles.p, L.tsys.p, P.rowptr.p, P.col.p, val32, val, (const float*)ps, qs, (const double*)x, b, invd, rs, (const BSys*)w.sys.p, pa);
+            else hipLaunchKernelGGL((k_br_spmv<false>), gt, bl, 0, c->stream, L.tiles.p, L.tsys.p, P.rowptr.p, P.col.p, val32, val, (const float*)ps, qs, (const double*)x, b, invd, rs, (const BSys*)w.sys.p, pa);
+            hipLaunchKernelGGL(k_br_alpha, gs, bl, 0, c->stream, w.sys.p, (const double*)pa, L.ntiles, L.stile.p, L.psys.p);
+            if (jac) hipLaunchKernelGGL((k_br_update<true>), gt, bl, 0, c->stream, L.tiles.p, L.tsys.p, xs, rs, (const float*)ps, (const float*)qs, invd32, (const BSys*)w.sys.p, pa);
+            else hipLaunchKernelGGL((k_br_update<false>), gt, bl, 0, c->stream, L.tiles.p, L.tsys.p, xs, rs, (const float*)ps, (const float*)qs, invd32, (const BSys*)w.sys.p, pa);
+            hipLaunchKernelGGL(k_br_beta, gs, bl, 0, c->stream, w.sys.p, (const double*)pa, L.ntiles, L.stile.p, L.psys.p, s->maxit,
+                               drop);
+            if (jac) hipLaunchKernelGGL((k_br_p<true>), gt, bl, 0, c->stream, L.tiles.p, L.tsys.p, ps, (const float*)rs, invd32, x, xs, (const BSys*)w.sys.p);
+            else hipLaunchKernelGGL((k_br_p<false>), gt, bl, 0, c->stream, L.tiles.p, L.tsys.p, ps, (const float*)rs, invd32, x, xs, (const BSys*)w.sys.p);
+        }
+        launched += batch;
+        PFEM_HIP(hipGetLastError());
+        PFEM_HIP(hipMemcpyAsync(w.h, w.sys.p, sizeof(BSys) * L.nsys, hipMemcpyDeviceToHost, c->stream));
+        PFEM_HIP(hipStreamSynchronize(c->stream));
+        int last = -1;
+        for (int q2 = 0; q2 < L.nsys; ++q2)
+            if (!w.h[L.order[q2]].done) last = q2;
+        if (last < 0 || launched > slots + batch) break;
+        gt = dim3(L.h_stile[last + 1]);
+        gs = dim3(last + 1);
+    }
+    // a system stopped by the iteration limit still holds its correction in x_s
+    hipLaunchKernelGGL(k_br_fold, dim3(blocks_for(n)), bl, 0, c->stream, (long)n, x, (const float*)xs);
 }
 
 static void solve_batched(Context* c, pfem_solver_t* s, pfem_csr_t* a, const std::vector<int>& rows, double* x,
@@ -388,7 +576,7 @@ static void solve_batched(Context* c, pfem_solver_t* s, pfem_csr_t* a, const std
         BatchWork& w = *s->batch;
         build_layout(c, P, rows, w.L);
         w.n = n;
-        w.r.alloc(n); w.p.alloc(n); w.q.alloc(n); w.xr.alloc(n);
+        w.r.alloc(n); w.p.alloc(n); w.q.alloc(n);
         w.part.alloc(2 * (size_t)w.L.ntiles);
         w.sys.alloc(w.L.nsys);
         PFEM_HIP(hipHostMalloc((void**)&w.h, sizeof(BSys) * w.L.nsys, hipHostMallocDefault));
@@ -399,45 +587,11 @@ static void solve_batched(Context* c, pfem_solver_t* s, pfem_csr_t* a, const std
     const bool jac = s->pc && s->pc->kind == PFEM_PRECOND_JACOBI;
     if (s->pc) precond_initialize(c, s->pc, a);
     PFEM_HIP(hipMemsetAsync(w.sys.p, 0, sizeof(BSys) * nsys, c->stream));
-    int refinements = 0;
-    std::vector<int> inner(nsys, 0);
     if (s->precision != PFEM_PRECISION_MIXED) {
         b_cg_run<double, double>(c, s, w, a->val.p, jac ? s->pc->invd.p : nullptr, x, b, w.r.p, w.p.p, w.q.p, 0, s->tol,
-                                 s->tol_mode, s->maxit, true);
+                                 s->tol_mode, s->maxit);
     } else {
-        if (!w.x32.p) {
-            w.x32.alloc(n); w.b32.alloc(n); w.r32.alloc(n); w.p32.alloc(n); w.q32.alloc(n);
-        }
-        if (!a->val32.p) a->val32.alloc(P.nnz);
-        hipLaunchKernelGGL(k_b_to32, dim3(blocks_for(P.nnz)), dim3(kBlock), 0, c->stream, (long)P.nnz, a->val32.p, a->val.p);
-        const double* invd = jac ? s->pc->invd.p : nullptr;
-        const float* invd32 = jac ? s->pc->invd32.p : nullptr;
-        PFEM_REQUIRE(!jac || invd32, "FP32 Jacobi diagonal missing");
-        const dim3 gt(L.ntiles), gs(nsys), bl(kBlock);
-        hipLaunchKernelGGL(k_b_arm, dim3(blocks_for(nsys)), bl, 0, c->stream, nsys, w.sys.p);
-        int total = 0;
-        for (bool first = true;; first = false) {
-            if (jac) hipLaunchKernelGGL((k_b_ir_residual<true>), gt, bl, 0, c->stream, L.tiles.p, L.tsys.p, P.rowptr.p, P.col.p, a->val.p, (const double*)x, b, invd, w.xr.p, (const BSys*)w.sys.p, w.part.p);
-            else hipLaunchKernelGGL((k_b_ir_residual<false>), gt, bl, 0, c->stream, L.tiles.p, L.tsys.p, P.rowptr.p, P.col.p, a->val.p, (const double*)x, b, invd, w.xr.p, (const BSys*)w.sys.p, w.part.p);
-            hipLaunchKernelGGL(k_b_ir_test, gs, bl, 0, c->stream, w.sys.p, (const double*)w.part.p, L.ntiles, L.stile.p,
-                               s->tol, s->tol_mode, first ? 1 : 0);
-            PFEM_HIP(hipMemcpyAsync(w.h, w.sys.p, sizeof(BSys) * nsys, hipMemcpyDeviceToHost, c->stream));
-            PFEM_HIP(hipStreamSynchronize(c->stream));
-            bool any = false;
-            for (int q = 0; q < nsys; ++q) any |= w.h[q].iractive != 0;
-            if (!any || refinements >= s->max_refine || total >= s->maxit) break;
-            hipLaunchKernelGGL(k_b_ir_down, gt, bl, 0, c->stream, L.tiles.p, L.tsys.p, w.b32.p, (const double*)w.xr.p, (const BSys*)w.sys.p);
-            b_cg_run<float, float>(c, s, w, a->val32.p, invd32, w.x32.p, w.b32.p, w.r32.p, w.p32.p, w.q32.p, 1,
-                                   s->inner_tol, PFEM_TOL_RELATIVE, std::max(1, s->maxit - total), false);
-            int most = 0;
-            for (int q = 0; q < nsys; ++q) {
-                inner[q] += w.h[q].it;
-                most = std::max(most, w.h[q].it);
-            }
-            total += most;
-            hipLaunchKernelGGL(k_b_ir_up, gt, bl, 0, c->stream, L.tiles.p, L.tsys.p, x, (const float*)w.x32.p, (const BSys*)w.sys.p);
-            ++refinements;
-        }
+        b_ru_run(c, s, w, a, x, b, jac);
     }
     PFEM_HIP(hipGetLastError());
     PFEM_HIP(hipMemcpyAsync(w.h, w.sys.p, sizeof(BSys) * nsys, hipMemcpyDeviceToHost, c->stream));
@@ -450,11 +604,11 @@ static void solve_batched(Context* c, pfem_solver_t* s, pfem_csr_t* a, const std
         if (stats) {
             pfem_solve_stats& o = stats[q];
             o = pfem_solve_stats{};
-            o.converged = mixed ? S.irconv : S.converged;
-            o.iterations = mixed ? inner[q] : S.it;
-            o.final_residual = mixed ? std::sqrt(std::fabs(S.irrz)) : S.res;
-            o.nan_detected = mixed ? S.irnan : S.nan;
-            o.refinements = refinements;
+            o.converged = S.converged;
+            o.iterations = S.it;
+            o.final_residual = S.res;
+            o.nan_detected = S.nan;
+            o.refinements = mixed ? S.nrep : 0;
         }
         if (time_ms) time_ms[q] = rate_khz > 0 ? (double)(S.tdone - S.t0) / rate_khz : 0.0;
     }

@@ -1075,6 +1075,10 @@ int pfem_solver_set_option(pfem_solver s, int option, int value)
             PFEM_REQUIRE(value >= 0 && value <= 2, "fusion level out of range");
             s->fuse = value;
             break;
+        case PFEM_SOLVER_OPT_REPLACE:
+            PFEM_REQUIRE(value >= 1 && value <= 8, "replacement interval out of range");
+            s->replace_digits = value;
+            break;
         case PFEM_SOLVER_OPT_POLL:
             PFEM_REQUIRE(value >= 1 && value <= 4096, "poll interval out of range");
             s->poll = value;

@@ -11,9 +11,14 @@
 // is shared by the columns and the gathers are G-wide. Dot products are reduced per column in a fixed
 // order (deterministic). The caller's X and B are column-major (column j at X + j * n).
 //
-// PFEM_PRECISION_MIXED: FP32 inner multi-RHS CG on the FP32 copy of A with FP64 iterative refinement of
-// every column (residuals, the stopping test and the solution update in FP64), as the single-RHS mixed
-// solver does.
+// PFEM_PRECISION_MIXED: PCG with reliable updates (residual replacement). The iterations run on FP32
+// vectors and the FP32 copy of A (half the bytes); the solution accumulates in FP64. When a column's
+// recursive residual has fallen by a set factor (10x by default) since the last replacement, or passes
+// the stopping test, the
+// column spends one iteration slot on a replacement: x += x_s in FP64, r = b - A x with the FP64 matrix,
+// the stopping test on that true residual, and the search direction continued from it (beta = new
+// gamma / old gamma). Unlike restarted refinement, the Krylov recurrence is never restarted, so the
+// iteration count stays close to the FP64 solver's, and convergence is decided on FP64 residuals.
 #include <algorithm>
 #include <cmath>
 #include <cstddef>
@@ -35,10 +40,12 @@ struct MState {
     int tol_mode, maxit, ncol, all_done;
     double gam[2][kMaxRhs], alpha[kMaxRhs], beta[kMaxRhs], thr[kMaxRhs], res[kMaxRhs];
     int it[kMaxRhs], done[kMaxRhs], converged[kMaxRhs], nan[kMaxRhs];
-    // refinement
-    double irthr[kMaxRhs], irrz[kMaxRhs], scale[kMaxRhs];
-    int irfirst, irconv[kMaxRhs], irnan[kMaxRhs], irdone;
+    // reliable updates: gamma of the current residual, gamma at the last replacement, the slot state
+    // (0 iterating, 1 replacement due, 2 direction update after a replacement) and the replacements made
+    double rz[kMaxRhs], rzmax[kMaxRhs], drop;
+    int mode[kMaxRhs], nrep[kMaxRhs];
 };
+
 
 // Per-column block sums of NV values: lane l holds column l % G. Reduced over the wave with shuffles that
 // keep the column (offsets 32 .. G), then over the four waves in order; part[(i * G + j) * gridDim.x +
@@ -88,36 +95,136 @@ __device__ __forceinline__ void series_totals(const double* __restrict__ part, d
 }
 
 // ---- FP64 / FP32 multi-RHS CG kernels (T = vector type, VT = matrix value type)
-// sum_k A_ik x_kj for the calling lane's row i and column j, left to right
-template <int G, typename T, typename VT>
-__device__ __forceinline__ T group_row_product(int i, int j, const int* __restrict__ rowptr, const int* __restrict__ col,
-                                               const VT* __restrict__ val, const T* __restrict__ x)
+constexpr int kMTileNnz = 1024;   // non-zeros of a row tile staged in LDS
+constexpr int kMRows = 2;         // rows per lane in a row tile
+
+// Row tiles of the multi-RHS products: tile t = rows [tiles[2t], tiles[2t+1]), at most kMRows * kBlock / G
+// rows and at most kMTileNnz non-zeros unless a single row is longer. Lane (q, j) of a workgroup takes
+// column j of rows q and q + kBlock / G of the tile. Workgroup b takes a contiguous range of tiles; the
+// ranges of XCD x (workgroups b = x mod 8) form the x-th band of rows, so neighbouring rows share their
+// gathered vector entries in one L2. A tile's values, column indices and row offsets are staged in LDS
+// with unit-stride loads while the next tile's bounds are fetched; each lane then adds val * x[col * G + j]
+// over its rows left to right (the sequential order), with up to 8 gathers per row in flight, and hands
+// each sum to f(row, j, s). The G lanes of a row gather G contiguous values.
+template <int G, typename T, typename VT, typename F>
+__device__ __forceinline__ void for_tile_rows(int ntiles, const int* __restrict__ tiles, const int* __restrict__ rowptr,
+                                              const int* __restrict__ col, const VT* __restrict__ val,
+                                              const T* __restrict__ x, F f)
 {
-    T s = 0;
-    const int k1 = rowptr[i + 1];
-    for (int k = rowptr[i]; k < k1; ++k) s += (T)val[k] * x[(size_t)col[k] * G + j];
-    return s;
+    constexpr int kLanesRows = kBlock / G;
+    constexpr int kTileRows = kMRows * kLanesRows;
+    constexpr int kU = 8;
+    __shared__ VT sval[kMTileNnz];
+    __shared__ int scol[kMTileNnz];
+    __shared__ int rp[kTileRows + 1];
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int rb = (b % 8) * (nb / 8) + b / 8;
+    const int t0 = (int)((long)ntiles * rb / nb), t1 = (int)((long)ntiles * (rb + 1) / nb);
+    const int j = threadIdx.x % G, q = threadIdx.x / G;
+    int r0 = 0, r1 = 0, k0 = 0, k1 = 0;
+    if (t0 < t1) {
+        r0 = tiles[2 * t0];
+        r1 = tiles[2 * t0 + 1];
+        k0 = rowptr[r0];
+        k1 = rowptr[r1];
+    }
+    for (int t = t0; t < t1; ++t) {
+        const int nk = k1 - k0;
+        const bool staged = nk <= kMTileNnz;
+        __syncthreads();   // the previous tile's readers are done with the LDS arrays
+        for (int i = threadIdx.x; i <= r1 - r0; i += kBlock) rp[i] = rowptr[r0 + i] - k0;
+        if (staged)
+            for (int k = threadIdx.x; k < nk; k += kBlock) {
+                sval[k] = val[k0 + k];
+                scol[k] = col[k0 + k];
+            }
+        // the next tile's bounds, fetched while this one is computed
+        int n0 = 0, n1 = 0, m0 = 0, m1 = 0;
+        if (t + 1 < t1) {
+            n0 = tiles[2 * t + 2];
+            n1 = tiles[2 * t + 3];
+            m0 = rowptr[n0];
+            m1 = rowptr[n1];
+        }
+        __syncthreads();
+        T s[kMRows];
+#pragma unroll
+        for (int u = 0; u < kMRows; ++u) s[u] = 0;
+        if (staged) {
+            int ka[kMRows], ke[kMRows];
+#pragma unroll
+            for (int u = 0; u < kMRows; ++u) {
+                const int lr = q + u * kLanesRows;
+                ka[u] = lr < r1 - r0 ? rp[lr] : 0;
+                ke[u] = lr < r1 - r0 ? rp[lr + 1] : 0;
+            }
+            int len = 0;
+#pragma unroll
+            for (int u = 0; u < kMRows; ++u) len = max(len, ke[u] - ka[u]);
+            for (int d = 0; d < len; d += kU) {
+                T xv[kMRows][kU], vv[kMRows][kU];
+#pragma unroll
+                for (int u = 0; u < kMRows; ++u)
+#pragma unroll
+                    for (int w = 0; w < kU; ++w) {
+                        const int k = ka[u] + d + w;
+                        if (k < ke[u]) {
+                            vv[u][w] = (T)sval[k];
+                            xv[u][w] = x[(size_t)scol[k] * G + j];
+                        }
+                    }
+#pragma unroll
+                for (int u = 0; u < kMRows; ++u)
+#pragma unroll
+                    for (int w = 0; w < kU; ++w)
+                        if (ka[u] + d + w < ke[u]) s[u] += vv[u][w] * xv[u][w];
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < kMRows; ++u) {
+                const int row = r0 + q + u * kLanesRows;
+                if (row < r1)
+                    for (int k = rowptr[row]; k < rowptr[row + 1]; ++k) s[u] += (T)val[k] * x[(size_t)col[k] * G + j];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kMRows; ++u) {
+            const int row = r0 + q + u * kLanesRows;
+            if (row < r1) f(row, j, s[u]);
+        }
+        r0 = n0;
+        r1 = n1;
+        k0 = m0;
+        k1 = m1;
+    }
 }
 
 // r = b - A x (or b when x = 0), p = z = M^-1 r; partials (r, z) and (b, M^-1 b) per column
 template <int G, typename T, typename VT, bool JAC>
-__global__ __launch_bounds__(kBlock) void k_m_init(int n, const int* __restrict__ rowptr, const int* __restrict__ col,
+__global__ __launch_bounds__(kBlock) void k_m_init(int n, int ntiles, const int* __restrict__ tiles,
+                                                   const int* __restrict__ rowptr, const int* __restrict__ col,
                                                    const VT* __restrict__ val, const T* __restrict__ x,
                                                    const T* __restrict__ b, const T* __restrict__ invd, T* __restrict__ r,
                                                    T* __restrict__ p, double* __restrict__ part, int x_zero)
 {
-    const int j = threadIdx.x % G;
     double v[2] = {0.0, 0.0};
-    for (int i = blockIdx.x * (kBlock / G) + threadIdx.x / G; i < n; i += gridDim.x * (kBlock / G)) {
+    auto row_init = [&](int i, int j, T ax) {
         const size_t o = (size_t)i * G + j;
         const T bb = b[o];
-        const T rr = x_zero ? bb : bb - group_row_product<G, T, VT>(i, j, rowptr, col, val, x);
+        const T rr = bb - ax;
         const T zz = JAC ? rr * invd[i] : rr;
         const T mb = JAC ? bb * invd[i] : bb;
         r[o] = rr;
         p[o] = zz;
         v[0] += (double)(rr * zz);
         v[1] += (double)(bb * mb);
+    };
+    if (x_zero) {
+        const int j = threadIdx.x % G;
+        for (int i = blockIdx.x * (kBlock / G) + threadIdx.x / G; i < n; i += gridDim.x * (kBlock / G))
+            row_init(i, j, (T)0);
+    } else {
+        for_tile_rows<G, T, VT>(ntiles, tiles, rowptr, col, val, x, row_init);
     }
     group_partials<G, 2>(v, part);
 }
@@ -147,19 +254,18 @@ __global__ void k_m_begin(MState* st, const double* __restrict__ part)
 
 // q = A p; partials (p, q) per column
 template <int G, typename T, typename VT>
-__global__ __launch_bounds__(kBlock) void k_m_spmv(int n, const int* __restrict__ rowptr, const int* __restrict__ col,
-                                                   const VT* __restrict__ val, const T* __restrict__ p, T* __restrict__ q,
-                                                   const MState* st, double* __restrict__ part)
+__global__ __launch_bounds__(kBlock) void k_m_spmv(int ntiles, const int* __restrict__ tiles, const int* __restrict__ rowptr,
+                                                   const int* __restrict__ col, const VT* __restrict__ val,
+                                                   const T* __restrict__ p, T* __restrict__ q, const MState* st,
+                                                   double* __restrict__ part)
 {
     if (st->all_done) return;
-    const int j = threadIdx.x % G;
     double v[1] = {0.0};
-    for (int i = blockIdx.x * (kBlock / G) + threadIdx.x / G; i < n; i += gridDim.x * (kBlock / G)) {
+    for_tile_rows<G, T, VT>(ntiles, tiles, rowptr, col, val, p, [&](int i, int j, T ap) {
         const size_t o = (size_t)i * G + j;
-        const T ap = group_row_product<G, T, VT>(i, j, rowptr, col, val, p);
         q[o] = ap;
         v[0] += (double)(p[o] * ap);
-    }
+    });
     group_partials<G, 1>(v, part);
 }
 
@@ -264,68 +370,244 @@ __global__ void k_m_unpack(int n, int ncol, const double* __restrict__ s, double
     }
 }
 
-// FP64 refinement: r = b - A x (FP64); partials (r, M^-1 r) and (b, M^-1 b) per column
+// ---- reliable-update PCG (PFEM_PRECISION_MIXED)
+// r = b - A x (FP64), r_s = (float) r, p_s = D_s r_s, x_s = 0; partials (r, D r) and (b, D b) in FP64
 template <int G, bool JAC>
-__global__ __launch_bounds__(kBlock) void k_m_ir_residual(int n, const int* __restrict__ rowptr, const int* __restrict__ col,
-                                                          const double* __restrict__ val, const double* __restrict__ x,
-                                                          const double* __restrict__ b, const double* __restrict__ invd,
-                                                          double* __restrict__ r, double* __restrict__ part)
+__global__ __launch_bounds__(kBlock) void k_mr_init(int ntiles, const int* __restrict__ tiles, const int* __restrict__ rowptr,
+                                                    const int* __restrict__ col, const double* __restrict__ val,
+                                                    const double* __restrict__ x, const double* __restrict__ b,
+                                                    const double* __restrict__ invd, const float* __restrict__ invd32,
+                                                    float* __restrict__ rs, float* __restrict__ ps, float* __restrict__ xs,
+                                                    double* __restrict__ part)
 {
-    const int j = threadIdx.x % G;
     double v[2] = {0.0, 0.0};
-    for (int i = blockIdx.x * (kBlock / G) + threadIdx.x / G; i < n; i += gridDim.x * (kBlock / G)) {
+    for_tile_rows<G, double, double>(ntiles, tiles, rowptr, col, val, x, [&](int i, int j, double ax) {
         const size_t o = (size_t)i * G + j;
         const double bb = b[o];
-        const double rr = bb - group_row_product<G, double, double>(i, j, rowptr, col, val, x);
-        r[o] = rr;
+        const double rr = bb - ax;
+        const float r32 = (float)rr;
+        rs[o] = r32;
+        ps[o] = JAC ? r32 * invd32[i] : r32;
+        xs[o] = 0.0f;
         const double d = JAC ? invd[i] : 1.0;
         v[0] += rr * (rr * d);
         v[1] += bb * (bb * d);
-    }
+    });
     group_partials<G, 2>(v, part);
 }
 
-// refinement test per column (the FP64 solver's criterion) and the scale of its next correction
 template <int G>
-__global__ void k_m_ir_test(MState* st, const double* __restrict__ part)
+__global__ void k_mr_begin(MState* st, const double* __restrict__ part)
 {
     double tot[2 * G];
     series_totals<2 * G>(part, tot);
     if (threadIdx.x != 0) return;
     int all = 1;
     for (int j = 0; j < G; ++j) {
-        if (st->irfirst) {
-            const bool rel = st->tol_mode == PFEM_TOL_RELATIVE && tot[G + j] > 0.0;
-            st->irthr[j] = rel ? st->tol * st->tol * tot[G + j] : st->tol * st->tol;
+        const double rz = tot[j], bmb = tot[G + j];
+        st->thr[j] = st->tol_mode == PFEM_TOL_RELATIVE && bmb > 0.0 ? st->tol * st->tol * bmb : st->tol * st->tol;
+        st->rz[j] = st->rzmax[j] = rz;
+        st->res[j] = sqrt(fabs(rz));
+        st->it[j] = 0;
+        st->mode[j] = 0;
+        st->nrep[j] = 0;
+        st->nan[j] = rz != rz;
+        st->converged[j] = rz < st->thr[j] ? 1 : 0;
+        st->done[j] = j >= st->ncol || st->converged[j] || st->nan[j];
+        all &= st->done[j];
+    }
+    st->all_done = all;
+}
+
+// Iterating columns: q_s = A_s p_s, partial (p_s, q_s). Columns due a replacement: r = b - A x with the
+// FP64 matrix, r_s = (float) r, partial (r, D r). The tile stages FP64 values only while some column is
+// being replaced ((float) of the FP64 value is the FP32 copy's value, so the FP32 products do not change).
+template <int G, bool JAC>
+__global__ __launch_bounds__(kBlock) void k_mr_spmv(int ntiles, const int* __restrict__ tiles, const int* __restrict__ rowptr,
+                                                    const int* __restrict__ col, const float* __restrict__ val32,
+                                                    const double* __restrict__ val, const float* __restrict__ ps,
+                                                    float* __restrict__ qs, const double* __restrict__ x,
+                                                    const double* __restrict__ b, const double* __restrict__ invd,
+                                                    float* __restrict__ rs, const MState* st, double* __restrict__ part)
+{
+    if (st->all_done) return;
+    int anyrep = 0;
+    for (int c = 0; c < G; ++c) anyrep |= st->mode[c] == 1 && !st->done[c];
+    double v[1] = {0.0};
+    if (!anyrep) {
+        for_tile_rows<G, float, float>(ntiles, tiles, rowptr, col, val32, ps, [&](int i, int j, float ap) {
+            const size_t o = (size_t)i * G + j;
+            qs[o] = ap;
+            v[0] += (double)(ps[o] * ap);
+        });
+    } else {
+        const int j = threadIdx.x % G;
+        const bool rep = st->mode[j] == 1;
+        __shared__ double sval[kMTileNnz];
+        __shared__ int scol[kMTileNnz];
+        const int nb = gridDim.x, bk = blockIdx.x;
+        const int rb = (bk % 8) * (nb / 8) + bk / 8;
+        const int t0 = (int)((long)ntiles * rb / nb), t1 = (int)((long)ntiles * (rb + 1) / nb);
+        for (int t = t0; t < t1; ++t) {
+            const int r0 = tiles[2 * t], r1 = tiles[2 * t + 1];
+            const int k0 = rowptr[r0], nk = rowptr[r1] - k0;
+            const bool staged = nk <= kMTileNnz;
+            if (staged) {
+                __syncthreads();
+                for (int k = threadIdx.x; k < nk; k += kBlock) {
+                    sval[k] = val[k0 + k];
+                    scol[k] = col[k0 + k];
+                }
+                __syncthreads();
+            }
+            for (int u = 0; u < kMRows; ++u) {
+                const int row = r0 + (int)threadIdx.x / G + u * (kBlock / G);
+                if (row >= r1) continue;
+                const size_t o = (size_t)row * G + j;
+                const int ka = rowptr[row], ke = rowptr[row + 1];
+                if (rep) {
+                    double ax = 0.0;
+                    for (int k = ka; k < ke; ++k) {
+                        const double a = staged ? sval[k - k0] : val[k];
+                        const int cc = staged ? scol[k - k0] : col[k];
+                        ax += a * x[(size_t)cc * G + j];
+                    }
+                    const double rr = b[o] - ax;
+                    rs[o] = (float)rr;
+                    v[0] += rr * (rr * (JAC ? invd[row] : 1.0));
+                } else {
+                    float ap = 0.0f;
+                    for (int k = ka; k < ke; ++k) {
+                        const float a = staged ? (float)sval[k - k0] : val32[k];
+                        const int cc = staged ? scol[k - k0] : col[k];
+                        ap += a * ps[(size_t)cc * G + j];
+                    }
+                    qs[o] = ap;
+                    v[0] += (double)(ps[o] * ap);
+                }
+            }
         }
-        st->irrz[j] = tot[j];
-        st->irnan[j] = tot[j] != tot[j];
-        st->irconv[j] = tot[j] < st->irthr[j] ? 1 : 0;
-        const bool stop = j >= st->ncol || st->irconv[j] || st->irnan[j];
-        // a finished column gets a zero inner right-hand side (scale 0): its inner solve stops at once
-        st->scale[j] = stop ? 0.0 : (sqrt(tot[j]) > 0.0 ? sqrt(tot[j]) : 1.0);
-        all &= stop ? 1 : 0;
     }
-    st->irfirst = 0;
-    st->irdone = all;
+    group_partials<G, 1>(v, part);
 }
 
+// alpha = gamma / (p_s, q_s) for iterating columns; for a replaced column the true gamma: the stopping
+// test, then beta for the direction update; one block
 template <int G>
-__global__ void k_m_ir_down(int n, float* __restrict__ d, const double* __restrict__ s, const MState* st)
+__global__ void k_mr_alpha(MState* st, const double* __restrict__ part)
 {
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < (long)n * G; t += (long)gridDim.x * blockDim.x) {
-        const double sc = st->scale[t % G];
-        d[t] = sc > 0.0 ? (float)(s[t] / sc) : 0.0f;
+    if (st->all_done) return;
+    double tot[G];
+    series_totals<G>(part, tot);
+    if (threadIdx.x != 0) return;
+    int all = 1;
+    for (int j = 0; j < G; ++j) {
+        if (!st->done[j]) {
+            if (st->mode[j] == 1) {
+                const double g = tot[j];
+                st->nrep[j] += 1;
+                st->res[j] = sqrt(fabs(g));
+                if (g != g) {
+                    st->nan[j] = 1;
+                    st->done[j] = 1;
+                } else if (g < st->thr[j]) {
+                    st->converged[j] = 1;
+                    st->done[j] = 1;
+                } else {
+                    st->beta[j] = g / st->rz[j];
+                    st->rz[j] = st->rzmax[j] = g;
+                    st->mode[j] = 2;
+                }
+            } else {
+                st->mode[j] = 0;
+                st->alpha[j] = st->rz[j] / tot[j];
+            }
+        }
+        all &= st->done[j];
+    }
+    st->all_done = all;
+}
+
+// x_s += alpha p_s, r_s -= alpha q_s for iterating columns (FP32); partial (r_s, D_s r_s)
+template <int G, bool JAC>
+__global__ void k_mr_update(int n, float* __restrict__ xs, float* __restrict__ rs, const float* __restrict__ ps,
+                            const float* __restrict__ qs, const float* __restrict__ invd32, const MState* st,
+                            double* __restrict__ part)
+{
+    if (st->all_done) return;
+    const int j = threadIdx.x % G;
+    const bool active = !st->done[j] && st->mode[j] == 0;
+    const float alpha = (float)st->alpha[j];
+    double v[1] = {0.0};
+    for (int i = blockIdx.x * (kBlock / G) + threadIdx.x / G; i < n && active; i += gridDim.x * (kBlock / G)) {
+        const size_t o = (size_t)i * G + j;
+        xs[o] += alpha * ps[o];
+        const float rr = rs[o] - alpha * qs[o];
+        rs[o] = rr;
+        v[0] += (double)(rr * (JAC ? rr * invd32[i] : rr));
+    }
+    group_partials<G, 1>(v, part);
+}
+
+// gamma of the recursive residual: the iteration count, then either beta or a replacement (after a 10x
+// drop since the last one, or when the recursive residual passes the stopping test); one block
+template <int G>
+__global__ void k_mr_beta(MState* st, const double* __restrict__ part)
+{
+    if (st->all_done) return;
+    double tot[G];
+    series_totals<G>(part, tot);
+    if (threadIdx.x != 0) return;
+    int all = 1;
+    for (int j = 0; j < G; ++j) {
+        if (!st->done[j] && st->mode[j] == 0) {
+            const double g = tot[j];
+            st->it[j] += 1;
+            if (g != g) {
+                st->nan[j] = 1;
+                st->done[j] = 1;
+            } else if (st->it[j] >= st->maxit) {
+                st->done[j] = 1;
+            } else if (g < st->thr[j] || g < st->drop * st->rzmax[j]) {
+                st->mode[j] = 1;
+            } else {
+                st->beta[j] = g / st->rz[j];
+                st->rz[j] = g;
+            }
+        }
+        all &= st->done[j];
+    }
+    st->all_done = all;
+}
+
+// p_s = D_s r_s + beta p_s (iterating columns and after a replacement); a column due a replacement
+// first moves its correction into the FP64 solution: x += x_s, x_s = 0
+template <int G, bool JAC>
+__global__ void k_mr_p(int n, float* __restrict__ ps, const float* __restrict__ rs, const float* __restrict__ invd32,
+                       double* __restrict__ x, float* __restrict__ xs, const MState* st)
+{
+    if (st->all_done) return;
+    const int j = threadIdx.x % G;
+    if (st->done[j]) return;
+    const int mode = st->mode[j];
+    const float beta = (float)st->beta[j];
+    for (int i = blockIdx.x * (kBlock / G) + threadIdx.x / G; i < n; i += gridDim.x * (kBlock / G)) {
+        const size_t o = (size_t)i * G + j;
+        if (mode == 1) {
+            x[o] += (double)xs[o];
+            xs[o] = 0.0f;
+        } else {
+            const float zz = JAC ? rs[o] * invd32[i] : rs[o];
+            ps[o] = zz + beta * ps[o];
+        }
     }
 }
 
-template <int G>
-__global__ void k_m_ir_up(int n, double* __restrict__ x, const float* __restrict__ d, const MState* st)
+// x += x_s over every column (a converged column's x_s is zero: its last replacement folded it in)
+__global__ void k_mr_fold(long m, double* __restrict__ x, const float* __restrict__ xs)
 {
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < (long)n * G; t += (long)gridDim.x * blockDim.x) {
-        const double sc = st->scale[t % G];
-        if (sc > 0.0) x[t] += sc * (double)d[t];
-    }
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < m) x[t] += (double)xs[t];
 }
 
 __global__ void k_m_to32(long n, float* __restrict__ d, const double* __restrict__ s)
@@ -336,9 +618,12 @@ __global__ void k_m_to32(long n, float* __restrict__ d, const double* __restrict
 
 // ---- host side
 struct MultiWork {
-    int n = 0, G = 0;
+    int n = 0, G = 0, ncol = 0;
+    const Pattern* pat = nullptr;
+    DevArray<int> tiles;                // row tiles of the products (for_tile_rows)
+    int ntiles = 0;
     DevArray<double> x, b, r, p, q;    // FP64 interleaved (x, b also the refinement's solution and RHS)
-    DevArray<float> x32, b32, r32, p32, q32;
+    DevArray<float> x32, r32, p32, q32;  // reliable updates: x_s, r_s, p_s, q_s
     DevArray<double> part;
     DevArray<MState> st;
     MState* h = nullptr;                // pinned, two slots
@@ -364,14 +649,13 @@ static void m_cg_run(Context* c, pfem_solver_t* s, MultiWork& w, const Pattern& 
     init.tol_mode = tol_mode;
     init.maxit = maxit;
     init.ncol = ncol;
-    // refinement fields are kept across inner passes
-    PFEM_HIP(hipMemcpyAsync(w.st.p, &init, offsetof(MState, irthr), hipMemcpyHostToDevice, c->stream));
+    PFEM_HIP(hipMemcpyAsync(w.st.p, &init, sizeof(MState), hipMemcpyHostToDevice, c->stream));
     if (x_zero) PFEM_HIP(hipMemsetAsync(x, 0, sizeof(T) * (size_t)n * G, c->stream));
     double* pa = w.part.p;
     double* pb = w.part.p + (size_t)2 * G * kMGrid;
     const dim3 gr(kMGrid), bl(kBlock);
-    if (jac) hipLaunchKernelGGL((k_m_init<G, T, VT, true>), gr, bl, 0, c->stream, n, P.rowptr.p, P.col.p, val, x, b, invd, r, p, pa, x_zero);
-    else hipLaunchKernelGGL((k_m_init<G, T, VT, false>), gr, bl, 0, c->stream, n, P.rowptr.p, P.col.p, val, x, b, invd, r, p, pa, x_zero);
+    if (jac) hipLaunchKernelGGL((k_m_init<G, T, VT, true>), gr, bl, 0, c->stream, n, w.ntiles, w.tiles.p, P.rowptr.p, P.col.p, val, x, b, invd, r, p, pa, x_zero);
+    else hipLaunchKernelGGL((k_m_init<G, T, VT, false>), gr, bl, 0, c->stream, n, w.ntiles, w.tiles.p, P.rowptr.p, P.col.p, val, x, b, invd, r, p, pa, x_zero);
     hipLaunchKernelGGL((k_m_begin<G>), dim3(1), bl, 0, c->stream, w.st.p, (const double*)pa);
     PFEM_HIP(hipGetLastError());
     const int batch = s->poll;
@@ -379,7 +663,7 @@ static void m_cg_run(Context* c, pfem_solver_t* s, MultiWork& w, const Pattern& 
     while (true) {
         for (int jb = 0; jb < batch; ++jb) {
             const int k = launched + jb;
-            hipLaunchKernelGGL((k_m_spmv<G, T, VT>), gr, bl, 0, c->stream, n, P.rowptr.p, P.col.p, val, p, q, w.st.p, pa);
+            hipLaunchKernelGGL((k_m_spmv<G, T, VT>), gr, bl, 0, c->stream, w.ntiles, w.tiles.p, P.rowptr.p, P.col.p, val, p, q, w.st.p, pa);
             hipLaunchKernelGGL((k_m_alpha<G>), dim3(1), bl, 0, c->stream, w.st.p, (const double*)pa, k);
             if (jac) hipLaunchKernelGGL((k_m_update<G, T, true>), gr, bl, 0, c->stream, n, x, r, p, q, invd, w.st.p, pb);
             else hipLaunchKernelGGL((k_m_update<G, T, false>), gr, bl, 0, c->stream, n, x, r, p, q, invd, w.st.p, pb);
@@ -403,17 +687,90 @@ static void m_cg_run(Context* c, pfem_solver_t* s, MultiWork& w, const Pattern& 
     PFEM_HIP(hipStreamSynchronize(c->stream));
 }
 
+// Reliable-update PCG over the interleaved system (x, b in w.x, w.b); ends with the state in h[0]
+template <int G>
+static void m_ru_run(Context* c, pfem_solver_t* s, MultiWork& w, const Pattern& P, pfem_csr_t* a, bool jac)
+{
+    const int n = P.n;
+    const size_t m = (size_t)n * G;
+    if (!w.x32.p) {
+        w.x32.alloc(m); w.r32.alloc(m); w.p32.alloc(m); w.q32.alloc(m);
+    }
+    if (!a->val32.p) a->val32.alloc(P.nnz);
+    hipLaunchKernelGGL(k_m_to32, dim3(blocks_for(P.nnz)), dim3(kBlock), 0, c->stream, (long)P.nnz, a->val32.p, a->val.p);
+    const double* invd = jac ? s->pc->invd.p : nullptr;
+    const float* invd32 = jac ? s->pc->invd32.p : nullptr;
+    PFEM_REQUIRE(!jac || invd32, "FP32 Jacobi diagonal missing");
+    MState init{};
+    init.tol = s->tol;
+    init.tol_mode = s->tol_mode;
+    init.maxit = s->maxit;
+    init.ncol = w.ncol;
+    init.drop = std::pow(10.0, -s->replace_digits);
+    PFEM_HIP(hipMemcpyAsync(w.st.p, &init, sizeof(MState), hipMemcpyHostToDevice, c->stream));
+    double* pa = w.part.p;
+    double* pb = w.part.p + (size_t)2 * G * kMGrid;
+    float *xs = w.x32.p, *rs = w.r32.p, *ps = w.p32.p, *qs = w.q32.p;
+    const dim3 gr(kMGrid), bl(kBlock);
+    if (jac) hipLaunchKernelGGL((k_mr_init<G, true>), gr, bl, 0, c->stream, w.ntiles, w.tiles.p, P.rowptr.p, P.col.p, a->val.p, (const double*)w.x.p, (const double*)w.b.p, invd, invd32, rs, ps, xs, pa);
+    else hipLaunchKernelGGL((k_mr_init<G, false>), gr, bl, 0, c->stream, w.ntiles, w.tiles.p, P.rowptr.p, P.col.p, a->val.p, (const double*)w.x.p, (const double*)w.b.p, invd, invd32, rs, ps, xs, pa);
+    hipLaunchKernelGGL((k_mr_begin<G>), dim3(1), bl, 0, c->stream, w.st.p, (const double*)pa);
+    PFEM_HIP(hipGetLastError());
+    const int batch = s->poll;
+    int launched = 0, pend = -1, slot = 0;
+    // a replacement takes one extra slot per tenfold residual drop; the slot budget allows for them
+    const int slots = s->maxit + s->maxit / 4 + 64;
+    while (true) {
+        for (int jb = 0; jb < batch; ++jb) {
+            if (jac) hipLaunchKernelGGL((k_mr_spmv<G, true>), gr, bl, 0, c->stream, w.ntiles, w.tiles.p, P.rowptr.p, P.col.p, (const float*)a->val32.p, (const double*)a->val.p, (const float*)ps, qs, (const double*)w.x.p, (const double*)w.b.p, invd, rs, (const MState*)w.st.p, pa);
+            else hipLaunchKernelGGL((k_mr_spmv<G, false>), gr, bl, 0, c->stream, w.ntiles, w.tiles.p, P.rowptr.p, P.col.p, (const float*)a->val32.p, (const double*)a->val.p, (const float*)ps, qs, (const double*)w.x.p, (const double*)w.b.p, invd, rs, (const MState*)w.st.p, pa);
+            hipLaunchKernelGGL((k_mr_alpha<G>), dim3(1), bl, 0, c->stream, w.st.p, (const double*)pa);
+            if (jac) hipLaunchKernelGGL((k_mr_update<G, true>), gr, bl, 0, c->stream, n, xs, rs, (const float*)ps, (const float*)qs, invd32, (const MState*)w.st.p, pb);
+            else hipLaunchKernelGGL((k_mr_update<G, false>), gr, bl, 0, c->stream, n, xs, rs, (const float*)ps, (const float*)qs, invd32, (const MState*)w.st.p, pb);
+            hipLaunchKernelGGL((k_mr_beta<G>), dim3(1), bl, 0, c->stream, w.st.p, (const double*)pb);
+            if (jac) hipLaunchKernelGGL((k_mr_p<G, true>), gr, bl, 0, c->stream, n, ps, (const float*)rs, invd32, w.x.p, xs, (const MState*)w.st.p);
+            else hipLaunchKernelGGL((k_mr_p<G, false>), gr, bl, 0, c->stream, n, ps, (const float*)rs, invd32, w.x.p, xs, (const MState*)w.st.p);
+        }
+        launched += batch;
+        PFEM_HIP(hipGetLastError());
+        PFEM_HIP(hipMemcpyAsync(w.h + slot, w.st.p, sizeof(MState), hipMemcpyDeviceToHost, c->stream));
+        PFEM_HIP(hipEventRecord(w.ev[slot], c->stream));
+        if (pend >= 0) {
+            PFEM_HIP(hipEventSynchronize(w.ev[pend]));
+            if (w.h[pend].all_done || launched > slots + 2 * batch) break;
+        }
+        pend = slot;
+        slot ^= 1;
+    }
+    // columns still iterating at the slot limit keep their FP64 part; their FP32 correction is folded in
+    hipLaunchKernelGGL(k_mr_fold, dim3(blocks_for((long)m)), bl, 0, c->stream, (long)m, w.x.p, (const float*)xs);
+    PFEM_HIP(hipMemcpyAsync(w.h, w.st.p, sizeof(MState), hipMemcpyDeviceToHost, c->stream));
+    PFEM_HIP(hipStreamSynchronize(c->stream));
+}
+
 template <int G>
 static void solve_multi_g(Context* c, pfem_solver_t* s, pfem_csr_t* a, int ncol, double* X, const double* B,
                           pfem_solve_stats* stats)
 {
     const Pattern& P = *a->pat;
     const int n = P.n;
-    if (!s->multi || s->multi->G != G || s->multi->n != n) {
+    if (!s->multi || s->multi->G != G || s->multi->n != n || s->multi->pat != &P) {
         s->multi.reset(new MultiWork());
         MultiWork& w = *s->multi;
         w.n = n;
         w.G = G;
+        w.pat = &P;
+        std::vector<int> tiles;
+        for (int r = 0; r < n;) {
+            int r1 = r;
+            const int base = P.h_rowptr[r];
+            while (r1 < n && r1 - r < kMRows * kBlock / G && (P.h_rowptr[r1 + 1] - base <= kMTileNnz || r1 == r)) ++r1;
+            tiles.push_back(r);
+            tiles.push_back(r1);
+            r = r1;
+        }
+        w.ntiles = (int)tiles.size() / 2;
+        upload(c, w.tiles, tiles);
         const size_t m = (size_t)n * G;
         w.x.alloc(m); w.b.alloc(m); w.r.alloc(m); w.p.alloc(m); w.q.alloc(m);
         w.part.alloc((size_t)4 * G * kMGrid);
@@ -422,6 +779,7 @@ static void solve_multi_g(Context* c, pfem_solver_t* s, pfem_csr_t* a, int ncol,
         for (hipEvent_t& e : w.ev) PFEM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
     MultiWork& w = *s->multi;
+    w.ncol = ncol;
     const size_t m = (size_t)n * G;
     const unsigned gp = (unsigned)std::min<size_t>(blocks_for((long)m), 8192);
     const bool jac = s->pc && s->pc->kind == PFEM_PRECOND_JACOBI;
@@ -439,48 +797,13 @@ static void solve_multi_g(Context* c, pfem_solver_t* s, pfem_csr_t* a, int ncol,
             st[j].nan_detected = w.h->nan[j];
         }
     } else {
-        if (!w.x32.p) {
-            w.x32.alloc(m); w.b32.alloc(m); w.r32.alloc(m); w.p32.alloc(m); w.q32.alloc(m);
-        }
-        if (!a->val32.p) a->val32.alloc(P.nnz);
-        hipLaunchKernelGGL(k_m_to32, dim3(blocks_for(P.nnz)), dim3(kBlock), 0, c->stream, (long)P.nnz, a->val32.p, a->val.p);
-        MState init{};
-        init.tol = s->tol;
-        init.tol_mode = s->tol_mode;
-        init.ncol = ncol;
-        init.irfirst = 1;
-        PFEM_HIP(hipMemcpyAsync(w.st.p, &init, sizeof(MState), hipMemcpyHostToDevice, c->stream));
-        const double* invd = jac ? s->pc->invd.p : nullptr;
-        const float* invd32 = jac ? s->pc->invd32.p : nullptr;
-        PFEM_REQUIRE(!jac || invd32, "FP32 Jacobi diagonal missing");
-        std::vector<int> inner(ncol, 0);
-        int refinements = 0, total = 0;
-        while (true) {
-            if (jac) hipLaunchKernelGGL((k_m_ir_residual<G, true>), dim3(kMGrid), dim3(kBlock), 0, c->stream, n, P.rowptr.p, P.col.p, a->val.p, (const double*)w.x.p, (const double*)w.b.p, invd, w.r.p, w.part.p);
-            else hipLaunchKernelGGL((k_m_ir_residual<G, false>), dim3(kMGrid), dim3(kBlock), 0, c->stream, n, P.rowptr.p, P.col.p, a->val.p, (const double*)w.x.p, (const double*)w.b.p, invd, w.r.p, w.part.p);
-            hipLaunchKernelGGL((k_m_ir_test<G>), dim3(1), dim3(kBlock), 0, c->stream, w.st.p, (const double*)w.part.p);
-            PFEM_HIP(hipMemcpyAsync(w.h, w.st.p, sizeof(MState), hipMemcpyDeviceToHost, c->stream));
-            PFEM_HIP(hipStreamSynchronize(c->stream));
-            if (w.h->irdone || refinements >= s->max_refine || total >= s->maxit) break;
-            hipLaunchKernelGGL((k_m_ir_down<G>), dim3(gp), dim3(kBlock), 0, c->stream, n, w.b32.p, (const double*)w.r.p, (const MState*)w.st.p);
-            const int maxit = std::max(1, s->maxit - total);
-            m_cg_run<G, float, float>(c, s, w, P, a->val32.p, invd32, w.x32.p, w.b32.p, w.r32.p, w.p32.p, w.q32.p, 1,
-                                      s->inner_tol, PFEM_TOL_RELATIVE, maxit, ncol);
-            int most = 0;
-            for (int j = 0; j < ncol; ++j) {
-                inner[j] += w.h->it[j];
-                most = std::max(most, w.h->it[j]);
-            }
-            total += most;
-            hipLaunchKernelGGL((k_m_ir_up<G>), dim3(gp), dim3(kBlock), 0, c->stream, n, w.x.p, (const float*)w.x32.p, (const MState*)w.st.p);
-            ++refinements;
-        }
+        m_ru_run<G>(c, s, w, P, a, jac);
         for (int j = 0; j < ncol; ++j) {
-            st[j].converged = w.h->irconv[j];
-            st[j].iterations = inner[j];
-            st[j].final_residual = std::sqrt(std::fabs(w.h->irrz[j]));
-            st[j].nan_detected = w.h->irnan[j];
-            st[j].refinements = refinements;
+            st[j].converged = w.h->converged[j];
+            st[j].iterations = w.h->it[j];
+            st[j].final_residual = w.h->res[j];
+            st[j].nan_detected = w.h->nan[j];
+            st[j].refinements = w.h->nrep[j];
         }
     }
     hipLaunchKernelGGL((k_m_unpack<G>), dim3(gp), dim3(kBlock), 0, c->stream, n, ncol, (const double*)w.x.p, X);

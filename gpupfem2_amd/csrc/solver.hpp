@@ -185,6 +185,7 @@ struct pfem_solver_t {
     // PCG kernel fusion in the factor's ordering: 0 none; 1 x/r update inside the forward sweep; 2 also
     // alpha and beta finished by the last workgroups of the SpMV and of the backward sweep
     int fuse = 2;
+    int replace_digits = 2;   // reliable updates (multi/batched MIXED): replace after gamma fell 10^-digits
     hipEvent_t ev[2] = {nullptr, nullptr};   // pipelined polls of the device state
     std::unique_ptr<pfem::MultiWork, pfem::MultiWorkDeleter> multi;   // pfem_solver_solve_multi
     std::unique_ptr<pfem::BatchWork, pfem::BatchWorkDeleter> batch;   // pfem_solver_solve_batched

@@ -197,6 +197,8 @@ int pfem_solver_set_precision(pfem_solver s, int precision, double inner_tol, in
  * 4 for GMRES): iterations launched between two host reads of the convergence flag. */
 #define PFEM_SOLVER_OPT_FACTOR_ORDER 1
 #define PFEM_SOLVER_OPT_POLL 2
+#define PFEM_SOLVER_OPT_REPLACE 4 /* multi-RHS / batched PFEM_PRECISION_MIXED, default 2: a residual replacement
+                                    follows every fall of (r, M^-1 r) by 10^-value (value 1..8) */
 #define PFEM_SOLVER_OPT_FUSE 3   /* PCG kernel fusion in factor order, default 2: 0 none; 1 the x/r update runs
                                     inside the forward sweep; 2 also alpha and beta are finished by the last
                                     workgroups of the SpMV and of the backward sweep */
@@ -210,15 +212,17 @@ int pfem_solver_solve(pfem_solver s, pfem_csr a, double* d_x, const double* d_b,
  * B at d_X + j*n and d_B + j*n (X is also the initial guess). Each column runs the solver's PCG recurrence
  * and stopping test with its own scalars and stops on its own; the matrix is streamed once per iteration
  * for all columns. The solver must be PCG with a Jacobi preconditioner or none, in PFEM_PRECISION_FP64 or
- * PFEM_PRECISION_MIXED (FP32 inner multi-RHS CG, FP64 refinement per column). stats: nrhs entries (or NULL). */
+ * PFEM_PRECISION_MIXED (PCG with reliable updates: FP32 vectors and matrix copy in the iterations, the solution
+ * in FP64, FP64 residual replacement after every tenfold drop and for the stopping test; stats.refinements
+ * counts the replacements). stats: nrhs entries (or NULL). */
 int pfem_solver_solve_multi(pfem_solver s, pfem_csr a, int nrhs, double* d_X, const double* d_B,
                             pfem_solve_stats* stats);
 /* Batched independent PCG solves (no counterpart in the reference, which solves one system per
  * SolverCG::solve call, linear_solver.cu:301-376): a is block diagonal, system k owning rows and columns
  * [h_sys_rows[k], h_sys_rows[k+1]) (h_sys_rows: nsys + 1 host ints from 0 to n; checked). Each system runs
  * the solver's PCG recurrence and stopping test with its own scalars and stops on its own; all systems
- * share every launch. Jacobi preconditioner or none; PFEM_PRECISION_FP64 or PFEM_PRECISION_MIXED (FP32
- * inner batched CG, FP64 refinement per system). d_x is the initial guess (FP64 path) and the solution.
+ * share every launch. Jacobi preconditioner or none; PFEM_PRECISION_FP64 or PFEM_PRECISION_MIXED (reliable
+ * updates per system, as pfem_solver_solve_multi). d_x is the initial guess (FP64 path) and the solution.
  * stats: nsys entries (or NULL); h_time_ms (or NULL): per-system time to solution on the device clock,
  * from the start of the call's first Krylov or refinement pass to the pass where the system stopped. */
 int pfem_solver_solve_batched(pfem_solver s, pfem_csr a, int nsys, const int* h_sys_rows, double* d_x,

@@ -2,8 +2,10 @@
 
 Each system runs the single-system PCG recurrence (linear_solver.cu:301-376) with its own scalars, so it is
 compared with the oracle's PCG on the system's own sub-matrix: relative L2 <= 1e-10 at tolerance 1e-12
-(dot products reduced in another order) and iteration counts within 3. The mixed-precision solve (FP32
-inner batched CG, FP64 refinement per system) must agree with the FP64 one to 1e-8 per system, and the
+(dot products reduced in another order) and iteration counts within 3. The mixed-precision solve (PCG with
+reliable updates per system) must agree with the FP64 one to 1e-8 per system with at most 60% more
+iterations (FP32 recurrences lose conjugacy faster on these 2D matrices, whose condition numbers reach
+1e5; measured: 1.3-1.45x), and the
 256-mesh batch (10k-500k triangles per mesh) is checked through each system's true residual.
 """
 import numpy as np
@@ -57,7 +59,7 @@ def test_batched_mixed_matches_fp64(ctx):
     mesh, a, b = cases.build_batched_poisson(ctx, case)
     rows = case["sys_rows"]
     n = rows[-1]
-    out = {}
+    out, its = {}, {}
     for prec in (capi.PRECISION_FP64, capi.PRECISION_MIXED):
         s = capi.Solver(a, capi.SOLVER_CG, 1e-12, 20000, 0, capi.TOL_RELATIVE, capi.Preconditioner(a, capi.PRECOND_JACOBI))
         if prec:
@@ -68,6 +70,8 @@ def test_batched_mixed_matches_fp64(ctx):
         if prec:
             assert all(st.refinements >= 1 for st in stats)
         out[prec] = x.download()
+        its[prec] = [st.iterations for st in stats]
+    assert all(m <= 1.6 * f + 5 for m, f in zip(its[1], its[0])), its
     for k in range(rows.size - 1):
         lo, hi = rows[k], rows[k + 1]
         assert rel_l2(out[1][lo:hi], out[0][lo:hi]) < 1e-8, k

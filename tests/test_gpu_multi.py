@@ -5,7 +5,8 @@ compared with the library's single-RHS solve and with the oracle's PCG on the sa
 <= 1e-10 at tolerance 1e-12 (the dot products are reduced in another order) and iteration counts within 3.
 The 3D box of 128^3 cubes with 8 right-hand sides (2.15M unknowns) is checked through size-independent
 properties: every column converges, its true residual ||b - A x|| / ||b|| meets the tolerance, and the
-mixed-precision solution (FP32 inner CG, FP64 refinement) agrees with the FP64 one to 1e-8.
+mixed-precision solution (PCG with reliable updates: FP32 iterations, FP64 residual replacement) agrees
+with the FP64 one to 1e-8 and needs at most 25% more iterations (the recurrence is never restarted).
 """
 import math
 
@@ -77,7 +78,7 @@ def test_multi_3d_oracle_and_mixed(ctx, oracle):
     rp, col, val = a.download()
     n = rp.size - 1
     Bh = B.download().reshape(8, n)
-    out = {}
+    out, its = {}, {}
     for prec in (capi.PRECISION_FP64, capi.PRECISION_MIXED):
         pc = capi.Preconditioner(a, capi.PRECOND_JACOBI)
         s = capi.Solver(a, capi.SOLVER_CG, 1e-12, 4000, 0, capi.TOL_RELATIVE, pc)
@@ -89,6 +90,8 @@ def test_multi_3d_oracle_and_mixed(ctx, oracle):
         if prec:
             assert all(st.refinements >= 1 for st in stats)
         out[prec] = X.download().reshape(8, n)
+        its[prec] = [st.iterations for st in stats]
+    assert all(m <= 1.25 * f + 5 for m, f in zip(its[1], its[0])), its
     for j in range(8):
         xr = np.zeros(n)
         ost = oracle.pcg(rp, col, val, xr, Bh[j], 1e-12, 4000, 1, "jacobi")
