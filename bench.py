@@ -6,6 +6,7 @@ PCG + Jacobi, relative tolerance 1e-12), in the mixed-precision pipeline and in 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
     python bench.py --kernels      # only the pressure-solve kernels (for counter collection)
     python bench.py --workload poisson3d   # 8-RHS 3D Poisson solves (secondary workload)
+    python bench.py --workload batched     # 256 independent Poisson systems (secondary workload)
 
 One step = advection + relocation + refill + reorder of all particles, projection, the two outer
 iterations of prediction / pressure / correction (5 assemblies + 5 solves each), particle velocity
@@ -275,6 +276,62 @@ def poisson3d_workload(args):
     print(json.dumps(line))
 
 
+def batched_workload(args):
+    """Secondary workload: 256 independent pressure-Poisson systems on jittered meshes of 10k-500k
+    triangles (log-uniform), one block-diagonal matrix, Jacobi PCG per system to relative 1e-10, FP64 and
+    mixed (reliable updates). A step is one batched solve; value = systems solved per second."""
+    import numpy as np
+    from gpupfem2_amd import capi, cases
+    case = cases.batched_poisson_case()
+    ctx = capi.Context(0)
+    mesh, a, b = cases.build_batched_poisson(ctx, case)
+    rows = case["sys_rows"]
+    n = int(rows[-1])
+    out = {}
+    for prec in (0, 1):
+        s = capi.Solver(a, capi.SOLVER_CG, 1e-10, 20000, 0, capi.TOL_RELATIVE,
+                        capi.Preconditioner(a, capi.PRECOND_JACOBI))
+        if prec:
+            s.set_precision(prec, 1e-5, 30)
+        x = ctx.zeros(n)
+        times, t2s, its = [], [], []
+        for k in range(args.warmup + args.steps):
+            x.zero()
+            ctx.sync()
+            t = time.perf_counter()
+            st, tms = s.solve_batched(a, rows, x, b)
+            ctx.sync()
+            assert all(q.converged for q in st)
+            if k >= args.warmup:
+                times.append(time.perf_counter() - t)
+                t2s.append(tms)
+            its = [q.iterations for q in st]
+        dt = sum(times) / len(times)
+        t2s = np.concatenate(t2s)
+        out[prec] = {"s_per_batch": dt, "p50_ms": float(np.percentile(t2s, 50)), "p99_ms": float(np.percentile(t2s, 99)),
+                     "iterations": [int(min(its)), int(max(its))]}
+        print(f"[bench] batched {'mixed' if prec else 'fp64'}: {dt * 1e3:.1f} ms per batch of {rows.size - 1}",
+              file=sys.stderr, flush=True)
+    head = out[1]
+    nsys = int(rows.size - 1)
+    line = {
+        "metric": "batched pressure-Poisson PCG+Jacobi solves/s (256 meshes, 10k-500k triangles)",
+        "value": round(nsys / head["s_per_batch"], 2),
+        "unit": "solves/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(head["s_per_batch"] * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic (seeded jittered meshes, N(0,1) right-hand sides)",
+        "config": {"workload": "256 independent 2D Poisson systems, rel tol 1e-10", "triangles": int(case["tri"].shape[0]),
+                   "nodes": n, "parallelism": "1 GPU"},
+        "fp64_solves_per_s": round(nsys / out[0]["s_per_batch"], 2),
+        "time_to_solution_ms": {("mixed" if k else "fp64"): {"p50": round(v["p50_ms"], 1), "p99": round(v["p99_ms"], 1)}
+                                for k, v in out.items()},
+        "iterations": {("mixed" if k else "fp64"): v["iterations"] for k, v in out.items()},
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -286,13 +343,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernels", action="store_true", help="pressure-solve kernels only (counter runs)")
     ap.add_argument("--launches", type=int, default=20)
-    ap.add_argument("--workload", choices=["cylinder", "poisson3d"], default="cylinder",
-                    help="cylinder: the headline PFEM-2 step; poisson3d: 8-RHS 3D Poisson solves")
+    ap.add_argument("--workload", choices=["cylinder", "poisson3d", "batched"], default="cylinder",
+                    help="cylinder: the headline PFEM-2 step; poisson3d: 8-RHS 3D Poisson solves; "
+                         "batched: 256 independent Poisson systems")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)")
     if args.workload == "poisson3d":
         poisson3d_workload(args)
+        return
+    if args.workload == "batched":
+        batched_workload(args)
         return
     if args.kernels:
         kernels_only(args.h, args.launches)
