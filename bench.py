@@ -181,9 +181,9 @@ def cpu_baseline(case, st, mesh, gpu_its):
 def kernels_only(h, launches):
     """Pressure-system kernels alone: the cylinder mesh's P1 stiffness matrix (the pressure matrix:
     it depends on the geometry only) with the pressure Dirichlet rows, its multicolour IC0 factor, then
-    `launches` IC0 applies on vectors in the factor's ordering (FP32-stored factor, then FP64), FP64 SpMVs,
-    and device-to-device copies of
-    256 MiB (the calibration of the byte counters)."""
+    `launches` IC0 applies on vectors in the factor's ordering (FP32-stored factor, then FP64), FP64 SpMVs
+    in the natural and in the factor's ordering, and device-to-device copies of 256 MiB (the calibration
+    of the byte counters)."""
     import numpy as np
     from gpupfem2_amd import capi, cases
     case = cases.cylinder_case(h=h)
@@ -198,7 +198,8 @@ def kernels_only(h, launches):
     n, nnz = a.n, a.nnz
     print(f"[kernels] n={n} nnz={nnz}", file=sys.stderr, flush=True)
     res = {}
-    for op, name in ((6, "ic0_apply_fp32_factor"), (5, "ic0_apply_fp64"), (0, "spmv_fp64")):
+    for op, name in ((6, "ic0_apply_fp32_factor"), (5, "ic0_apply_fp64"), (0, "spmv_fp64"),
+                     (7, "spmv_fp64_factor_order")):
         ms = capi.time_kernel(ctx, op, a.h, pc.h if op else None, launches=launches)
         res[name] = round(ms * 1e3, 2)
         print(f"[kernels] {name}: {ms * 1e3:.2f} us", file=sys.stderr, flush=True)

@@ -663,7 +663,8 @@ class Pfem2(_Handle):
 
 def time_kernel(ctx, op, csr_handle, precond_handle=None, launches=50):
     """average ms per launch (HIP events on the context stream): 0 SpMV FP64, 1 SpMV FP32,
-    2 preconditioner apply FP64, 3 preconditioner apply FP32"""
+    2 preconditioner apply FP64, 3 preconditioner apply FP32, 4 apply with the FP32-stored factor,
+    5 / 6 ops 2 / 4 in the factor's ordering, 7 SpMV FP64 in the factor's ordering"""
     ms = C.c_double()
     ctx.check(lib().pfem_time_kernel(op, csr_handle, precond_handle, launches, C.byref(ms)))
     return ms.value

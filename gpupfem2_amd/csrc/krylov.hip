@@ -1091,9 +1091,9 @@ int pfem_solver_set_option(pfem_solver s, int option, int value)
 int pfem_time_kernel(int op, pfem_csr a, pfem_precond pc, int launches, double* avg_ms)
 {
     PFEM_TRY(a ? a->ctx : nullptr, {
-        PFEM_REQUIRE(a && avg_ms && launches > 0 && op >= 0 && op <= 6, "bad arguments");
+        PFEM_REQUIRE(a && avg_ms && launches > 0 && op >= 0 && op <= 7, "bad arguments");
         PFEM_REQUIRE(op < 5 || (pc && pc->ordering != PFEM_ORDER_NATURAL && pc->sweep),
-                     "ops 5 and 6 need a multicolour preconditioner with single-pass sweeps");
+                     "ops 5 to 7 need a multicolour preconditioner with single-pass sweeps");
         PFEM_REQUIRE(op < 2 || pc, "ops 2 and 3 need a preconditioner");
         Context* c = C(a->ctx);
         const Pattern& P = *a->pat;
@@ -1118,7 +1118,8 @@ int pfem_time_kernel(int op, pfem_csr a, pfem_precond pc, int launches, double* 
             case 3: precond_apply32(c, pc, y32.p, x32.p); break;
             case 4: precond_apply_t<double>(c, pc, y.p, x.p, true, ApplyOpts<double>{}); break;
             case 5: precond_apply_t<double>(c, pc, y.p, x.p, false, ApplyOpts<double>{nullptr, 0, true}); break;
-            default: precond_apply_t<double>(c, pc, y.p, x.p, true, ApplyOpts<double>{nullptr, 0, true}); break;
+            case 6: precond_apply_t<double>(c, pc, y.p, x.p, true, ApplyOpts<double>{nullptr, 0, true}); break;
+            default: launch_spmv(c, *pc->fpat, pc->aval.p, x.p, y.p); break;
             }
         };
         launch();   // warm-up

@@ -356,7 +356,8 @@ int pfem_pfem2_precond(pfem_pfem2 s, int which, pfem_precond* out);
  * src/common/gpu_timer.cuh:14-98). op 0: FP64 SpMV of a; op 1: FP32 SpMV of a (values converted
  * first); op 2: FP64 apply of pc (initialised from a first); op 3: the FP32 apply of pc; op 4: the apply
  * to FP64 vectors with the FP32 factor (PFEM_PRECISION_MIXED_PRECOND); ops 5 and 6: ops 2 and 4 on
- * vectors already in a multicolour factor's ordering (as the Krylov solvers run them). */
+ * vectors already in a multicolour factor's ordering (as the Krylov solvers run them); op 7: FP64 SpMV
+ * of the matrix in that ordering (P a P^T, as the Krylov solvers run it). */
 int pfem_time_kernel(int op, pfem_csr a, pfem_precond pc, int launches, double* avg_ms);
 
 /* ------------------------------------------------------------------ export (src/data_export.cuh) */
