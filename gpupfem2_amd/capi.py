@@ -104,6 +104,7 @@ SIGNATURES = {
     "pfem_precond_apply": (_i, [_vp, _vp, _vp]),
     "pfem_precond_download": (_i, [_vp, _vp, C.POINTER(_i), _vp]),
     "pfem_precond_set_sweep": (_i, [_vp, _i]),
+    "pfem_precond_set_spin_limit": (_i, [_vp, _i]),
     "pfem_solver_set_option": (_i, [_vp, _i, _i]),
     "pfem_precond_create_custom": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
     "pfem_mesh2d_device_ptr": (_i, [_vp, _i, _vp]),
@@ -437,6 +438,10 @@ class Preconditioner(_Handle):
     def set_sweep(self, single_pass):
         """IC0/ILU0 triangular solves as one launch each (True) or one launch per level (False)."""
         self.ctx.check(lib().pfem_precond_set_sweep(self.h, 1 if single_pass else 0))
+
+    def set_spin_limit(self, polls):
+        """Polls of one single-pass wait before it is declared failed (PFEM_ENUMERIC)."""
+        self.ctx.check(lib().pfem_precond_set_spin_limit(self.h, int(polls)))
 
     def download(self):
         nl = C.c_int()

@@ -138,7 +138,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv_alpha(const int* __restrict__ tiles, const int* __restrict__ rowptr,
                                                           const int* __restrict__ col, const T* __restrict__ val,
                                                           const T* __restrict__ p, T* __restrict__ q, CgState* st,
-                                                          double* part, int k, int* err)
+                                                          double* part, int k, int* err, int limit)
 {
     __shared__ T prod[kTileNnz];
     if (st->done) return;
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_alpha(const int* __restrict_
     block_sum<1>(v);
     publish_partial(v[0], part);
     if (blockIdx.x == gridDim.x - 1) {
-        const double pq = collect_partials(part, gridDim.x, err, kSpinLimit);
+        const double pq = collect_partials(part, gridDim.x, err, limit);
         if (threadIdx.x == 0) st->alpha[k & 1] = st->gam[k & 1] / pq;
     }
 }
@@ -620,7 +620,8 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
             int nbk = nb;
             if (fused2) {
                 hipLaunchKernelGGL((k_cg_spmv_alpha<T>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p,
-                                   P.rowptr.p, P.col.p, A.val, A.p, A.q, st, s->pa, k, A.pc->sweep_err.p);
+                                   P.rowptr.p, P.col.p, A.val, A.p, A.q, st, s->pa, k, A.pc->sweep_err.p,
+                                   A.pc->spin_limit);
                 ApplyOpts<T> o{s->pb, s->pb_cap, true};
                 o.done = &st->done;
                 o.x = A.x;

@@ -725,7 +725,7 @@ static void sweeps(Context* c, pfem_precond_t* pc, const TV* src, TV* out, const
     a.sy = sy;
     a.sw = sw;
     a.err = pc->sweep_err.p;
-    a.limit = kSpinLimit;
+    a.limit = pc->spin_limit;
     a.done = o.done;
     a.src = src;
     a.x = o.x;
@@ -939,6 +939,14 @@ int pfem_precond_set_sweep(pfem_precond pc, int single_pass)
     PFEM_TRY(pc ? pc->ctx : nullptr, {
         PFEM_REQUIRE(pc, "null argument");
         pc->sweep = single_pass ? 1 : 0;
+    });
+}
+
+int pfem_precond_set_spin_limit(pfem_precond pc, int polls)
+{
+    PFEM_TRY(pc ? pc->ctx : nullptr, {
+        PFEM_REQUIRE(pc && polls >= 0, "bad arguments");
+        pc->spin_limit = polls;
     });
 }
 

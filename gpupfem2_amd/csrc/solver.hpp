@@ -147,6 +147,7 @@ struct pfem_precond_t {
     // ({offset into lrows/urows, count} pairs); sy / sw hold the forward / backward results in the
     // factor's ordering and read as a sentinel NaN until a row has published its value
     int sweep = 1;                        // 1: one launch per triangular solve, 0: one launch per level
+    int spin_limit = 1 << 16;             // polls of one single-pass wait before it is declared failed
     pfem::DevArray<int> lchunks, uchunks;
     int nlchunks = 0, nuchunks = 0;
     bool contiguous = false;              // levels are row ranges (multicolour): chunks hold row ranges

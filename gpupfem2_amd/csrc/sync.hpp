@@ -28,7 +28,6 @@ struct Sentinel<float> {
     static constexpr U nan = 0x7FC00000u;
 };
 
-constexpr int kSpinLimit = 1 << 16;   // polls of one wait (~0.1 s) before it is declared failed
 
 template <typename T>
 __device__ __forceinline__ T await_value(T* p, int* err, int limit)

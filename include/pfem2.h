@@ -166,6 +166,10 @@ int pfem_precond_download(pfem_precond pc, double* factor, int* n_levels, int* p
  * bit-identical results. Replaces the analysis + cusparseSpSV_solve pair of applyPreconditioner
  * (preconditioners.cu:116-133, :195-212). */
 int pfem_precond_set_sweep(pfem_precond pc, int single_pass);
+/* Polls a single-pass wait makes before it is declared failed (no counterpart in the reference; default
+ * 65536, about 0.1 s). A failed wait does not hang or fault: the launch completes, and the solve or apply
+ * reports PFEM_ENUMERIC and re-arms the sweep buffers. 0 fails every wait that is not satisfied at once. */
+int pfem_precond_set_spin_limit(pfem_precond pc, int polls);
 
 /* ------------------------------------------------------------------ Krylov solvers (linear_solver.cuh) */
 #define PFEM_SOLVER_CG 0

@@ -388,6 +388,40 @@ def test_factor_order_solve(ctx, oracle, kind, precond, prec):
                 assert st.iterations == out[1][0] and np.array_equal(x.download(), out[1][1]), (fuse, rep)
 
 
+def test_sweep_wait_failure_is_reported(ctx, oracle):
+    """A single-pass wait that exceeds its spin limit, forced from the host with a limit of 0 polls, ends
+    its launch normally (no hang), and the solve reports PFEM_ENUMERIC; with the default limit restored,
+    the same solver and preconditioner solve again and reproduce the undisturbed result bit for bit."""
+    from gpupfem2_amd import capi
+    xy, m, a, rhs, rp, col, val = bc_system(ctx, oracle, n=160)
+    n = xy.shape[0]
+
+    def solver():
+        pc = capi.Preconditioner(a, capi.PRECOND_IC0, capi.ORDER_MULTICOLOR)
+        return pc, capi.Solver(a, capi.SOLVER_CG, 1e-10, 4000, 0, capi.TOL_RELATIVE, pc)
+
+    pc0, s0 = solver()
+    x0 = ctx.zeros(n)
+    st0 = s0.solve(a, x0, rhs)
+    assert st0.converged
+    pc, s = solver()
+    pc.set_spin_limit(0)
+    x = ctx.zeros(n)
+    try:
+        st = s.solve(a, x, rhs)
+        # no wait had to poll: then nothing failed and the result is the undisturbed one
+        assert st.converged and np.array_equal(x.download(), x0.download())
+        print("spin limit 0: no wait had to poll", flush=True)
+    except capi.PfemError as e:
+        assert e.code == -4 and "spin limit" in str(e)
+        print("spin limit 0: failure reported", flush=True)
+    pc.set_spin_limit(1 << 16)
+    x = ctx.zeros(n)
+    st = s.solve(a, x, rhs)
+    assert st.converged and st.iterations == st0.iterations
+    assert np.array_equal(x.download(), x0.download())
+
+
 def test_solver_failure_paths(ctx, oracle):
     """maxIterations reached -> converged = 0 (the reference returns false); GMRES without restart leaves
     x untouched."""
