@@ -161,6 +161,13 @@ struct pfem_precond_t {
     pfem::DevArray<double> split;
     pfem::DevArray<float> split32;
     long split_diag = 0;                  // offset of the diagonal values in split
+    // colour-range chunks (contiguous levels): the split factor sliced by chunk, column-major within a
+    // chunk: entry w of the chunk's row t at base + w * rows + t; a row's unused slots hold column -1.
+    // Headers {first row, rows, base, width} per chunk; sell_pos: factor slot of each entry (-1: none).
+    pfem::DevArray<int> sell_lhdr, sell_uhdr, sell_lcol, sell_ucol, sell_lpos, sell_upos;
+    pfem::DevArray<double> sell_l, sell_u;
+    pfem::DevArray<float> sell_l32, sell_u32;
+    bool sell = false;
 };
 
 struct pfem_solver_t {
