@@ -157,6 +157,52 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_alpha(const int* __restrict_
     }
 }
 
+// As k_cg_spmv_alpha over the tile-sliced matrix (tile header {first row, rows, base, width}; entry w of
+// the tile's row t at base + w * rows + t, unused slots with column -1): a lane finds its row's entries
+// without loading row offsets, the loads of one entry index are unit-stride over the tile, and the row's
+// products are added in CSR order, so q is bit-identical to the CSR tile kernel's.
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_alpha_sliced(const int* __restrict__ hdr, const int* __restrict__ col,
+                                                                 const double* __restrict__ val,
+                                                                 const double* __restrict__ p, double* __restrict__ q,
+                                                                 CgState* st, double* part, int k, int* err, int limit)
+{
+    if (st->done) return;
+    const int* h = hdr + 4 * blockIdx.x;
+    const int cnt = h[1], width = h[3];
+    double v[1] = {0.0};
+    if ((int)threadIdx.x < cnt) {
+        const int row = h[0] + (int)threadIdx.x, e0 = h[2] + (int)threadIdx.x;
+        double s = 0.0;
+        constexpr int kU = 8;
+        for (int w0 = 0; w0 < width; w0 += kU) {
+            int cv[kU];
+            double av[kU], xv[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                cv[u] = -1;
+                if (w0 + u < width) {
+                    cv[u] = col[e0 + (w0 + u) * cnt];
+                    av[u] = val[e0 + (w0 + u) * cnt];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+                if (cv[u] >= 0) xv[u] = p[cv[u]];
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+                if (cv[u] >= 0) s += av[u] * xv[u];
+        }
+        q[row] = s;
+        v[0] = p[row] * s;
+    }
+    block_sum<1>(v);
+    publish_partial(v[0], part);
+    if (blockIdx.x == gridDim.x - 1) {
+        const double pq = collect_partials(part, gridDim.x, err, limit);
+        if (threadIdx.x == 0) st->alpha[k & 1] = st->gam[k & 1] / pq;
+    }
+}
+
 // p = z + beta p with beta_{k+1} = gamma_{k+1} / gamma_k already recorded (by the backward sweep's last
 // workgroup): k_cg_p's update without its partial sum.
 template <typename T>
@@ -619,9 +665,19 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
             const int k = launched + j;
             int nbk = nb;
             if (fused2) {
-                hipLaunchKernelGGL((k_cg_spmv_alpha<T>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p,
-                                   P.rowptr.p, P.col.p, A.val, A.p, A.q, st, s->pa, k, A.pc->sweep_err.p,
-                                   A.pc->spin_limit);
+                bool sliced = false;
+                if constexpr (sizeof(T) == 8) {
+                    sliced = A.pc->asl_val.p && A.val == A.pc->aval.p;
+                    if (sliced)
+                        hipLaunchKernelGGL(k_cg_spmv_alpha_sliced, dim3(P.ntiles), dim3(kBlock), 0, c->stream,
+                                           A.pc->asl_hdr.p, A.pc->asl_col.p, (const double*)A.pc->asl_val.p,
+                                           (const double*)A.p, A.q, st, s->pa, k, A.pc->sweep_err.p,
+                                           A.pc->spin_limit);
+                }
+                if (!sliced)
+                    hipLaunchKernelGGL((k_cg_spmv_alpha<T>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p,
+                                       P.rowptr.p, P.col.p, A.val, A.p, A.q, st, s->pa, k, A.pc->sweep_err.p,
+                                       A.pc->spin_limit);
                 ApplyOpts<T> o{s->pb, s->pb_cap, true};
                 o.done = &st->done;
                 o.x = A.x;

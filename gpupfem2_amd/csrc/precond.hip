@@ -730,6 +730,32 @@ static void build_factor_structure(Context* c, pfem_precond_t* pc)
         slice(lch, true, pc->sell_lhdr, pc->sell_lcol, pc->sell_lpos, pc->sell_l);
         slice(uch, false, pc->sell_uhdr, pc->sell_ucol, pc->sell_upos, pc->sell_u);
         pc->sell = true;
+        // the whole matrix over the SpMV tiles (rows [first, end) per tile, in the tile table's order)
+        std::vector<int> ftiles(2 * (size_t)F.ntiles);
+        PFEM_HIP(hipMemcpy(ftiles.data(), F.tiles.p, ftiles.size() * sizeof(int), hipMemcpyDeviceToHost));
+        std::vector<int> hdr, col, pos;
+        for (int t = 0; t < F.ntiles; ++t) {
+            const int r0 = ftiles[2 * t], cnt = ftiles[2 * t + 1] - r0;
+            int width = 0;
+            for (int i = r0; i < r0 + cnt; ++i) width = std::max(width, F.h_rowptr[i + 1] - F.h_rowptr[i]);
+            const int base = (int)col.size();
+            col.resize(col.size() + (size_t)width * cnt, -1);
+            pos.resize(col.size(), -1);
+            for (int q = 0; q < cnt; ++q)
+                for (int k = F.h_rowptr[r0 + q]; k < F.h_rowptr[r0 + q + 1]; ++k) {
+                    col[base + (size_t)(k - F.h_rowptr[r0 + q]) * cnt + q] = F.h_col[k];
+                    pos[base + (size_t)(k - F.h_rowptr[r0 + q]) * cnt + q] = k;
+                }
+            hdr.insert(hdr.end(), {r0, cnt, base, width});
+        }
+        if (col.empty()) {
+            col.push_back(-1);
+            pos.push_back(-1);
+        }
+        upload(c, pc->asl_hdr, hdr);
+        upload(c, pc->asl_col, col);
+        upload(c, pc->asl_pos, pos);
+        pc->asl_val.alloc(col.size());
     }
     pc->fac.alloc(F.nnz);
     pc->tmp.alloc(n);
@@ -766,6 +792,11 @@ void precond_initialize(Context* c, pfem_precond_t* pc, pfem_csr_t* a)
         if (!pc->aval.p) pc->aval.alloc(F.nnz);
         hipLaunchKernelGGL((k_copyT<double>), dim3(blocks_for(F.nnz)), dim3(kBlock), 0, c->stream, (long)F.nnz,
                            pc->aval.p, (const double*)pc->fac.p);
+        if (pc->asl_val.p) {
+            const long m = (long)pc->asl_val.n;
+            hipLaunchKernelGGL(k_gather_or_zero, dim3(blocks_for(m)), dim3(kBlock), 0, c->stream, m, pc->asl_pos.p,
+                               (const double*)pc->aval.p, pc->asl_val.p);
+        }
     }
     for (size_t l = 0; l + 1 < pc->h_lptr.size(); ++l) {
         const int cnt = pc->h_lptr[l + 1] - pc->h_lptr[l];

@@ -168,6 +168,10 @@ struct pfem_precond_t {
     pfem::DevArray<double> sell_l, sell_u;
     pfem::DevArray<float> sell_l32, sell_u32;
     bool sell = false;
+    // A in the factor's ordering (aval) sliced the same way over the SpMV tiles of fpat, for the fused
+    // PCG's SpMV: header {first row, rows, base, width} per tile, entries column-major per tile
+    pfem::DevArray<int> asl_hdr, asl_col, asl_pos;
+    pfem::DevArray<double> asl_val;
 };
 
 struct pfem_solver_t {
