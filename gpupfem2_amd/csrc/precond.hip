@@ -901,9 +901,7 @@ static void sweeps(Context* c, pfem_precond_t* pc, const TV* src, TV* out, const
     up.chunks = pc->uchunks.p;
     lo.ptr = pc->split_ptr.p;
     up.ptr = pc->split_ptr.p + (F.n + 1);
-    // the sliced triangles for the FP32-stored factor (36.0 -> 25.8 us per apply on the cylinder's pressure
-    // matrix); with FP64 values the row-offset form measured as fast or faster (37.4 against 39.2 us)
-    if (pc->contiguous && pc->sell && sizeof(TS) == 4) {
+    if (pc->contiguous && pc->sell) {
         lo.chunks = pc->sell_lhdr.p;
         up.chunks = pc->sell_uhdr.p;
         lo.col = pc->sell_lcol.p;
