@@ -210,7 +210,8 @@ __global__ void k_cg_p_recorded(int n, T* __restrict__ p, const T* __restrict__ 
 {
     if (st->done) return;
     const T beta = (T)(st->gam[(k + 1) & 1] / st->gam[k & 1]);
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = z[i] + beta * p[i];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;   // one element per lane: a single streaming pass
+    if (i < n) p[i] = z[i] + beta * p[i];
 }
 
 // alpha_k = gamma_k / (p, q) from the SpMV partials
@@ -688,7 +689,7 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
                 o.k = k;
                 PFEM_REQUIRE(precond_apply_t<T>(c, A.pc, A.z, A.r, A.pc_fp32, o) > 0,
                              "fused PCG iteration without its partials");
-                hipLaunchKernelGGL((k_cg_p_recorded<T>), dim3(g), dim3(kBlock), 0, c->stream, n, A.p, A.z, st, k);
+                hipLaunchKernelGGL((k_cg_p_recorded<T>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, A.p, A.z, st, k);
                 continue;
             }
             hipLaunchKernelGGL((k_cg_spmv<T>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.p, A.q, st, s->pa);

@@ -315,7 +315,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_upper(SweepArgs<TV, TS, TF> a)
         publish(a.sw + i, s);
         const int o = a.perm ? a.perm[i] : i;
         const TV z = (TV)s;
-        a.out[o] = z;
+        if ((const void*)a.out != (const void*)a.sw) a.out[o] = z;
         if (a.part) v = (double)(a.rdot[o] * z);
     }
     if (!a.part) return;
@@ -878,6 +878,10 @@ static void sweeps(Context* c, pfem_precond_t* pc, const TV* src, TV* out, const
     a.perm = pc->ordering == PFEM_ORDER_NATURAL || o.factor_order ? nullptr : pc->perm.p;
     a.sy = sy;
     a.sw = sw;
+    // in the factor's ordering with z and r distinct and z of the arithmetic's type, the backward sweep
+    // publishes straight into z (the forward sweep re-arms z instead of sw): one 8-byte pass less
+    if (o.factor_order && pc->contiguous && sizeof(TV) == sizeof(TF) && (const void*)out != (const void*)src)
+        a.sw = reinterpret_cast<TF*>(out);
     a.err = pc->sweep_err.p;
     a.limit = pc->spin_limit;
     a.done = o.done;
