@@ -203,15 +203,24 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_alpha_sliced(const int* __re
     }
 }
 
-// p = z + beta p with beta_{k+1} = gamma_{k+1} / gamma_k already recorded (by the backward sweep's last
-// workgroup): k_cg_p's update without its partial sum.
+// End of fused PCG iteration k: x += alpha_k p_k (the x half of updateXR, linear_solver.cu:68-77, moved
+// here from the forward sweep so that p_k is read once for both updates), then, unless iteration k
+// stopped the solve, p = z + beta p with beta_{k+1} = gamma_{k+1} / gamma_k already recorded by the
+// backward sweep's last workgroup (updateP, linear_solver.cu:79-87). One element per lane.
 template <typename T>
-__global__ void k_cg_p_recorded(int n, T* __restrict__ p, const T* __restrict__ z, const CgState* st, int k)
+__global__ void k_cg_xp_recorded(int n, T* __restrict__ x, T* __restrict__ p, const T* __restrict__ z,
+                                 const CgState* st, int k)
 {
-    if (st->done) return;
-    const T beta = (T)(st->gam[(k + 1) & 1] / st->gam[k & 1]);
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;   // one element per lane: a single streaming pass
-    if (i < n) p[i] = z[i] + beta * p[i];
+    if (st->done && st->it != k + 1) return;   // stopped before this iteration
+    const bool last = st->done;
+    const T al = (T)st->alpha[k & 1];
+    const T beta = last ? (T)0 : (T)(st->gam[(k + 1) & 1] / st->gam[k & 1]);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const T pi = p[i];
+        x[i] += al * pi;
+        if (!last) p[i] = z[i] + beta * pi;
+    }
 }
 
 // alpha_k = gamma_k / (p, q) from the SpMV partials
@@ -681,15 +690,14 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
                                        A.pc->spin_limit);
                 ApplyOpts<T> o{s->pb, s->pb_cap, true};
                 o.done = &st->done;
-                o.x = A.x;
-                o.p = A.p;
                 o.q = A.q;
                 o.alpha = &st->alpha[k & 1];
                 o.cg = st;
                 o.k = k;
                 PFEM_REQUIRE(precond_apply_t<T>(c, A.pc, A.z, A.r, A.pc_fp32, o) > 0,
                              "fused PCG iteration without its partials");
-                hipLaunchKernelGGL((k_cg_p_recorded<T>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, A.p, A.z, st, k);
+                hipLaunchKernelGGL((k_cg_xp_recorded<T>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, A.x, A.p,
+                                   (const T*)A.z, st, k);
                 continue;
             }
             hipLaunchKernelGGL((k_cg_spmv<T>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.p, A.q, st, s->pa);

@@ -278,9 +278,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep_lower(SweepArgs<TV, TS, TF> a)
     if (!sweep_row<LAYOUT>(a, i, e0, stride, width)) return;
     TV b;
     if constexpr (XR) {
-        // updateXR (linear_solver.cu:68-77) for this row, as the separate vector kernel computes it
+        // updateXR (linear_solver.cu:68-77) for this row, as the separate vector kernel computes it (the
+        // x half is left to the p-update kernel when a.x is null)
         const TV al = (TV)*a.alpha;
-        a.x[i] += al * a.p[i];
+        if (a.x) a.x[i] += al * a.p[i];
         b = a.r[i] - al * a.q[i];
         a.r[i] = b;
     } else {
@@ -896,8 +897,8 @@ static void sweeps(Context* c, pfem_precond_t* pc, const TV* src, TV* out, const
     a.part = part;
     a.cg = part ? o.cg : nullptr;
     a.k = o.k;
-    const bool xr = o.x != nullptr;
-    PFEM_REQUIRE(!xr || (o.factor_order && pc->contiguous && o.p && o.q && o.alpha),
+    const bool xr = o.alpha != nullptr;
+    PFEM_REQUIRE(!xr || (o.factor_order && pc->contiguous && o.q && (o.p || !o.x)),
                  "the fused x/r update needs a multicolour factor and vectors in its ordering");
     const dim3 gl(pc->nlchunks), gu(pc->nuchunks), bl(kBlock);
     SweepArgs<TV, TS, TF> lo = a, up = a;

@@ -223,8 +223,8 @@ struct ApplyOpts {
     double* part = nullptr;
     int cap = 0;
     bool factor_order = false;
-    // PCG fusion (single-pass sweeps in factor order): skip when *done; x += alpha p and r -= alpha q
-    // (r = the apply's input) before the forward sweep uses r
+    // PCG fusion (single-pass sweeps in factor order): skip when *done; with alpha, r -= alpha q (r = the
+    // apply's input) before the forward sweep uses r, and x += alpha p as well when x is given
     const int* done = nullptr;
     TV* x = nullptr;
     const TV* p = nullptr;
