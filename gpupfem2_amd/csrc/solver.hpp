@@ -8,6 +8,38 @@
 
 namespace pfem {
 
+// A matrix or triangle sliced by row chunks, column-major within a chunk: entry w of the chunk's row t
+// in slot base + w * rows + t; a row's unused slots follow its entries and hold column -1. hdr: {first
+// row, rows, base, width} per chunk; col: 32-bit column of each slot; col16: its 16-bit code (colour << 14
+// | offset from cbase[4 * chunk + colour], 0xFFFF unused), valid where cbase[4 * chunk] >= 0; pos: the
+// CSR slot each entry's value comes from (-1 unused).
+struct SlicedCols {
+    DevArray<int> hdr, cbase, col, pos;
+    DevArray<unsigned short> col16;
+};
+
+// A chunk's four colour bases, loaded once (uniform values) so that decoding a column is register work
+struct ChunkBases {
+    int b0, b1, b2, b3;
+    bool wide;
+};
+
+__device__ __forceinline__ ChunkBases chunk_bases(const int* __restrict__ cb)
+{
+    return ChunkBases{cb[0], cb[1], cb[2], cb[3], cb[0] < 0};
+}
+
+// Column of sliced slot e (-1 for an unused slot)
+__device__ __forceinline__ int slot_col(int e, const int* __restrict__ col32, const unsigned short* __restrict__ col16,
+                                        const ChunkBases& cb)
+{
+    if (cb.wide) return col32[e];
+    const unsigned v = col16[e];
+    const unsigned c = v >> 14;
+    const int base = c == 0 ? cb.b0 : c == 1 ? cb.b1 : c == 2 ? cb.b2 : cb.b3;
+    return v == 0xFFFFu ? -1 : base + (int)(v & 0x3FFFu);
+}
+
 struct MultiWork;   // multi-RHS PCG work space (multi.hip)
 struct MultiWorkDeleter {
     void operator()(MultiWork* w) const;
@@ -161,17 +193,13 @@ struct pfem_precond_t {
     pfem::DevArray<double> split;
     pfem::DevArray<float> split32;
     long split_diag = 0;                  // offset of the diagonal values in split
-    // colour-range chunks (contiguous levels): the split factor sliced by chunk, column-major within a
-    // chunk: entry w of the chunk's row t at base + w * rows + t; a row's unused slots hold column -1.
-    // Headers {first row, rows, base, width} per chunk; sell_pos: factor slot of each entry (-1: none).
-    pfem::DevArray<int> sell_lhdr, sell_uhdr, sell_lcol, sell_ucol, sell_lpos, sell_upos;
-    pfem::DevArray<double> sell_l, sell_u;
+    // colour-range chunks (contiguous levels): the split factor's triangles sliced by chunk (sl_lower,
+    // sl_upper), and A in the factor's ordering sliced over the SpMV tiles of fpat (sl_all, values in
+    // asl_val, for the fused PCG's SpMV)
+    pfem::SlicedCols sl_lower, sl_upper, sl_all;
+    pfem::DevArray<double> sell_l, sell_u, asl_val;
     pfem::DevArray<float> sell_l32, sell_u32;
     bool sell = false;
-    // A in the factor's ordering (aval) sliced the same way over the SpMV tiles of fpat, for the fused
-    // PCG's SpMV: header {first row, rows, base, width} per tile, entries column-major per tile
-    pfem::DevArray<int> asl_hdr, asl_col, asl_pos;
-    pfem::DevArray<double> asl_val;
 };
 
 struct pfem_solver_t {
