@@ -768,10 +768,41 @@ static void build_factor_structure(Context* c, pfem_precond_t* pc)
         slice(lch, 0, pc->sl_lower, pc->sell_l);
         slice(uch, 1, pc->sl_upper, pc->sell_u);
         pc->sell = true;
-        // the whole matrix over the SpMV tiles (rows [first, end) per tile, in the tile table's order)
+        // The whole matrix over the SpMV tiles, dealt by region across colours: XCD x (workgroups b = x mod
+        // 8) takes the tiles of the x-th eighth of every colour's range. A row's neighbours lie in the other
+        // colours at about the same relative position, so they are gathered from the L2 of the XCD that
+        // also computes them, where a band of the factor order would span one colour only.
         std::vector<int> ftiles(2 * (size_t)F.ntiles);
         PFEM_HIP(hipMemcpy(ftiles.data(), F.tiles.p, ftiles.size() * sizeof(int), hipMemcpyDeviceToHost));
-        for (int t = 0; t < F.ntiles; ++t) ftiles[2 * t + 1] -= ftiles[2 * t];
+        {
+            constexpr int kXcds = 8;
+            std::vector<std::pair<int, int>> t2;
+            for (int t = 0; t < F.ntiles; ++t) t2.emplace_back(ftiles[2 * t], ftiles[2 * t + 1]);
+            std::sort(t2.begin(), t2.end());
+            std::vector<std::vector<std::pair<int, int>>> region(kXcds);
+            for (const auto& tr : t2) {
+                const int cc = (int)(std::upper_bound(cptr.begin(), cptr.end(), tr.first) - cptr.begin()) - 1;
+                const long size = std::max(1, cptr[cc + 1] - cptr[cc]);
+                const int x = (int)std::min<long>(kXcds - 1, (long)(tr.first - cptr[cc]) * kXcds / size);
+                region[x].push_back(tr);
+            }
+            std::vector<size_t> next(kXcds, 0);
+            size_t spill_x = 0;
+            for (int b = 0; b < F.ntiles; ++b) {
+                int x = b % kXcds;
+                if (next[x] == region[x].size()) {   // this region is used up: take from the next non-empty one
+                    while (next[spill_x] == region[spill_x].size()) ++spill_x;
+                    x = (int)spill_x;
+                }
+                const auto tr = region[x][next[x]++];
+                ftiles[2 * b] = tr.first;
+                ftiles[2 * b + 1] = tr.second;
+            }
+            // the CSR tile kernels of the factor's pattern use the same order (the same partials, so every
+            // form of the fused PCG gives the same bits)
+            upload(c, pc->fpat->tiles, ftiles);
+            for (int b = 0; b < F.ntiles; ++b) ftiles[2 * b + 1] -= ftiles[2 * b];
+        }
         slice(ftiles, 2, pc->sl_all, pc->asl_val);
     }
     pc->fac.alloc(F.nnz);
