@@ -219,8 +219,23 @@ __global__ void k_cg_xp_recorded(int n, T* __restrict__ x, T* __restrict__ p, co
     const bool last = st->done;
     const T al = (T)st->alpha[k & 1];
     const T beta = last ? (T)0 : (T)(st->gam[(k + 1) & 1] / st->gam[k & 1]);
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
+    // two elements per lane (16-byte loads and stores where n allows)
+    const int i = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
+    if (i + 1 < n) {
+        using V2 = HIP_vector_type<T, 2>;
+        const V2 pv = *reinterpret_cast<const V2*>(p + i);
+        V2 xv = *reinterpret_cast<const V2*>(x + i);
+        xv.x += al * pv.x;
+        xv.y += al * pv.y;
+        *reinterpret_cast<V2*>(x + i) = xv;
+        if (!last) {
+            const V2 zv = *reinterpret_cast<const V2*>(z + i);
+            V2 nv;
+            nv.x = zv.x + beta * pv.x;
+            nv.y = zv.y + beta * pv.y;
+            *reinterpret_cast<V2*>(p + i) = nv;
+        }
+    } else if (i < n) {
         const T pi = p[i];
         x[i] += al * pi;
         if (!last) p[i] = z[i] + beta * pi;
@@ -702,7 +717,7 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
                 o.k = k;
                 PFEM_REQUIRE(precond_apply_t<T>(c, A.pc, A.z, A.r, A.pc_fp32, o) > 0,
                              "fused PCG iteration without its partials");
-                hipLaunchKernelGGL((k_cg_xp_recorded<T>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, A.x, A.p,
+                hipLaunchKernelGGL((k_cg_xp_recorded<T>), dim3(blocks_for((n + 1) / 2)), dim3(kBlock), 0, c->stream, n, A.x, A.p,
                                    (const T*)A.z, st, k);
                 continue;
             }
