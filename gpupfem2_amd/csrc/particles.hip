@@ -207,12 +207,29 @@ __global__ __launch_bounds__(kBlock) void k_advect_relocate(
 __global__ void k_census(long P, const int* __restrict__ cell, const double2* __restrict__ lam12,
                          const double* __restrict__ lam3, int L, double step, uint32_t* __restrict__ mask)
 {
+    // Particles arrive grouped by cell (the array was sorted by cell at the end of the previous step and
+    // advection moves few of them), so the lanes of a wave mostly share two or three cells. The subcell
+    // bits are OR-ed across lanes with the same cell first (a suffix scan keyed on the cell), and only the
+    // first lane of each run of equal cells issues the atomic: one atomic per run instead of per particle.
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    const int c = cell[p];
-    if (c < 0) return;
-    const double2 l12 = lam12[p];
-    atomicOr(&mask[c], 1u << subcell(L, step, l12.x, l12.y, lam3[p]));
+    const int lane = threadIdx.x & 63;
+    int c = -1;
+    uint32_t bits = 0u;
+    if (p < P) {
+        c = cell[p];
+        if (c >= 0) {
+            const double2 l12 = lam12[p];
+            bits = 1u << subcell(L, step, l12.x, l12.y, lam3[p]);
+        }
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t ob = __shfl_down(bits, off, 64);
+        const int oc = __shfl_down(c, off, 64);
+        if (lane + off < 64 && oc == c) bits |= ob;
+    }
+    const int prev = __shfl_up(c, 1, 64);
+    if (c >= 0 && (lane == 0 || prev != c)) atomicOr(&mask[c], bits);
 }
 
 // kCountParticlesToBeAdded (:183-195)
@@ -472,16 +489,30 @@ __global__ void k_proj_cells(int nt, const int* __restrict__ start, const double
     double s[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) s[k] = 0.0;
-    for (int p = start[c]; p < start[c + 1]; ++p) {
-        const double2 l12 = lam12[p];
-        const double l[3] = {l12.x, l12.y, lam3[p]};
-        const double2 v = vel[p];
+    // the cell's particles in array order; the loads of up to 8 particles are issued together
+    constexpr int kU = 8;
+    const int p1 = start[c + 1];
+    for (int p0 = start[c]; p0 < p1; p0 += kU) {
+        double2 l12[kU], v[kU];
+        double l3[kU];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            s[3 * i + 0] += l[i] * v.x;
-            s[3 * i + 1] += l[i] * v.y;
-            s[3 * i + 2] += l[i];
-        }
+        for (int u = 0; u < kU; ++u)
+            if (p0 + u < p1) {
+                l12[u] = lam12[p0 + u];
+                l3[u] = lam3[p0 + u];
+                v[u] = vel[p0 + u];
+            }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if (p0 + u < p1) {
+                const double l[3] = {l12[u].x, l12[u].y, l3[u]};
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    s[3 * i + 0] += l[i] * v[u].x;
+                    s[3 * i + 1] += l[i] * v[u].y;
+                    s[3 * i + 2] += l[i];
+                }
+            }
     }
 #pragma unroll
     for (int k = 0; k < 9; ++k) part[9 * (size_t)c + k] = s[k];
