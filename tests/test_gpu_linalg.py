@@ -264,6 +264,31 @@ def test_precond_sweep_forms(ctx, oracle, kind, ordering):
     assert np.array_equal(dr.download(), z0.download())
 
 
+@pytest.mark.parametrize("kind", [2, 3])
+def test_precond_sweep_forms_full_size(ctx, kind):
+    """The sliced sweep layouts at the 2M-triangle cylinder's size (1.01M rows, 4 colours), where some
+    chunks' columns of one colour span more than a 16-bit code holds (32-bit fallback) and others are
+    coded: single-pass sweeps against per-level launches on the combined factor, bit for bit."""
+    from gpupfem2_amd import capi, cases
+    print("sweep forms full size", kind, flush=True)
+    case = cases.cylinder_case()
+    m = capi.Mesh2D(ctx, case.xy, case.tri, fill_neighbors=False)
+    a = capi.CSR(ctx, mesh=m)
+    rhs = ctx.zeros(case.xy.shape[0])
+    capi.assemble_poisson2d(m, a, rhs, coeff=1.0, rhs_kind=2, rhs_param=1.0)
+    capi.Dirichlet(ctx, case.xy.shape[0], case.bcp_node, case.bcp_val).apply(a, rhs, 1)
+    n = case.xy.shape[0]
+    pc = capi.Preconditioner(a, kind, capi.ORDER_MULTICOLOR)
+    pc.initialize(a)
+    r = np.random.default_rng(7).standard_normal(n)
+    dr, z1, z0 = ctx.to_device(r), ctx.empty(n), ctx.empty(n)
+    pc.set_sweep(True)
+    pc.apply(z1, dr)
+    pc.set_sweep(False)
+    pc.apply(z0, dr)
+    assert np.array_equal(z1.download(), z0.download())
+
+
 @pytest.mark.parametrize("precond,ordering", [(0, 0), (1, 0), (2, 0), (3, 0), (2, 1)])
 @pytest.mark.parametrize("tol_mode", [0, 1])
 def test_pcg(ctx, oracle, precond, ordering, tol_mode):
