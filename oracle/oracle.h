@@ -10,8 +10,9 @@
  * undefined (device atomics), the oracle fixes one order and states it; the GPU library uses
  * the same order so the two agree bit-for-bit wherever the arithmetic is identical.
  *
- * Pinning (what checks each part, besides the GPU comparison) is listed per function in
- * DESIGN.md and in the tests that exercise it.
+ * Pinning (what checks each part besides the GPU comparison: closed forms, invariants, brute force,
+ * known-answer vectors and the reference's fixtures) is done per function in tests/test_oracle_*.py;
+ * every function here has such a check.
  */
 #ifndef PFEM2_ORACLE_H
 #define PFEM2_ORACLE_H
