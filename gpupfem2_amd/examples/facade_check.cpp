@@ -126,6 +126,25 @@ int main(int argc, char* argv[])
         report(ok, "gmres_restarted", gm.lastStats().iterations);
     }
 
+    {
+        // two right-hand sides at once (rhs and 2 rhs): each column equals the single solve's result
+        const auto hb = rhs.copyToHost();
+        std::vector<double> hb2(2 * (size_t)n);
+        for (int i = 0; i < n; ++i) {
+            hb2[i] = hb[i];
+            hb2[n + i] = 2.0 * hb[i];
+        }
+        deviceVector<double> B(hb2), X(std::vector<double>(2 * (size_t)n, 0.0));
+        PreconditionerJacobi jac(n, &LA);
+        SolverCG cg(1e-10, 5000, &LA, &jac);
+        cg.init(a);
+        const bool ok = cg.solveMulti(a, X, B, 2);
+        const auto hx = X.copyToHost();
+        double d = 0.0;
+        for (int i = 0; i < n; ++i) d = std::fmax(d, std::fmax(std::fabs(hx[i] - h0[i]), std::fabs(hx[n + i] - 2.0 * h0[i])));
+        report(ok && d < 1e-6 && cg.multiStats().size() == 2, "cg_multi_rhs", cg.multiStats()[0].iterations, d);
+    }
+
     ParticleHandler2D ph(&mesh, 2);
     ph.seedParticles();
     report(ph.getParticleCount() == 4 * mesh.getCells().size, "seed_centroids", ph.getParticleCount());

@@ -587,6 +587,38 @@ public:
         return stats_.converged != 0;
     }
 
+    /* Additions (no counterpart in the reference, which solves one system per call): nrhs <= 8 right-hand
+     * sides of A at once, column j at X + j*n and B + j*n (pfem_solver_solve_multi); and independent
+     * systems of a block-diagonal A, system k owning rows sysRows[k]..sysRows[k+1]
+     * (pfem_solver_solve_batched). PCG with Jacobi or no preconditioner. True when every column / system
+     * converged; per-column / per-system statistics in multiStats(). */
+    bool solveMulti(const SparseMatrixCSR& A, deviceVector<double>& X, const deviceVector<double>& B, int nrhs)
+    {
+        if (!h_) init(A);
+        if ((long)X.size < (long)nrhs * n || (long)B.size < (long)nrhs * n)
+            throw gpupfem2::Error(PFEM_EINVAL, "solveMulti: X and B need nrhs * n entries");
+        multi_.assign(nrhs, pfem_solve_stats{});
+        gpupfem2::check(pfem_solver_solve_multi(h_, A.handle(), nrhs, X.data, B.data, multi_.data()));
+        bool all = true;
+        for (const auto& st : multi_) all = all && st.converged;
+        return all;
+    }
+    bool solveBatched(const SparseMatrixCSR& A, const std::vector<int>& sysRows, deviceVector<double>& x,
+                      const deviceVector<double>& b, std::vector<double>* timeMs = nullptr)
+    {
+        if (!h_) init(A);
+        const int nsys = (int)sysRows.size() - 1;
+        if (nsys < 1) throw gpupfem2::Error(PFEM_EINVAL, "solveBatched: no systems");
+        multi_.assign(nsys, pfem_solve_stats{});
+        if (timeMs) timeMs->assign(nsys, 0.0);
+        gpupfem2::check(pfem_solver_solve_batched(h_, A.handle(), nsys, sysRows.data(), x.data, b.data, multi_.data(),
+                                                  timeMs ? timeMs->data() : nullptr));
+        bool all = true;
+        for (const auto& st : multi_) all = all && st.converged;
+        return all;
+    }
+    const std::vector<pfem_solve_stats>& multiStats() const { return multi_; }
+
     void setRelativeTolerance(bool relative) { tolMode_ = relative ? PFEM_TOL_RELATIVE : PFEM_TOL_ABSOLUTE; }
     void setPrecision(int precision, double innerTolerance = 1e-6)
     {
@@ -610,6 +642,7 @@ private:
     double innerTol_ = 1e-6;
     pfem_solver h_ = nullptr;
     pfem_solve_stats stats_{};
+    std::vector<pfem_solve_stats> multi_;
 };
 
 /* SolverCG (linear_solver.cu:173-376) */
