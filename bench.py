@@ -91,17 +91,19 @@ def roofline_ic0(ctx, st, precision):
     from gpupfem2_amd import capi
     a, pc = st.matrix_handle(2), st.precond_handle(1)
     n, nnz = csr_info(a)
-    # the apply as the pressure PCG runs it: vectors in the multicolour factor's ordering
-    op = 6 if precision == 1 else 5
+    # the apply as the pressure PCG runs it: vectors in the multicolour factor's ordering, y and z in
+    # one tagged buffer (fusion level 3)
+    fp32 = precision == 1
+    op = 8 if fp32 else 9
     ms = capi.time_kernel(ctx, op, a, pc, launches=200)
-    b = ic0_apply_bytes(n, nnz, 4 if op == 6 else 8)
+    b = ic0_apply_bytes(n, nnz, 4 if fp32 else 8)
     gbs = b / (ms * 1e-3) / 1e9
-    traffic = measured_traffic("traffic_bytes_ic0_apply_fp32_factor" if op == 6 else "traffic_bytes_ic0_apply_fp64")
+    traffic = measured_traffic("traffic_bytes_ic0_apply_fp32_factor" if fp32 else "traffic_bytes_ic0_apply_fp64")
     return dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(gbs / HBM_PEAK_GBS, 4), traffic=traffic,
                 traffic_source="profiles/ic0_apply_cylinder2M.md" if traffic else None,
-                kernel=(f"IC0 apply = k_sweep_lower + k_sweep_upper, pressure matrix n={n} nnz={nnz}, "
-                        f"factor {'FP32' if op == 6 else 'FP64'}, vectors FP64 in factor order"),
+                kernel=(f"IC0 apply = k_sweep_lower + k_sweep_upper (tagged), pressure matrix n={n} nnz={nnz}, "
+                        f"factor {'FP32' if fp32 else 'FP64'}, vectors FP64 in factor order"),
                 avg_us=round(ms * 1e3, 2), bytes_per_launch=b)
 
 
@@ -198,8 +200,8 @@ def kernels_only(h, launches):
     n, nnz = a.n, a.nnz
     print(f"[kernels] n={n} nnz={nnz}", file=sys.stderr, flush=True)
     res = {}
-    for op, name in ((6, "ic0_apply_fp32_factor"), (5, "ic0_apply_fp64"), (0, "spmv_fp64"),
-                     (7, "spmv_fp64_factor_order")):
+    for op, name in ((8, "ic0_apply_fp32_factor"), (9, "ic0_apply_fp64"), (6, "ic0_apply_fp32_factor_sentinel"),
+                     (5, "ic0_apply_fp64_sentinel"), (0, "spmv_fp64"), (7, "spmv_fp64_factor_order")):
         ms = capi.time_kernel(ctx, op, a.h, pc.h if op else None, launches=launches)
         res[name] = round(ms * 1e3, 2)
         print(f"[kernels] {name}: {ms * 1e3:.2f} us", file=sys.stderr, flush=True)

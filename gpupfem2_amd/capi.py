@@ -464,7 +464,7 @@ class Solver(_Handle):
 
     def set_option(self, option, value):
         """SOLVER_OPT_FACTOR_ORDER (0/1), SOLVER_OPT_POLL (iterations between convergence polls) or
-        SOLVER_OPT_FUSE (0/1/2)."""
+        SOLVER_OPT_FUSE (0/1/2/3, default 3)."""
         self.ctx.check(lib().pfem_solver_set_option(self.h, option, int(value)))
 
     def set_precision(self, precision, inner_tol=0.0, max_refinements=0):
@@ -669,7 +669,8 @@ class Pfem2(_Handle):
 def time_kernel(ctx, op, csr_handle, precond_handle=None, launches=50):
     """average ms per launch (HIP events on the context stream): 0 SpMV FP64, 1 SpMV FP32,
     2 preconditioner apply FP64, 3 preconditioner apply FP32, 4 apply with the FP32-stored factor,
-    5 / 6 ops 2 / 4 in the factor's ordering, 7 SpMV FP64 in the factor's ordering"""
+    5 / 6 ops 2 / 4 in the factor's ordering, 7 SpMV FP64 in the factor's ordering, 8 / 9 ops 6 / 5 with
+    the tagged sweeps (PCG fusion level 3)"""
     ms = C.c_double()
     ctx.check(lib().pfem_time_kernel(op, csr_handle, precond_handle, launches, C.byref(ms)))
     return ms.value

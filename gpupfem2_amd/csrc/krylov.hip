@@ -645,6 +645,10 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
     // four launches per iteration, the reductions finished inside the launches that produce them: SpMV
     // (+ alpha), forward sweep (+ x/r), backward sweep (+ beta and the stopping test), p update
     const bool fused2 = fused && s->fuse >= 2 && P.ntiles <= s->pa_cap;
+    // level 3: the sweeps share z's storage for y and z, told apart by tags instead of re-armed sentinels
+    // (two 8-byte passes less per iteration; y and z change by at most one unit in the last place)
+    bool tagged = false;
+    if constexpr (sizeof(T) == 8) tagged = fused2 && s->fuse >= 3 && A.pc->sell;
     CgState init{};
     init.tol = A.tol;
     init.tol_mode = A.tol_mode;
@@ -681,6 +685,10 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
                            (long)P.ntiles, s->pa);
         hipLaunchKernelGGL((k_fill_sentinel<double>), dim3(blocks_for(A.pc->nuchunks)), dim3(kBlock), 0, c->stream,
                            (long)A.pc->nuchunks, s->pb);
+        // z_0 has been copied into p: z's storage now becomes the sweeps' tagged buffer
+        if constexpr (sizeof(T) == 8)
+            if (tagged)
+                hipLaunchKernelGGL((k_fill_tag1<T>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, (long)n, A.z);
     }
     PFEM_HIP(hipGetLastError());
     const int batch = s->poll;
@@ -715,6 +723,7 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
                 o.alpha = &st->alpha[k & 1];
                 o.cg = st;
                 o.k = k;
+                o.tagged = tagged;
                 PFEM_REQUIRE(precond_apply_t<T>(c, A.pc, A.z, A.r, A.pc_fp32, o) > 0,
                              "fused PCG iteration without its partials");
                 hipLaunchKernelGGL((k_cg_xp_recorded<T>), dim3(blocks_for((n + 1) / 2)), dim3(kBlock), 0, c->stream, n, A.x, A.p,
@@ -1159,7 +1168,7 @@ int pfem_solver_set_option(pfem_solver s, int option, int value)
         switch (option) {
         case PFEM_SOLVER_OPT_FACTOR_ORDER: s->factor_order = value ? 1 : 0; break;
         case PFEM_SOLVER_OPT_FUSE:
-            PFEM_REQUIRE(value >= 0 && value <= 2, "fusion level out of range");
+            PFEM_REQUIRE(value >= 0 && value <= 3, "fusion level out of range");
             s->fuse = value;
             break;
         case PFEM_SOLVER_OPT_REPLACE:
@@ -1178,9 +1187,10 @@ int pfem_solver_set_option(pfem_solver s, int option, int value)
 int pfem_time_kernel(int op, pfem_csr a, pfem_precond pc, int launches, double* avg_ms)
 {
     PFEM_TRY(a ? a->ctx : nullptr, {
-        PFEM_REQUIRE(a && avg_ms && launches > 0 && op >= 0 && op <= 7, "bad arguments");
+        PFEM_REQUIRE(a && avg_ms && launches > 0 && op >= 0 && op <= 9, "bad arguments");
         PFEM_REQUIRE(op < 5 || (pc && pc->ordering != PFEM_ORDER_NATURAL && pc->sweep),
-                     "ops 5 to 7 need a multicolour preconditioner with single-pass sweeps");
+                     "ops 5 to 9 need a multicolour preconditioner with single-pass sweeps");
+        PFEM_REQUIRE(op < 8 || pc->sell, "ops 8 and 9 need the sliced sweep layout");
         PFEM_REQUIRE(op < 2 || pc, "ops 2 and 3 need a preconditioner");
         Context* c = C(a->ctx);
         const Pattern& P = *a->pat;
@@ -1194,9 +1204,13 @@ int pfem_time_kernel(int op, pfem_csr a, pfem_precond pc, int launches, double* 
             hipLaunchKernelGGL(k_to32, dim3(blocks_for(P.nnz)), dim3(kBlock), 0, c->stream, (long)P.nnz, a->val32.p, a->val.p);
         }
         if (op >= 2) {
-            if (op == 3 || op == 4 || op == 6) precond_enable_fp32(c, pc);
+            if (op == 3 || op == 4 || op == 6 || op == 8) precond_enable_fp32(c, pc);
             precond_initialize(c, pc, a);
         }
+        ApplyOpts<double> tag_opts{nullptr, 0, true};
+        tag_opts.tagged = true;
+        if (op >= 8)   // the tagged form's output buffer carries tag 1 between applies
+            hipLaunchKernelGGL((k_fill_tag1<double>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, (long)n, y.p);
         auto launch = [&]() {
             switch (op) {
             case 0: launch_spmv(c, P, a->val.p, x.p, y.p); break;
@@ -1206,7 +1220,8 @@ int pfem_time_kernel(int op, pfem_csr a, pfem_precond pc, int launches, double* 
             case 4: precond_apply_t<double>(c, pc, y.p, x.p, true, ApplyOpts<double>{}); break;
             case 5: precond_apply_t<double>(c, pc, y.p, x.p, false, ApplyOpts<double>{nullptr, 0, true}); break;
             case 6: precond_apply_t<double>(c, pc, y.p, x.p, true, ApplyOpts<double>{nullptr, 0, true}); break;
-            default: launch_spmv(c, *pc->fpat, pc->aval.p, x.p, y.p); break;
+            case 7: launch_spmv(c, *pc->fpat, pc->aval.p, x.p, y.p); break;
+            default: precond_apply_t<double>(c, pc, y.p, x.p, op == 8, tag_opts); break;
             }
         };
         launch();   // warm-up

@@ -171,8 +171,10 @@ __device__ __forceinline__ T dep_sum(T s, int p0, int p1, const int* __restrict_
 }
 
 // As dep_sum, over a row of a chunk-sliced triangle: entry w at slot e0 + w * stride (w < width), unused
-// slots marked by column -1 (they follow the row's entries, which keep their order).
-template <typename T, typename TS>
+// slots marked by column -1 (they follow the row's entries, which keep their order). WAIT < 0: buf holds
+// the sentinel until published; WAIT = 0 or 1: buf is a tagged buffer (FP64) and a dependency is ready
+// once it carries tag WAIT.
+template <int WAIT, typename T, typename TS>
 __device__ __forceinline__ T dep_sum_sliced(T s, int e0, int stride, int width, const int* __restrict__ col32,
                                             const unsigned short* __restrict__ col16, const ChunkBases& cb,
                                             const TS* __restrict__ f, T* buf, int* err, int limit)
@@ -199,7 +201,12 @@ __device__ __forceinline__ T dep_sum_sliced(T s, int e0, int stride, int width, 
 #pragma unroll
         for (int k = 0; k < kDeps; ++k)
             if (cv[k] >= 0) {
-                if (yv[k] == sent) yv[k] = __builtin_bit_cast(U, await_value(buf + cv[k], err, limit));
+                if constexpr (WAIT < 0) {
+                    if (yv[k] == sent) yv[k] = __builtin_bit_cast(U, await_value(buf + cv[k], err, limit));
+                } else {
+                    static_assert(sizeof(T) == 8, "tagged buffers hold FP64 values");
+                    if ((yv[k] & 1ull) != (U)WAIT) yv[k] = __builtin_bit_cast(U, await_tagged<WAIT>(buf + cv[k], err, limit));
+                }
                 s -= fv[k] * __builtin_bit_cast(T, yv[k]);
             }
     }
@@ -267,17 +274,19 @@ __device__ __forceinline__ bool sweep_row(const SweepArgs<TV, TS, TF>& a, int& i
     }
 }
 
-template <int LAYOUT, typename TV, typename TS, typename TF>
+template <int LAYOUT, int WAIT, typename TV, typename TS, typename TF>
 __device__ __forceinline__ TF sweep_sum(const SweepArgs<TV, TS, TF>& a, TF s, int i, int e0, int stride, int width, TF* buf)
 {
     if constexpr (LAYOUT == kSliced)
-        return dep_sum_sliced(s, e0, stride, width, a.col, a.col16,
+        return dep_sum_sliced<WAIT>(s, e0, stride, width, a.col, a.col16,
                               a.col16 ? chunk_bases(a.cbase + 4 * blockIdx.x) : ChunkBases{0, 0, 0, 0, true}, a.f,
                               buf, a.err, a.limit);
     else return dep_sum(s, a.ptr[i], a.ptr[i + 1], a.col, a.f, buf, a.err, a.limit);
 }
 
-template <typename TV, typename TS, typename TF, bool UNIT, int LAYOUT, bool XR>
+// TAGGED (sliced layout, FP64 published values): sy and sw are one buffer; the forward sweep publishes y
+// with tag 0, the backward sweep replaces it by z with tag 1, and neither re-arms anything.
+template <typename TV, typename TS, typename TF, bool UNIT, int LAYOUT, bool XR, bool TAGGED = false>
 __global__ __launch_bounds__(kBlock) void k_sweep_lower(SweepArgs<TV, TS, TF> a)
 {
     if (a.done && *a.done) return;
@@ -294,14 +303,20 @@ __global__ __launch_bounds__(kBlock) void k_sweep_lower(SweepArgs<TV, TS, TF> a)
     } else {
         b = a.src[a.perm ? a.perm[i] : i];
     }
-    TF s = sweep_sum<LAYOUT>(a, (TF)b, i, e0, stride, width, a.sy);
-    publish(a.sy + i, UNIT ? s : s / (TF)a.d[i]);
-    a.sw[i] = sentinel_value<TF>();
+    if constexpr (TAGGED) {
+        static_assert(LAYOUT == kSliced && sizeof(TF) == 8, "tagged sweeps: sliced layout, FP64 values");
+        const TF s = sweep_sum<LAYOUT, 0>(a, (TF)b, i, e0, stride, width, a.sy);
+        publish_tagged<0>(a.sy + i, UNIT ? s : s / (TF)a.d[i]);
+    } else {
+        const TF s = sweep_sum<LAYOUT, -1>(a, (TF)b, i, e0, stride, width, a.sy);
+        publish(a.sy + i, UNIT ? s : s / (TF)a.d[i]);
+        a.sw[i] = sentinel_value<TF>();
+    }
 }
 
 // backward solve, all levels: z_i = (y_i - sum_{k>i} U_ik z_k) / U_ii, out[perm[i]] = z_i; re-arms sy.
 // part != null: the chunk's partial of (r, z) (r = rdot, in the vectors' order) goes to part[chunk].
-template <typename TV, typename TS, typename TF, int LAYOUT>
+template <typename TV, typename TS, typename TF, int LAYOUT, bool TAGGED = false>
 __global__ __launch_bounds__(kBlock) void k_sweep_upper(SweepArgs<TV, TS, TF> a)
 {
     // (r, z) partial without a workgroup barrier after the sweep: each wave leaves its sum in LDS and the
@@ -317,10 +332,17 @@ __global__ __launch_bounds__(kBlock) void k_sweep_upper(SweepArgs<TV, TS, TF> a)
     int i, e0, stride, width;
     if (sweep_row<LAYOUT>(a, i, e0, stride, width)) {
         const TF yi = a.sy[i];
-        a.sy[i] = sentinel_value<TF>();
-        TF s = sweep_sum<LAYOUT>(a, yi, i, e0, stride, width, a.sw);
-        s = s / (TF)a.d[i];
-        publish(a.sw + i, s);
+        TF s;
+        if constexpr (TAGGED) {
+            s = sweep_sum<LAYOUT, 1>(a, yi, i, e0, stride, width, a.sw);
+            s = tagged<1>(s / (TF)a.d[i]);
+            publish_tagged<1>(a.sw + i, s);
+        } else {
+            a.sy[i] = sentinel_value<TF>();
+            s = sweep_sum<LAYOUT, -1>(a, yi, i, e0, stride, width, a.sw);
+            s = s / (TF)a.d[i];
+            publish(a.sw + i, s);
+        }
         const int o = a.perm ? a.perm[i] : i;
         const TV z = (TV)s;
         if ((const void*)a.out != (const void*)a.sw) a.out[o] = z;
@@ -971,6 +993,24 @@ static void sweeps(Context* c, pfem_precond_t* pc, const TV* src, TV* out, const
             lo.f = pc->sell_l.p;
             up.f = pc->sell_u.p;
         }
+        if constexpr (sizeof(TF) == 8) {
+            if (o.tagged) {
+                // y and z share out: y with tag 0, then z with tag 1 (out must carry tag 1 on entry)
+                PFEM_REQUIRE(o.factor_order && (const void*)out != (const void*)src && sizeof(TV) == 8,
+                             "tagged sweeps need vectors in the factor's ordering and z distinct from r");
+                lo.sy = lo.sw = up.sy = up.sw = reinterpret_cast<TF*>(out);
+                if (xr) {
+                    if (ic) hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, false, kSliced, true, true>), gl, bl, 0, c->stream, lo);
+                    else hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, true, kSliced, true, true>), gl, bl, 0, c->stream, lo);
+                } else {
+                    if (ic) hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, false, kSliced, false, true>), gl, bl, 0, c->stream, lo);
+                    else hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, true, kSliced, false, true>), gl, bl, 0, c->stream, lo);
+                }
+                hipLaunchKernelGGL((k_sweep_upper<TV, TS, TF, kSliced, true>), gu, bl, 0, c->stream, up);
+                return;
+            }
+        }
+        PFEM_REQUIRE(!o.tagged, "tagged sweeps need the sliced layout and FP64 values");
         if (xr) {
             if (ic) hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, false, kSliced, true>), gl, bl, 0, c->stream, lo);
             else hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, true, kSliced, true>), gl, bl, 0, c->stream, lo);
@@ -980,6 +1020,7 @@ static void sweeps(Context* c, pfem_precond_t* pc, const TV* src, TV* out, const
         }
         hipLaunchKernelGGL((k_sweep_upper<TV, TS, TF, kSliced>), gu, bl, 0, c->stream, up);
     } else if (pc->contiguous) {
+        PFEM_REQUIRE(!o.tagged, "tagged sweeps need the sliced layout");
         if (xr) {
             if (ic) hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, false, kRange, true>), gl, bl, 0, c->stream, lo);
             else hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, true, kRange, true>), gl, bl, 0, c->stream, lo);
@@ -989,6 +1030,7 @@ static void sweeps(Context* c, pfem_precond_t* pc, const TV* src, TV* out, const
         }
         hipLaunchKernelGGL((k_sweep_upper<TV, TS, TF, kRange>), gu, bl, 0, c->stream, up);
     } else {
+        PFEM_REQUIRE(!o.tagged, "tagged sweeps need the sliced layout");
         lo.rows = pc->lrows.p;
         up.rows = pc->urows.p;
         if (ic) hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, false, kRows, false>), gl, bl, 0, c->stream, lo);

@@ -224,7 +224,7 @@ struct pfem_solver_t {
     int factor_order = 1;   // run multicolour IC0/ILU0 solves in the factor's ordering
     // PCG kernel fusion in the factor's ordering: 0 none; 1 x/r update inside the forward sweep; 2 also
     // alpha and beta finished by the last workgroups of the SpMV and of the backward sweep
-    int fuse = 2;
+    int fuse = 3;
     int replace_digits = 2;   // reliable updates (multi/batched MIXED): replace after gamma fell 10^-digits
     hipEvent_t ev[2] = {nullptr, nullptr};   // pipelined polls of the device state
     std::unique_ptr<pfem::MultiWork, pfem::MultiWorkDeleter> multi;   // pfem_solver_solve_multi
@@ -262,6 +262,9 @@ struct ApplyOpts {
     // adds them and ends PCG iteration k in cg (cg_record); nothing further is needed for beta
     CgState* cg = nullptr;
     int k = 0;
+    // single-pass sliced sweeps in factor order with FP64 values: y and z share z's storage and are told
+    // apart by tags (no sentinel re-arming); z must carry tag 1 on entry and leaves with tag 1
+    bool tagged = false;
 };
 // z = M^-1 r with vectors of type TV. fp32 (TV = double): IC0/ILU0 with the factor stored in FP32 and the
 // solves computed in FP64. Returns

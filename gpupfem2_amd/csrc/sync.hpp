@@ -73,6 +73,52 @@ __global__ void k_fill_sentinel(long n, T* __restrict__ d)
     if (i < n) d[i] = sentinel_value<T>();
 }
 
+// Tagged publication: two launches that alternate on one FP64 buffer tell their values apart by the
+// lowest mantissa bit, the tag (0 in one launch, 1 in the other), so no launch has to re-arm a sentinel
+// before the next one. A published value has its lowest bit forced to the launch's tag (a change of at
+// most one unit in the last place, 2^-52 relative); a location reads with the other tag until it is
+// published in this launch, which plays the part of the sentinel. Quiet NaNs stay NaNs (their top
+// mantissa bit is set).
+template <int TAG>
+__device__ __forceinline__ double tagged(double v)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    return __builtin_bit_cast(double, TAG ? (b | 1ull) : (b & ~1ull));
+}
+
+template <int TAG>
+__device__ __forceinline__ void publish_tagged(double* p, double v)
+{
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), __builtin_bit_cast(unsigned long long, tagged<TAG>(v)),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Polls a location that did not yet carry tag TAG (the first ordinary load saw the other tag).
+template <int TAG>
+__device__ __forceinline__ double await_tagged(double* p, int* err, int limit)
+{
+    using U = unsigned long long;
+    U* q = reinterpret_cast<U*>(p);
+    U v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if ((v & 1ull) != (U)TAG) {
+        int spins = 0;
+        do {
+            __builtin_amdgcn_s_sleep(1);
+            v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } while ((v & 1ull) != (U)TAG && ++spins < limit);
+        if ((v & 1ull) != (U)TAG) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return __builtin_bit_cast(double, v);
+}
+
+// Every element of d carries tag 1 (the state a forward launch with tag 0 expects to find).
+template <typename T>
+__global__ void k_fill_tag1(long n, T* __restrict__ d)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = __builtin_bit_cast(T, 1ull);
+}
+
 
 // Grid total finished inside the launch by its last workgroup. Every workgroup publishes its partial
 // (one double, after block_sum) into part[blockIdx.x], which reads as the sentinel until then. The last

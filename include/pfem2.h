@@ -203,9 +203,12 @@ int pfem_solver_set_precision(pfem_solver s, int precision, double inner_tol, in
 #define PFEM_SOLVER_OPT_POLL 2
 #define PFEM_SOLVER_OPT_REPLACE 4 /* multi-RHS / batched PFEM_PRECISION_MIXED, default 2: a residual replacement
                                     follows every fall of (r, M^-1 r) by 10^-value (value 1..8) */
-#define PFEM_SOLVER_OPT_FUSE 3   /* PCG kernel fusion in factor order, default 2: 0 none; 1 the x/r update runs
+#define PFEM_SOLVER_OPT_FUSE 3   /* PCG kernel fusion in factor order, default 3: 0 none; 1 the x/r update runs
                                     inside the forward sweep; 2 also alpha and beta are finished by the last
-                                    workgroups of the SpMV and of the backward sweep */
+                                    workgroups of the SpMV and of the backward sweep; 3 also the two sweeps
+                                    share one buffer whose values carry a 1-bit tag in the lowest mantissa
+                                    bit instead of re-armed sentinels (y and z change by at most one unit in
+                                    the last place; levels 0 to 2 give the same bits as each other) */
 int pfem_solver_set_option(pfem_solver s, int option, int value);
 int pfem_solver_destroy(pfem_solver s);
 /* LinearSolver::solve(A, x, b) -> bool (linear_solver.cu:146-154, 301-376, 421-521): warm start from x,
@@ -361,7 +364,8 @@ int pfem_pfem2_precond(pfem_pfem2 s, int which, pfem_precond* out);
  * first); op 2: FP64 apply of pc (initialised from a first); op 3: the FP32 apply of pc; op 4: the apply
  * to FP64 vectors with the FP32 factor (PFEM_PRECISION_MIXED_PRECOND); ops 5 and 6: ops 2 and 4 on
  * vectors already in a multicolour factor's ordering (as the Krylov solvers run them); op 7: FP64 SpMV
- * of the matrix in that ordering (P a P^T, as the Krylov solvers run it). */
+ * of the matrix in that ordering (P a P^T, as the Krylov solvers run it); ops 8 and 9: ops 6 and 5 with
+ * the tagged sweeps of PCG fusion level 3 (sliced layout). */
 int pfem_time_kernel(int op, pfem_csr a, pfem_precond pc, int launches, double* avg_ms);
 
 /* ------------------------------------------------------------------ export (src/data_export.cuh) */

@@ -411,6 +411,24 @@ def test_factor_order_solve(ctx, oracle, kind, precond, prec):
                 x.upload(np.linspace(0.0, 0.1, xy.shape[0]))
                 st = s.solve(a, x, rhs)
                 assert st.iterations == out[1][0] and np.array_equal(x.download(), out[1][1]), (fuse, rep)
+        # level 3: y and z tagged in one buffer (values within one unit in the last place of level 2's):
+        # the same iteration count within 1, the same solution to 1e-12, and repeated solves bit-identical
+        pc = capi.Preconditioner(a, precond, capi.ORDER_MULTICOLOR)
+        s = capi.Solver(a, kind, 1e-12, 5000, 30, capi.TOL_RELATIVE, pc)
+        if prec:
+            s.set_precision(prec, 1e-5, 20)
+        s.set_option(capi.SOLVER_OPT_FUSE, 3)
+        first = None
+        for rep in range(2):
+            x = ctx.zeros(xy.shape[0])
+            x.upload(np.linspace(0.0, 0.1, xy.shape[0]))
+            st = s.solve(a, x, rhs)
+            assert st.converged and abs(st.iterations - out[1][0]) <= 1, (st.iterations, out[1][0])
+            assert rel_l2(x.download(), out[1][1]) < 1e-12
+            if first is None:
+                first = x.download()
+            else:
+                assert np.array_equal(x.download(), first)
 
 
 def test_sweep_wait_failure_is_reported(ctx, oracle):
