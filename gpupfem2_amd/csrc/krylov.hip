@@ -688,7 +688,7 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
         // z_0 has been copied into p: z's storage now becomes the sweeps' tagged buffer
         if constexpr (sizeof(T) == 8)
             if (tagged)
-                hipLaunchKernelGGL((k_fill_tag1<T>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, (long)n, A.z);
+                hipLaunchKernelGGL((k_fill_tag0<T>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, (long)n, A.z);
     }
     PFEM_HIP(hipGetLastError());
     const int batch = s->poll;
@@ -1209,8 +1209,8 @@ int pfem_time_kernel(int op, pfem_csr a, pfem_precond pc, int launches, double* 
         }
         ApplyOpts<double> tag_opts{nullptr, 0, true};
         tag_opts.tagged = true;
-        if (op >= 8)   // the tagged form's output buffer carries tag 1 between applies
-            hipLaunchKernelGGL((k_fill_tag1<double>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, (long)n, y.p);
+        if (op >= 8)   // the tagged form's output buffer carries tag 0 between applies
+            hipLaunchKernelGGL((k_fill_tag0<double>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, (long)n, y.p);
         auto launch = [&]() {
             switch (op) {
             case 0: launch_spmv(c, P, a->val.p, x.p, y.p); break;

@@ -285,7 +285,8 @@ __device__ __forceinline__ TF sweep_sum(const SweepArgs<TV, TS, TF>& a, TF s, in
 }
 
 // TAGGED (sliced layout, FP64 published values): sy and sw are one buffer; the forward sweep publishes y
-// with tag 0, the backward sweep replaces it by z with tag 1, and neither re-arms anything.
+// with tag 1, the backward sweep replaces it by z with tag 0, and neither re-arms anything. z keeps tag 0
+// so that an exact zero (a Dirichlet row's) stays exactly zero.
 template <typename TV, typename TS, typename TF, bool UNIT, int LAYOUT, bool XR, bool TAGGED = false>
 __global__ __launch_bounds__(kBlock) void k_sweep_lower(SweepArgs<TV, TS, TF> a)
 {
@@ -305,8 +306,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_lower(SweepArgs<TV, TS, TF> a)
     }
     if constexpr (TAGGED) {
         static_assert(LAYOUT == kSliced && sizeof(TF) == 8, "tagged sweeps: sliced layout, FP64 values");
-        const TF s = sweep_sum<LAYOUT, 0>(a, (TF)b, i, e0, stride, width, a.sy);
-        publish_tagged<0>(a.sy + i, UNIT ? s : s / (TF)a.d[i]);
+        const TF s = sweep_sum<LAYOUT, 1>(a, (TF)b, i, e0, stride, width, a.sy);
+        publish_tagged<1>(a.sy + i, UNIT ? s : s / (TF)a.d[i]);
     } else {
         const TF s = sweep_sum<LAYOUT, -1>(a, (TF)b, i, e0, stride, width, a.sy);
         publish(a.sy + i, UNIT ? s : s / (TF)a.d[i]);
@@ -334,9 +335,9 @@ __global__ __launch_bounds__(kBlock) void k_sweep_upper(SweepArgs<TV, TS, TF> a)
         const TF yi = a.sy[i];
         TF s;
         if constexpr (TAGGED) {
-            s = sweep_sum<LAYOUT, 1>(a, yi, i, e0, stride, width, a.sw);
-            s = tagged<1>(s / (TF)a.d[i]);
-            publish_tagged<1>(a.sw + i, s);
+            s = sweep_sum<LAYOUT, 0>(a, yi, i, e0, stride, width, a.sw);
+            s = tagged<0>(s / (TF)a.d[i]);
+            publish_tagged<0>(a.sw + i, s);
         } else {
             a.sy[i] = sentinel_value<TF>();
             s = sweep_sum<LAYOUT, -1>(a, yi, i, e0, stride, width, a.sw);
@@ -995,7 +996,7 @@ static void sweeps(Context* c, pfem_precond_t* pc, const TV* src, TV* out, const
         }
         if constexpr (sizeof(TF) == 8) {
             if (o.tagged) {
-                // y and z share out: y with tag 0, then z with tag 1 (out must carry tag 1 on entry)
+                // y and z share out: y with tag 1, then z with tag 0 (out must carry tag 0 on entry)
                 PFEM_REQUIRE(o.factor_order && (const void*)out != (const void*)src && sizeof(TV) == 8,
                              "tagged sweeps need vectors in the factor's ordering and z distinct from r");
                 lo.sy = lo.sw = up.sy = up.sw = reinterpret_cast<TF*>(out);

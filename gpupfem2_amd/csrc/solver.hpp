@@ -263,7 +263,7 @@ struct ApplyOpts {
     CgState* cg = nullptr;
     int k = 0;
     // single-pass sliced sweeps in factor order with FP64 values: y and z share z's storage and are told
-    // apart by tags (no sentinel re-arming); z must carry tag 1 on entry and leaves with tag 1
+    // apart by tags (no sentinel re-arming); z must carry tag 0 on entry (+0.0 does) and leaves with tag 0
     bool tagged = false;
 };
 // z = M^-1 r with vectors of type TV. fp32 (TV = double): IC0/ILU0 with the factor stored in FP32 and the

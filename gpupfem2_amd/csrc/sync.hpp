@@ -111,12 +111,12 @@ __device__ __forceinline__ double await_tagged(double* p, int* err, int limit)
     return __builtin_bit_cast(double, v);
 }
 
-// Every element of d carries tag 1 (the state a forward launch with tag 0 expects to find).
+// Every element of d reads +0.0, which carries tag 0 (the state a launch publishing with tag 1 expects).
 template <typename T>
-__global__ void k_fill_tag1(long n, T* __restrict__ d)
+__global__ void k_fill_tag0(long n, T* __restrict__ d)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) d[i] = __builtin_bit_cast(T, 1ull);
+    if (i < n) d[i] = (T)0;
 }
 
 

@@ -388,6 +388,8 @@ def test_factor_order_solve(ctx, oracle, kind, precond, prec):
         if prec:
             s.set_precision(prec, 1e-5, 20)
         s.set_option(capi.SOLVER_OPT_FACTOR_ORDER, fo)
+        if kind == 0:
+            s.set_option(capi.SOLVER_OPT_FUSE, 2)
         x = ctx.zeros(xy.shape[0])
         x.upload(np.linspace(0.0, 0.1, xy.shape[0]))      # a warm start exercises the permutation of x
         st = s.solve(a, x, rhs)
@@ -396,7 +398,7 @@ def test_factor_order_solve(ctx, oracle, kind, precond, prec):
     assert abs(out[0][0] - out[1][0]) <= max(3, out[0][0] // 20)
     assert rel_l2(out[1][1], out[0][1]) < 1e-10
     if kind == 0:
-        # PCG fused (the default: alpha and beta finished by the last workgroups of the SpMV and of the
+        # PCG fused (level 2: alpha and beta finished by the last workgroups of the SpMV and of the
         # backward sweep, the x/r update inside the forward sweep; level 1: only the x/r update) and
         # unfused: the same operations, so the same bits; two solves in a row check that the partial
         # slots are re-armed
@@ -411,8 +413,9 @@ def test_factor_order_solve(ctx, oracle, kind, precond, prec):
                 x.upload(np.linspace(0.0, 0.1, xy.shape[0]))
                 st = s.solve(a, x, rhs)
                 assert st.iterations == out[1][0] and np.array_equal(x.download(), out[1][1]), (fuse, rep)
-        # level 3: y and z tagged in one buffer (values within one unit in the last place of level 2's):
-        # the same iteration count within 1, the same solution to 1e-12, and repeated solves bit-identical
+        # level 3 (the default): y and z tagged in one buffer (values within one unit in the last place of
+        # level 2's): the same iteration count within 1, the same solution to 1e-12, repeated solves
+        # bit-identical, and the Dirichlet rows' zeros kept exactly
         pc = capi.Preconditioner(a, precond, capi.ORDER_MULTICOLOR)
         s = capi.Solver(a, kind, 1e-12, 5000, 30, capi.TOL_RELATIVE, pc)
         if prec:
@@ -425,6 +428,7 @@ def test_factor_order_solve(ctx, oracle, kind, precond, prec):
             st = s.solve(a, x, rhs)
             assert st.converged and abs(st.iterations - out[1][0]) <= 1, (st.iterations, out[1][0])
             assert rel_l2(x.download(), out[1][1]) < 1e-12
+            assert np.array_equal(x.download()[out[1][1] == 0.0], out[1][1][out[1][1] == 0.0])
             if first is None:
                 first = x.download()
             else:
