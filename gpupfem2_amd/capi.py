@@ -18,7 +18,7 @@ ORDER_NATURAL, ORDER_MULTICOLOR = 0, 1
 SOLVER_CG, SOLVER_GMRES = 0, 1
 TOL_ABSOLUTE, TOL_RELATIVE = 0, 1
 PRECISION_FP64, PRECISION_MIXED, PRECISION_MIXED_PRECOND = 0, 1, 2
-SOLVER_OPT_FACTOR_ORDER, SOLVER_OPT_POLL, SOLVER_OPT_FUSE, SOLVER_OPT_REPLACE = 1, 2, 3, 4
+SOLVER_OPT_FACTOR_ORDER, SOLVER_OPT_POLL, SOLVER_OPT_FUSE, SOLVER_OPT_REPLACE, SOLVER_OPT_GRAPH = 1, 2, 3, 4, 5
 
 
 class SolveStats(C.Structure):
@@ -464,7 +464,7 @@ class Solver(_Handle):
 
     def set_option(self, option, value):
         """SOLVER_OPT_FACTOR_ORDER (0/1), SOLVER_OPT_POLL (iterations between convergence polls) or
-        SOLVER_OPT_FUSE (0/1/2/3, default 3)."""
+        SOLVER_OPT_FUSE (0/1/2/3, default 3), SOLVER_OPT_REPLACE or SOLVER_OPT_GRAPH (0/1, default 0)."""
         self.ctx.check(lib().pfem_solver_set_option(self.h, option, int(value)))
 
     def set_precision(self, precision, inner_tol=0.0, max_refinements=0):

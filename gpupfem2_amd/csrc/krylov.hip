@@ -138,10 +138,13 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv_alpha(const int* __restrict__ tiles, const int* __restrict__ rowptr,
                                                           const int* __restrict__ col, const T* __restrict__ val,
                                                           const T* __restrict__ p, T* __restrict__ q, CgState* st,
-                                                          double* part, int k, int* err, int limit)
+                                                          double* part, int* err, int limit)
 {
     __shared__ T prod[kTileNnz];
-    if (st->done) return;
+    if (st->done) {
+        if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) st->xp_seen = st->it;
+        return;
+    }
     int row, r1;
     const T ap = tile_row_product<T, T>(tiles, rowptr, col, val, p, prod, row, r1);
     double v[1] = {0.0};
@@ -153,7 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_alpha(const int* __restrict_
     publish_partial(v[0], part);
     if (blockIdx.x == gridDim.x - 1) {
         const double pq = collect_partials(part, gridDim.x, err, limit);
-        if (threadIdx.x == 0) st->alpha[k & 1] = st->gam[k & 1] / pq;
+        if (threadIdx.x == 0) cg_alpha_record(st, pq);
     }
 }
 
@@ -167,9 +170,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_alpha_sliced(const int* __re
                                                                  const int* __restrict__ cbase,
                                                                  const double* __restrict__ val,
                                                                  const double* __restrict__ p, double* __restrict__ q,
-                                                                 CgState* st, double* part, int k, int* err, int limit)
+                                                                 CgState* st, double* part, int* err, int limit)
 {
-    if (st->done) return;
+    if (st->done) {
+        if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) st->xp_seen = st->it;
+        return;
+    }
     const int* h = hdr + 4 * blockIdx.x;
     const ChunkBases cb = col16 ? chunk_bases(cbase + 4 * blockIdx.x) : ChunkBases{0, 0, 0, 0, true};
     const int cnt = h[1], width = h[3];
@@ -203,19 +209,22 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_alpha_sliced(const int* __re
     publish_partial(v[0], part);
     if (blockIdx.x == gridDim.x - 1) {
         const double pq = collect_partials(part, gridDim.x, err, limit);
-        if (threadIdx.x == 0) st->alpha[k & 1] = st->gam[k & 1] / pq;
+        if (threadIdx.x == 0) cg_alpha_record(st, pq);
     }
 }
 
-// End of fused PCG iteration k: x += alpha_k p_k (the x half of updateXR, linear_solver.cu:68-77, moved
-// here from the forward sweep so that p_k is read once for both updates), then, unless iteration k
-// stopped the solve, p = z + beta p with beta_{k+1} = gamma_{k+1} / gamma_k already recorded by the
-// backward sweep's last workgroup (updateP, linear_solver.cu:79-87). One element per lane.
+// End of fused PCG iteration k = it - 1 (the backward sweep has just counted it): x += alpha_k p_k (the x
+// half of updateXR, linear_solver.cu:68-77, moved here from the forward sweep so that p_k is read once for
+// both updates), then, unless iteration k stopped the solve, p = z + beta p with beta_{k+1} =
+// gamma_{k+1} / gamma_k already recorded by the backward sweep's last workgroup (updateP,
+// linear_solver.cu:79-87). A launch queued after the solve stopped finds it == xp_seen and does nothing.
 template <typename T>
 __global__ void k_cg_xp_recorded(int n, T* __restrict__ x, T* __restrict__ p, const T* __restrict__ z,
-                                 const CgState* st, int k)
+                                 const CgState* st)
 {
-    if (st->done && st->it != k + 1) return;   // stopped before this iteration
+    const int it = st->it;
+    if (it == st->xp_seen) return;
+    const int k = it - 1;
     const bool last = st->done;
     const T al = (T)st->alpha[k & 1];
     const T beta = last ? (T)0 : (T)(st->gam[(k + 1) & 1] / st->gam[k & 1]);
@@ -694,40 +703,66 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
     const int batch = s->poll;
     const int na = P.ntiles, nb = (int)g;
     int launched = 0;
+    // one fused iteration (four launches); the iteration index lives on the device, so every iteration is
+    // the same sequence of launches
+    auto fused_iteration = [&]() {
+        bool sliced = false;
+        if constexpr (sizeof(T) == 8) {
+            sliced = A.pc->asl_val.p && A.val == A.pc->aval.p;
+            if (sliced)
+                hipLaunchKernelGGL(k_cg_spmv_alpha_sliced, dim3(P.ntiles), dim3(kBlock), 0, c->stream,
+                                   A.pc->sl_all.hdr.p, A.pc->sl_all.col.p,
+                                   (const unsigned short*)nullptr, A.pc->sl_all.cbase.p,
+                                   (const double*)A.pc->asl_val.p,
+                                   (const double*)A.p, A.q, st, s->pa, A.pc->sweep_err.p, A.pc->spin_limit);
+        }
+        if (!sliced)
+            hipLaunchKernelGGL((k_cg_spmv_alpha<T>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p,
+                               P.rowptr.p, P.col.p, A.val, A.p, A.q, st, s->pa, A.pc->sweep_err.p,
+                               A.pc->spin_limit);
+        ApplyOpts<T> o{s->pb, s->pb_cap, true};
+        o.done = &st->done;
+        o.q = A.q;
+        o.alpha = st->alpha;
+        o.alpha_cg = st;
+        o.cg = st;
+        o.tagged = tagged;
+        PFEM_REQUIRE(precond_apply_t<T>(c, A.pc, A.z, A.r, A.pc_fp32, o) > 0,
+                     "fused PCG iteration without its partials");
+        hipLaunchKernelGGL((k_cg_xp_recorded<T>), dim3(blocks_for((n + 1) / 2)), dim3(kBlock), 0, c->stream, n, A.x,
+                           A.p, (const T*)A.z, st);
+    };
+    // a batch of fused iterations captured once per solve as a HIP graph and replayed
+    hipGraphExec_t exec = nullptr;
+    if (fused2 && s->graph) {
+        hipGraph_t graph = nullptr;
+        PFEM_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            for (int j = 0; j < batch; ++j) fused_iteration();
+        } catch (...) {
+            (void)hipStreamEndCapture(c->stream, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        PFEM_HIP(hipStreamEndCapture(c->stream, &graph));
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        PFEM_HIP(e);
+    }
+    struct ExecGuard {
+        hipGraphExec_t& e;
+        ~ExecGuard() { if (e) (void)hipGraphExecDestroy(e); }
+    } exec_guard{exec};
     // the state of batch b is copied to slot b & 1 and checked after batch b + 1 is queued, so the stream
     // never drains while the host waits; a converged solve runs at most one batch of no-op launches
     int pend = -1, pend_launched = 0, slot = 0;
     while (true) {
-        for (int j = 0; j < batch; ++j) {
+        if (exec) PFEM_HIP(hipGraphLaunch(exec, c->stream));
+        for (int j = 0; j < batch && !exec; ++j) {
             const int k = launched + j;
             int nbk = nb;
             if (fused2) {
-                bool sliced = false;
-                if constexpr (sizeof(T) == 8) {
-                    sliced = A.pc->asl_val.p && A.val == A.pc->aval.p;
-                    if (sliced)
-                        hipLaunchKernelGGL(k_cg_spmv_alpha_sliced, dim3(P.ntiles), dim3(kBlock), 0, c->stream,
-                                           A.pc->sl_all.hdr.p, A.pc->sl_all.col.p,
-                                           (const unsigned short*)nullptr, A.pc->sl_all.cbase.p,
-                                           (const double*)A.pc->asl_val.p,
-                                           (const double*)A.p, A.q, st, s->pa, k, A.pc->sweep_err.p,
-                                           A.pc->spin_limit);
-                }
-                if (!sliced)
-                    hipLaunchKernelGGL((k_cg_spmv_alpha<T>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p,
-                                       P.rowptr.p, P.col.p, A.val, A.p, A.q, st, s->pa, k, A.pc->sweep_err.p,
-                                       A.pc->spin_limit);
-                ApplyOpts<T> o{s->pb, s->pb_cap, true};
-                o.done = &st->done;
-                o.q = A.q;
-                o.alpha = &st->alpha[k & 1];
-                o.cg = st;
-                o.k = k;
-                o.tagged = tagged;
-                PFEM_REQUIRE(precond_apply_t<T>(c, A.pc, A.z, A.r, A.pc_fp32, o) > 0,
-                             "fused PCG iteration without its partials");
-                hipLaunchKernelGGL((k_cg_xp_recorded<T>), dim3(blocks_for((n + 1) / 2)), dim3(kBlock), 0, c->stream, n, A.x, A.p,
-                                   (const T*)A.z, st, k);
+                fused_iteration();
                 continue;
             }
             hipLaunchKernelGGL((k_cg_spmv<T>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.p, A.q, st, s->pa);
@@ -1175,6 +1210,7 @@ int pfem_solver_set_option(pfem_solver s, int option, int value)
             PFEM_REQUIRE(value >= 1 && value <= 8, "replacement interval out of range");
             s->replace_digits = value;
             break;
+        case PFEM_SOLVER_OPT_GRAPH: s->graph = value ? 1 : 0; break;
         case PFEM_SOLVER_OPT_POLL:
             PFEM_REQUIRE(value >= 1 && value <= 4096, "poll interval out of range");
             s->poll = value;

@@ -243,7 +243,7 @@ struct SweepArgs {
     const TV* rdot;
     double* part;
     CgState* cg;
-    int k;
+    const CgState* acg;   // non-null: alpha = acg->alpha[acg->it & 1] (the device's iteration index)
 };
 
 // forward solve, all levels: y_i = (b_i - sum_{k<i} L_ik y_k) [/ L_ii for IC0], b_i = src[perm[i]];
@@ -297,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_lower(SweepArgs<TV, TS, TF> a)
     if constexpr (XR) {
         // updateXR (linear_solver.cu:68-77) for this row, as the separate vector kernel computes it (the
         // x half is left to the p-update kernel when a.x is null)
-        const TV al = (TV)*a.alpha;
+        const TV al = (TV)(a.acg ? a.acg->alpha[a.acg->it & 1] : *a.alpha);
         if (a.x) a.x[i] += al * a.p[i];
         b = a.r[i] - al * a.q[i];
         a.r[i] = b;
@@ -366,7 +366,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_upper(SweepArgs<TV, TS, TF> a)
         // gamma_{k+1} = (r, z) and the stopping test, which the separate p-update kernel does otherwise
         __syncthreads();
         const double g = collect_partials(a.part, gridDim.x, a.err, a.limit);
-        if (threadIdx.x == 0) cg_record(a.cg, g, a.k);
+        // iteration k = it (it counts the iterations ended so far)
+        if (threadIdx.x == 0) cg_record(a.cg, g, a.cg->it);
     }
 }
 
@@ -966,7 +967,7 @@ static void sweeps(Context* c, pfem_precond_t* pc, const TV* src, TV* out, const
     a.rdot = src;
     a.part = part;
     a.cg = part ? o.cg : nullptr;
-    a.k = o.k;
+    a.acg = o.alpha_cg;
     const bool xr = o.alpha != nullptr;
     PFEM_REQUIRE(!xr || (o.factor_order && pc->contiguous && o.q && (o.p || !o.x)),
                  "the fused x/r update needs a multicolour factor and vectors in its ordering");

@@ -60,6 +60,10 @@ struct CgState {
     int tol_mode, maxit;
     double gam[2], alpha[2], thr, res;
     int it, done, converged, nan;
+    // fused iterations: the value of it when the current iteration's SpMV ran; the x/p update of an
+    // iteration applies only while it differs (the iteration ended since), so launches queued past
+    // the end of the solve change nothing
+    int xp_seen;
 };
 
 // End of PCG iteration k, given gnew = (r_{k+1}, z_{k+1}): gamma_{k+1}, the iteration count and the
@@ -78,6 +82,16 @@ __device__ __forceinline__ void cg_record(CgState* st, double gnew, int k)
     } else if (k + 1 >= st->maxit) {
         st->done = 1;
     }
+}
+
+// alpha_k = gamma_k / (p_k, A p_k) of the iteration the state is in (k = it), recorded by the fused SpMV's
+// last workgroup, which also marks the iteration's x/p update as due (xp_seen = k < it once the
+// iteration ends); one thread.
+__device__ __forceinline__ void cg_alpha_record(CgState* st, double pq)
+{
+    const int k = st->it;
+    st->alpha[k & 1] = st->gam[k & 1] / pq;
+    st->xp_seen = k;
 }
 
 // Device-resident GMRES(m) scalars and small arrays (H column-major with leading dimension m).
@@ -225,6 +239,7 @@ struct pfem_solver_t {
     // PCG kernel fusion in the factor's ordering: 0 none; 1 x/r update inside the forward sweep; 2 also
     // alpha and beta finished by the last workgroups of the SpMV and of the backward sweep
     int fuse = 3;
+    int graph = 0;                        // fused PCG: replay a captured batch of iterations
     int replace_digits = 2;   // reliable updates (multi/batched MIXED): replace after gamma fell 10^-digits
     hipEvent_t ev[2] = {nullptr, nullptr};   // pipelined polls of the device state
     std::unique_ptr<pfem::MultiWork, pfem::MultiWorkDeleter> multi;   // pfem_solver_solve_multi
@@ -259,9 +274,13 @@ struct ApplyOpts {
     const TV* q = nullptr;
     const double* alpha = nullptr;
     // with part: the backward sweep publishes its partials (sentinel-armed slots) and its last workgroup
-    // adds them and ends PCG iteration k in cg (cg_record); nothing further is needed for beta
+    // adds them and ends the PCG iteration cg is in (iteration cg->it, cg_record); nothing further is
+    // needed for beta
     CgState* cg = nullptr;
-    int k = 0;
+    // with alpha pointing at alpha_cg->alpha: the forward sweep takes alpha[it & 1] of the iteration the
+    // state is in (the iteration index lives on the device, so a batch of iterations is always the same
+    // sequence of launches)
+    const CgState* alpha_cg = nullptr;
     // single-pass sliced sweeps in factor order with FP64 values: y and z share z's storage and are told
     // apart by tags (no sentinel re-arming); z must carry tag 0 on entry (+0.0 does) and leaves with tag 0
     bool tagged = false;

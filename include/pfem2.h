@@ -203,6 +203,9 @@ int pfem_solver_set_precision(pfem_solver s, int precision, double inner_tol, in
 #define PFEM_SOLVER_OPT_POLL 2
 #define PFEM_SOLVER_OPT_REPLACE 4 /* multi-RHS / batched PFEM_PRECISION_MIXED, default 2: a residual replacement
                                     follows every fall of (r, M^-1 r) by 10^-value (value 1..8) */
+#define PFEM_SOLVER_OPT_GRAPH 5  /* fused PCG (fusion level 2 or more), default 0: 1 makes each solve capture one
+                                    batch of iterations (the poll interval) as a HIP graph and replay it; 0
+                                    launches the kernels one by one (the same bits either way) */
 #define PFEM_SOLVER_OPT_FUSE 3   /* PCG kernel fusion in factor order, default 3: 0 none; 1 the x/r update runs
                                     inside the forward sweep; 2 also alpha and beta are finished by the last
                                     workgroups of the SpMV and of the backward sweep; 3 also the two sweeps

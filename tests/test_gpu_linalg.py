@@ -433,6 +433,30 @@ def test_factor_order_solve(ctx, oracle, kind, precond, prec):
                 first = x.download()
             else:
                 assert np.array_equal(x.download(), first)
+        # fused batches replayed from a HIP graph or launched kernel by kernel (the default), with an odd
+        # poll interval too (the iteration parity is kept on the device): the same bits, and eager work
+        # between the solves does not disturb the replays
+        for fuse in (2, 3):
+            ref = None
+            for graph, poll in ((0, 8), (1, 8), (1, 3), (1, 8)):
+                pc = capi.Preconditioner(a, precond, capi.ORDER_MULTICOLOR)
+                s = capi.Solver(a, kind, 1e-12, 5000, 30, capi.TOL_RELATIVE, pc)
+                if prec:
+                    s.set_precision(prec, 1e-5, 20)
+                s.set_option(capi.SOLVER_OPT_FUSE, fuse)
+                s.set_option(capi.SOLVER_OPT_GRAPH, graph)
+                s.set_option(capi.SOLVER_OPT_POLL, poll)
+                x = ctx.zeros(xy.shape[0])
+                x.upload(np.linspace(0.0, 0.1, xy.shape[0]))
+                st = s.solve(a, x, rhs)
+                y = ctx.empty(xy.shape[0])
+                a.spmv(x, y)
+                got = (st.iterations, x.download(), y.download())
+                if ref is None:
+                    ref = got
+                else:
+                    assert got[0] == ref[0] and np.array_equal(got[1], ref[1]), (fuse, graph, poll)
+                    assert np.array_equal(got[2], ref[2])
 
 
 def test_sweep_wait_failure_is_reported(ctx, oracle):
