@@ -375,7 +375,7 @@ def test_mixed_factor_precision(ctx, oracle, kind, precond, ordering):
     assert rel_l2(xs[1], exact) < 1e-9 and rel_l2(xs[0], exact) < 1e-9
 
 
-@pytest.mark.parametrize("kind,precond,prec", [(0, 2, 0), (0, 2, 2), (1, 3, 0)])
+@pytest.mark.parametrize("kind,precond,prec", [(0, 2, 0), (0, 2, 2), (0, 2, 1), (1, 3, 0)])
 def test_factor_order_solve(ctx, oracle, kind, precond, prec):
     """A Krylov solve with a multicolour IC0/ILU0 run in the factor's ordering (P A P^T, P b, P x) gives the
     natural-order solve's solution (reduction order aside) and about its iteration count."""
