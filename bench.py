@@ -92,18 +92,18 @@ def roofline_ic0(ctx, st, precision):
     a, pc = st.matrix_handle(2), st.precond_handle(1)
     n, nnz = csr_info(a)
     # the apply as the pressure PCG runs it: vectors in the multicolour factor's ordering, y and z in
-    # one tagged buffer (fusion level 3)
-    fp32 = precision == 1
-    op = 8 if fp32 else 9
+    # one tagged buffer (fusion level 3); in mixed precision the step's factor copy is 16-bit
+    mixed = precision == 1
+    op = 8 if mixed else 9
     ms = capi.time_kernel(ctx, op, a, pc, launches=200)
-    b = ic0_apply_bytes(n, nnz, 4 if fp32 else 8)
+    b = ic0_apply_bytes(n, nnz, 2 if mixed else 8)
     gbs = b / (ms * 1e-3) / 1e9
-    traffic = measured_traffic("traffic_bytes_ic0_apply_fp32_factor" if fp32 else "traffic_bytes_ic0_apply_fp64")
+    traffic = measured_traffic("traffic_bytes_ic0_apply_fp16_factor" if mixed else "traffic_bytes_ic0_apply_fp64")
     return dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(gbs / HBM_PEAK_GBS, 4), traffic=traffic,
                 traffic_source="profiles/ic0_apply_cylinder2M.md" if traffic else None,
                 kernel=(f"IC0 apply = k_sweep_lower + k_sweep_upper (tagged), pressure matrix n={n} nnz={nnz}, "
-                        f"factor {'FP32' if fp32 else 'FP64'}, vectors FP64 in factor order"),
+                        f"factor {'FP16 (power-of-two scaled)' if mixed else 'FP64'}, vectors FP64 in factor order"),
                 avg_us=round(ms * 1e3, 2), bytes_per_launch=b)
 
 
@@ -183,7 +183,7 @@ def cpu_baseline(case, st, mesh, gpu_its):
 def kernels_only(h, launches):
     """Pressure-system kernels alone: the cylinder mesh's P1 stiffness matrix (the pressure matrix:
     it depends on the geometry only) with the pressure Dirichlet rows, its multicolour IC0 factor, then
-    `launches` IC0 applies on vectors in the factor's ordering (FP32-stored factor, then FP64), FP64 SpMVs
+    `launches` IC0 applies on vectors in the factor's ordering (16-bit, FP32 and FP64 factor), FP64 SpMVs
     in the natural and in the factor's ordering, and device-to-device copies of 256 MiB (the calibration
     of the byte counters)."""
     import numpy as np
@@ -197,12 +197,16 @@ def kernels_only(h, launches):
     capi.Dirichlet(ctx, case.xy.shape[0], case.bcp_node, case.bcp_val).apply(a, rhs, capi.DIRICHLET_MASK)
     pc = capi.Preconditioner(a, capi.PRECOND_IC0, capi.ORDER_MULTICOLOR)
     pc.initialize(a)
+    pc16 = capi.Preconditioner(a, capi.PRECOND_IC0, capi.ORDER_MULTICOLOR)
+    pc16.set_factor_bits(16)
+    pc16.initialize(a)
     n, nnz = a.n, a.nnz
     print(f"[kernels] n={n} nnz={nnz}", file=sys.stderr, flush=True)
     res = {}
-    for op, name in ((8, "ic0_apply_fp32_factor"), (9, "ic0_apply_fp64"), (6, "ic0_apply_fp32_factor_sentinel"),
-                     (5, "ic0_apply_fp64_sentinel"), (0, "spmv_fp64"), (7, "spmv_fp64_factor_order")):
-        ms = capi.time_kernel(ctx, op, a.h, pc.h if op else None, launches=launches)
+    for op, name, p in ((8, "ic0_apply_fp16_factor", pc16), (8, "ic0_apply_fp32_factor", pc), (9, "ic0_apply_fp64", pc),
+                        (6, "ic0_apply_fp32_factor_sentinel", pc), (5, "ic0_apply_fp64_sentinel", pc),
+                        (0, "spmv_fp64", None), (7, "spmv_fp64_factor_order", pc)):
+        ms = capi.time_kernel(ctx, op, a.h, p.h if p else None, launches=launches)
         res[name] = round(ms * 1e3, 2)
         print(f"[kernels] {name}: {ms * 1e3:.2f} us", file=sys.stderr, flush=True)
     mib = 256
@@ -212,6 +216,7 @@ def kernels_only(h, launches):
     ctx.sync()
     print(f"[kernels] copied {launches} x {mib} MiB", file=sys.stderr, flush=True)
     res["copy_bytes_read"] = mib << 20
+    res["ic0_bytes_fp16_factor"] = ic0_apply_bytes(n, nnz, 2)
     res["ic0_bytes_fp32_factor"] = ic0_apply_bytes(n, nnz, 4)
     res["ic0_bytes_fp64"] = ic0_apply_bytes(n, nnz, 8)
     print(json.dumps(res))

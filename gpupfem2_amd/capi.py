@@ -39,7 +39,7 @@ class StepConfig(C.Structure):
                 ("tol", C.c_double), ("tol_mode", C.c_int), ("maxit", C.c_int), ("restart", C.c_int),
                 ("dirichlet_mode", C.c_int), ("body_id", C.c_int), ("body_cx", C.c_double),
                 ("body_cy", C.c_double), ("force_coeff", C.c_double), ("precision", C.c_int),
-                ("inner_tol", C.c_double), ("pres_ordering", C.c_int)]
+                ("inner_tol", C.c_double), ("pres_ordering", C.c_int), ("pres_factor_bits", C.c_int)]
 
 
 class SimulationParameters(C.Structure):
@@ -105,6 +105,7 @@ SIGNATURES = {
     "pfem_precond_download": (_i, [_vp, _vp, C.POINTER(_i), _vp]),
     "pfem_precond_set_sweep": (_i, [_vp, _i]),
     "pfem_precond_set_spin_limit": (_i, [_vp, _i]),
+    "pfem_precond_set_factor_bits": (_i, [_vp, _i]),
     "pfem_solver_set_option": (_i, [_vp, _i, _i]),
     "pfem_precond_create_custom": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
     "pfem_mesh2d_device_ptr": (_i, [_vp, _i, _vp]),
@@ -442,6 +443,10 @@ class Preconditioner(_Handle):
     def set_spin_limit(self, polls):
         """Polls of one single-pass wait before it is declared failed (PFEM_ENUMERIC)."""
         self.ctx.check(lib().pfem_precond_set_spin_limit(self.h, int(polls)))
+
+    def set_factor_bits(self, bits):
+        """16 or 32: storage of the factor copy PRECISION_MIXED_PRECOND applies (before set_precision)."""
+        self.ctx.check(lib().pfem_precond_set_factor_bits(self.h, int(bits)))
 
     def download(self):
         nl = C.c_int()

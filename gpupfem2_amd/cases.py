@@ -228,8 +228,9 @@ def build_batched_poisson(ctx, case):
     return mesh, a, ctx.to_device(case["rhs"])
 
 
-def step_config(case, precision=0, inner_tol=1e-6, pres_ordering=1):
-    """pfem_step_config (include/pfem2.h) for a case."""
+def step_config(case, precision=0, inner_tol=1e-6, pres_ordering=1, pres_factor_bits=16):
+    """pfem_step_config (include/pfem2.h) for a case; in mixed precision a multicolour IC0 pressure factor
+    is applied from a 16-bit copy by default."""
     from .capi import StepConfig, make_params
     cfg = StepConfig()
     cfg.prm = make_params(case.rho, case.mu, case.dt, case.scheme, case.traction_mask, case.normals)
@@ -246,6 +247,7 @@ def step_config(case, precision=0, inner_tol=1e-6, pres_ordering=1):
     cfg.precision = precision
     cfg.inner_tol = inner_tol
     cfg.pres_ordering = pres_ordering
+    cfg.pres_factor_bits = pres_factor_bits
     return cfg
 
 

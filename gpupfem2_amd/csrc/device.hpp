@@ -15,6 +15,13 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+    return v;
+}
+
 __device__ __forceinline__ float wave_sumf(float v)
 {
 #pragma unroll

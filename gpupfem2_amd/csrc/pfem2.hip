@@ -82,10 +82,14 @@ struct Phase {
     }
 };
 
-void make_solver(pfem_pfem2_t* s, pfem_csr a, int kind, int pc_kind, int ordering, pfem_precond* pc, pfem_solver* sv)
+void make_solver(pfem_pfem2_t* s, pfem_csr a, int kind, int pc_kind, int ordering, pfem_precond* pc, pfem_solver* sv,
+                 int factor_bits = 32)
 {
     const pfem_step_config& c = s->cfg;
     check(pfem_precond_create(a, pc_kind, ordering, pc), s->ctx);
+    if (c.precision == PFEM_PRECISION_MIXED && factor_bits == 16 && pc_kind == PFEM_PRECOND_IC0 &&
+        ordering != PFEM_ORDER_NATURAL)
+        check(pfem_precond_set_factor_bits(*pc, 16), s->ctx);
     check(pfem_solver_create(a, kind, c.tol, c.maxit, c.restart, c.tol_mode, *pc, sv), s->ctx);
     if (c.precision == PFEM_PRECISION_MIXED) {
         const bool factor = pc_kind == PFEM_PRECOND_IC0 || pc_kind == PFEM_PRECOND_ILU0;
@@ -130,7 +134,10 @@ int pfem_pfem2_create(pfem_mesh2d m, pfem_particles ph, const pfem_step_config* 
         check(pfem_dirichlet_create(s->ctx, s->n, nbcu, bcu_node, bcu_val1, &s->bcpred[1]), s->ctx);
         check(pfem_dirichlet_create(s->ctx, s->n, nbcp, bcp_node, bcp_val, &s->bcp), s->ctx);
         make_solver(s.get(), s->A0, cfg->pred_solver, cfg->pred_precond, PFEM_ORDER_NATURAL, &s->pc_pred, &s->s_pred);
-        make_solver(s.get(), s->Ap, cfg->pres_solver, cfg->pres_precond, cfg->pres_ordering, &s->pc_pres, &s->s_pres);
+        PFEM_REQUIRE(cfg->pres_factor_bits == 0 || cfg->pres_factor_bits == 16 || cfg->pres_factor_bits == 32,
+                     "pres_factor_bits must be 0, 16 or 32");
+        make_solver(s.get(), s->Ap, cfg->pres_solver, cfg->pres_precond, cfg->pres_ordering, &s->pc_pres, &s->s_pres,
+                    cfg->pres_factor_bits ? cfg->pres_factor_bits : 32);
         make_solver(s.get(), s->Ac, cfg->corr_solver, cfg->corr_precond, PFEM_ORDER_NATURAL, &s->pc_corr, &s->s_corr);
         PFEM_HIP(hipEventCreate(&s->ev[0]));
         PFEM_HIP(hipEventCreate(&s->ev[1]));

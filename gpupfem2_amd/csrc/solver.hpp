@@ -213,6 +213,11 @@ struct pfem_precond_t {
     pfem::SlicedCols sl_lower, sl_upper, sl_all;
     pfem::DevArray<double> sell_l, sell_u, asl_val;
     pfem::DevArray<float> sell_l32, sell_u32;
+    // the reduced-precision copy used with FP64 vectors (PFEM_PRECISION_MIXED_PRECOND): 32 or 16 bits per
+    // value (16: the split and sliced factor only, for the single-pass sweeps)
+    int factor_bits = 32;
+    pfem::DevArray<_Float16> split16, sell_l16, sell_u16;   // c L, c = scale16[0] (a power of two)
+    pfem::DevArray<double> scale16;       // c, c^2, and the largest |L| found (as bits)
     bool sell = false;
 };
 

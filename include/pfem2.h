@@ -170,6 +170,10 @@ int pfem_precond_set_sweep(pfem_precond pc, int single_pass);
  * 65536, about 0.1 s). A failed wait does not hang or fault: the launch completes, and the solve or apply
  * reports PFEM_ENUMERIC and re-arms the sweep buffers. 0 fails every wait that is not satisfied at once. */
 int pfem_precond_set_spin_limit(pfem_precond pc, int polls);
+/* Storage of the reduced-precision factor copy that PFEM_PRECISION_MIXED_PRECOND applies to FP64 vectors
+ * (no counterpart in the reference, whose factor is FP64): 32 (default) or 16 bits per value (IEEE half;
+ * multicolour factors with the single-pass sweeps only). Call before the solver's precision is set. */
+int pfem_precond_set_factor_bits(pfem_precond pc, int bits);
 
 /* ------------------------------------------------------------------ Krylov solvers (linear_solver.cuh) */
 #define PFEM_SOLVER_CG 0
@@ -337,6 +341,8 @@ typedef struct {
                                   PFEM_PRECISION_MIXED_PRECOND */
     double inner_tol;          /* MIXED: relative tolerance of each FP32 inner solve */
     int pres_ordering;         /* PFEM_ORDER_* for an IC0/ILU0 pressure preconditioner */
+    int pres_factor_bits;      /* MIXED with a multicolour IC0 pressure factor: bits of its reduced copy, 16 or
+                                  32 (0 means 32; pfem_precond_set_factor_bits) */
 } pfem_step_config;
 /* The time loop body of cases/Cylinder2D/main.cu:793-880 on a mesh with edge IDs set. Velocity BCs
  * share one node list with a value per component; pressure BCs have their own list. */

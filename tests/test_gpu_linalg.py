@@ -504,3 +504,54 @@ def test_solver_failure_paths(ctx, oracle):
     x.upload(np.full(xy.shape[0], 0.25))
     st = capi.Solver(a, capi.SOLVER_GMRES, 1e-30, 6, 0, 0, capi.Preconditioner(a)).solve(a, x, rhs)
     assert not st.converged and st.iterations == 6 and np.all(x.download() == 0.25)
+
+
+def test_half_factor(ctx, oracle):
+    """16-bit storage of the multicolour IC0 factor (PFEM_PRECISION_MIXED_PRECOND, factor bits 16): the
+    apply is that of c L rounded to half precision, c the power of two that puts the largest entry in
+    [2^13, 2^14), with c^2 b as the forward solve's input. Checked through the first PCG step, x_1 =
+    alpha M^-1 b, against the oracle's apply of the factor round16(c L) / c (the same direction to 1e-13);
+    then full solves converge like the 32-bit storage, also on the system scaled by 1e12 and by 1e-12,
+    whose factors would overflow or vanish in half precision without the scale."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    from gpupfem2_amd import capi
+    xy, m, a, rhs, rp, col, val = bc_system(ctx, oracle)
+    n = xy.shape[0]
+    b = rhs.download()
+
+    def solver(mat, bits, maxit, tol=1e-12):
+        pc = capi.Preconditioner(mat, capi.PRECOND_IC0, capi.ORDER_MULTICOLOR)
+        pc.set_factor_bits(bits)
+        s = capi.Solver(mat, capi.SOLVER_CG, tol, maxit, 0, capi.TOL_RELATIVE, pc)
+        s.set_precision(capi.PRECISION_MIXED_PRECOND, 1e-5, 20)
+        return pc, s
+
+    pc, s = solver(a, 16, 1, 1e-30)
+    x = ctx.zeros(n)
+    s.solve(a, x, rhs)
+    x1 = x.download()
+    fac, nl, perm = pc.download()
+    B = sp.csr_matrix((val, col, rp), shape=(n, n))[perm][:, perm].tocsr()
+    B.sort_indices()
+    c = 2.0 ** (14 - np.frexp(np.abs(fac).max())[1])
+    f16 = (fac * c).astype(np.float16).astype(np.float64) / c
+    assert not np.array_equal(f16, fac)
+    zf = oracle.ic0_apply(B.indptr.astype(np.int32), B.indices.astype(np.int32), f16, b[perm])
+    z = np.empty(n)
+    z[perm] = zf
+    assert rel_l2(x1 / np.linalg.norm(x1), z / np.linalg.norm(z)) < 1e-13
+    exact = spla.spsolve(sp.csr_matrix((val, col, rp)).tocsc(), b)
+    its = {}
+    for scale in (1.0, 1e12, 1e-12):
+        ms = capi.CSR(ctx, mesh=m)
+        ms.upload_values(np.ascontiguousarray(val * scale))
+        for bits in (32, 16):
+            mat = a if scale == 1.0 else ms
+            pc, s = solver(mat, bits, 5000)
+            x = ctx.zeros(n)
+            st = s.solve(mat, x, ctx.to_device(b * scale))
+            assert st.converged and not st.nan_detected, (scale, bits)
+            assert rel_l2(x.download(), exact) < 1e-9, (scale, bits)
+            its[(scale, bits)] = st.iterations
+    assert all(abs(v - its[(1.0, 32)]) <= 3 for v in its.values()), its
