@@ -116,6 +116,22 @@ int main(int argc, char* argv[])
         report(ok && d < 1e-6, ordering ? "ic0_multicolour" : "ic0_natural", cg.lastStats().iterations, d);
     }
     {
+        // the multicolour IC0 factor applied from its 16-bit copy, FP64 Krylov (PFEM_PRECISION_MIXED_PRECOND)
+        deviceVector<double> x;
+        x.allocate(n);
+        x.clearValues();
+        PreconditionerIC ic(a, &LA, PFEM_ORDER_MULTICOLOR);
+        ic.setFactorBits(a, 16);
+        SolverCG cg(1e-10, 5000, &LA, &ic);
+        cg.setPrecision(PFEM_PRECISION_MIXED_PRECOND);
+        cg.init(a);
+        const bool ok = cg.solve(a, x, rhs);
+        const auto hx = x.copyToHost();
+        double d = 0.0;
+        for (int i = 0; i < n; ++i) d = std::fmax(d, std::fabs(hx[i] - h0[i]));
+        report(ok && d < 1e-6, "ic0_half_factor", cg.lastStats().iterations, d);
+    }
+    {
         deviceVector<double> x;
         x.allocate(n);
         x.clearValues();

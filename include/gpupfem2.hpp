@@ -539,6 +539,12 @@ public:
     {
         handleFor(matrix);
     }
+    /* additive: storage of the factor copy applied in PFEM_PRECISION_MIXED_PRECOND (16 or 32 bits;
+     * pfem_precond_set_factor_bits); call before the solver's precision is set */
+    void setFactorBits(const SparseMatrixCSR& matrix, int bits)
+    {
+        gpupfem2::check(pfem_precond_set_factor_bits(handleFor(matrix), bits));
+    }
 };
 
 /* PreconditionerILU(matrix, LA) (preconditioners.cu:48-134) */
