@@ -72,18 +72,21 @@ __device__ __forceinline__ void group_partials(double (&v)[NV], double* __restri
     }
 }
 
-// Totals of NJ partial series of kMGrid terms each (series j at part + j * kMGrid), valid in thread 0.
-// Every load of a lane is issued before the first addition (one memory round trip); lane t adds terms
-// t, t + 256, ... of each series in order, then block_sum.
+// Totals of NJ partial series of nb <= kMGrid terms each (series j at part + j * nb: the grid of the
+// kernel that wrote them), valid in thread 0. Every load of a lane is issued before the first addition
+// (one memory round trip); lane t adds terms t, t + 256, ... of each series in order, then block_sum.
 template <int NJ>
-__device__ __forceinline__ void series_totals(const double* __restrict__ part, double (&tot)[NJ])
+__device__ __forceinline__ void series_totals(const double* __restrict__ part, int nb, double (&tot)[NJ])
 {
     constexpr int kPer = kMGrid / kBlock;
     double t[NJ][kPer];
 #pragma unroll
     for (int j = 0; j < NJ; ++j)
 #pragma unroll
-        for (int u = 0; u < kPer; ++u) t[j][u] = part[(size_t)j * kMGrid + threadIdx.x + u * kBlock];
+        for (int u = 0; u < kPer; ++u) {
+            const int k = threadIdx.x + u * kBlock;
+            t[j][u] = k < nb ? part[(size_t)j * nb + k] : 0.0;
+        }
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
         double s = 0.0;
@@ -232,10 +235,10 @@ __global__ __launch_bounds__(kBlock) void k_m_init(int n, int ntiles, const int*
 // gamma_0, the per-column thresholds (linear_solver.cu:360; relative to (b, M^-1 b) on request, absolute
 // for a zero column) and the columns already converged; one block
 template <int G>
-__global__ void k_m_begin(MState* st, const double* __restrict__ part)
+__global__ void k_m_begin(MState* st, const double* __restrict__ part, int nb)
 {
     double tot[2 * G];
-    series_totals<2 * G>(part, tot);
+    series_totals<2 * G>(part, nb, tot);
     if (threadIdx.x != 0) return;
     int all = 1;
     for (int j = 0; j < G; ++j) {
@@ -271,11 +274,11 @@ __global__ __launch_bounds__(kBlock) void k_m_spmv(int ntiles, const int* __rest
 
 // alpha_j = gamma_j / (p_j, q_j) for the active columns; one block
 template <int G>
-__global__ void k_m_alpha(MState* st, const double* __restrict__ part, int k)
+__global__ void k_m_alpha(MState* st, const double* __restrict__ part, int nb, int k)
 {
     if (st->all_done) return;
     double tot[G];
-    series_totals<G>(part, tot);
+    series_totals<G>(part, nb, tot);
     if (threadIdx.x != 0) return;
     for (int j = 0; j < G; ++j) st->alpha[j] = st->done[j] ? 0.0 : st->gam[k & 1][j] / tot[j];
 }
@@ -304,11 +307,11 @@ __global__ void k_m_update(int n, T* __restrict__ x, T* __restrict__ r, const T*
 
 // gamma_{k+1}, beta and the stopping test per active column; one block
 template <int G>
-__global__ void k_m_beta(MState* st, const double* __restrict__ part, int k)
+__global__ void k_m_beta(MState* st, const double* __restrict__ part, int nb, int k)
 {
     if (st->all_done) return;
     double tot[G];
-    series_totals<G>(part, tot);
+    series_totals<G>(part, nb, tot);
     if (threadIdx.x != 0) return;
     int all = 1;
     for (int j = 0; j < G; ++j) {
@@ -397,10 +400,10 @@ __global__ __launch_bounds__(kBlock) void k_mr_init(int ntiles, const int* __res
 }
 
 template <int G>
-__global__ void k_mr_begin(MState* st, const double* __restrict__ part)
+__global__ void k_mr_begin(MState* st, const double* __restrict__ part, int nb)
 {
     double tot[2 * G];
-    series_totals<2 * G>(part, tot);
+    series_totals<2 * G>(part, nb, tot);
     if (threadIdx.x != 0) return;
     int all = 1;
     for (int j = 0; j < G; ++j) {
@@ -494,11 +497,11 @@ __global__ __launch_bounds__(kBlock) void k_mr_spmv(int ntiles, const int* __res
 // alpha = gamma / (p_s, q_s) for iterating columns; for a replaced column the true gamma: the stopping
 // test, then beta for the direction update; one block
 template <int G>
-__global__ void k_mr_alpha(MState* st, const double* __restrict__ part)
+__global__ void k_mr_alpha(MState* st, const double* __restrict__ part, int nb)
 {
     if (st->all_done) return;
     double tot[G];
-    series_totals<G>(part, tot);
+    series_totals<G>(part, nb, tot);
     if (threadIdx.x != 0) return;
     int all = 1;
     for (int j = 0; j < G; ++j) {
@@ -552,11 +555,11 @@ __global__ void k_mr_update(int n, float* __restrict__ xs, float* __restrict__ r
 // gamma of the recursive residual: the iteration count, then either beta or a replacement (after a 10x
 // drop since the last one, or when the recursive residual passes the stopping test); one block
 template <int G>
-__global__ void k_mr_beta(MState* st, const double* __restrict__ part)
+__global__ void k_mr_beta(MState* st, const double* __restrict__ part, int nb)
 {
     if (st->all_done) return;
     double tot[G];
-    series_totals<G>(part, tot);
+    series_totals<G>(part, nb, tot);
     if (threadIdx.x != 0) return;
     int all = 1;
     for (int j = 0; j < G; ++j) {
@@ -636,6 +639,20 @@ struct MultiWork {
     }
 };
 
+// Grid of a row-tile kernel: as many workgroups as are resident at once (occupancy per CU times the
+// CUs), at most kMGrid. Each workgroup takes a contiguous range of tiles, so with the whole grid resident
+// from the start no workgroup waits for a second round (3D 8-RHS solve: FP64 298 -> 275 ms at 5
+// workgroups per CU, FP32 iterations 264 -> 205 ms at 7, against a fixed 2048).
+template <typename K>
+static int resident_grid(K kernel)
+{
+    int per_cu = 0, dev = 0, cus = 0;
+    PFEM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kernel), kBlock, 0));
+    PFEM_HIP(hipGetDevice(&dev));
+    PFEM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    return std::max(kBlock, std::min(kMGrid, per_cu * cus));
+}
+
 // One CG pass over the interleaved system (T = double: FP64 solve of b; T = float: inner FP32 solve from
 // x = 0). Ends with st copied to h[0].
 template <int G, typename T, typename VT>
@@ -654,20 +671,22 @@ static void m_cg_run(Context* c, pfem_solver_t* s, MultiWork& w, const Pattern& 
     double* pa = w.part.p;
     double* pb = w.part.p + (size_t)2 * G * kMGrid;
     const dim3 gr(kMGrid), bl(kBlock);
-    if (jac) hipLaunchKernelGGL((k_m_init<G, T, VT, true>), gr, bl, 0, c->stream, n, w.ntiles, w.tiles.p, P.rowptr.p, P.col.p, val, x, b, invd, r, p, pa, x_zero);
-    else hipLaunchKernelGGL((k_m_init<G, T, VT, false>), gr, bl, 0, c->stream, n, w.ntiles, w.tiles.p, P.rowptr.p, P.col.p, val, x, b, invd, r, p, pa, x_zero);
-    hipLaunchKernelGGL((k_m_begin<G>), dim3(1), bl, 0, c->stream, w.st.p, (const double*)pa);
+    const int ns = resident_grid(k_m_spmv<G, T, VT>);   // the tile kernels' grid
+    const dim3 gs(ns);
+    if (jac) hipLaunchKernelGGL((k_m_init<G, T, VT, true>), gs, bl, 0, c->stream, n, w.ntiles, w.tiles.p, P.rowptr.p, P.col.p, val, x, b, invd, r, p, pa, x_zero);
+    else hipLaunchKernelGGL((k_m_init<G, T, VT, false>), gs, bl, 0, c->stream, n, w.ntiles, w.tiles.p, P.rowptr.p, P.col.p, val, x, b, invd, r, p, pa, x_zero);
+    hipLaunchKernelGGL((k_m_begin<G>), dim3(1), bl, 0, c->stream, w.st.p, (const double*)pa, ns);
     PFEM_HIP(hipGetLastError());
     const int batch = s->poll;
     int launched = 0, pend = -1, pend_launched = 0, slot = 0;
     while (true) {
         for (int jb = 0; jb < batch; ++jb) {
             const int k = launched + jb;
-            hipLaunchKernelGGL((k_m_spmv<G, T, VT>), gr, bl, 0, c->stream, w.ntiles, w.tiles.p, P.rowptr.p, P.col.p, val, p, q, w.st.p, pa);
-            hipLaunchKernelGGL((k_m_alpha<G>), dim3(1), bl, 0, c->stream, w.st.p, (const double*)pa, k);
+            hipLaunchKernelGGL((k_m_spmv<G, T, VT>), gs, bl, 0, c->stream, w.ntiles, w.tiles.p, P.rowptr.p, P.col.p, val, p, q, w.st.p, pa);
+            hipLaunchKernelGGL((k_m_alpha<G>), dim3(1), bl, 0, c->stream, w.st.p, (const double*)pa, ns, k);
             if (jac) hipLaunchKernelGGL((k_m_update<G, T, true>), gr, bl, 0, c->stream, n, x, r, p, q, invd, w.st.p, pb);
             else hipLaunchKernelGGL((k_m_update<G, T, false>), gr, bl, 0, c->stream, n, x, r, p, q, invd, w.st.p, pb);
-            hipLaunchKernelGGL((k_m_beta<G>), dim3(1), bl, 0, c->stream, w.st.p, (const double*)pb, k);
+            hipLaunchKernelGGL((k_m_beta<G>), dim3(1), bl, 0, c->stream, w.st.p, (const double*)pb, kMGrid, k);
             if (jac) hipLaunchKernelGGL((k_m_p<G, T, true>), gr, bl, 0, c->stream, n, p, r, invd, w.st.p);
             else hipLaunchKernelGGL((k_m_p<G, T, false>), gr, bl, 0, c->stream, n, p, r, invd, w.st.p);
         }
@@ -712,9 +731,11 @@ static void m_ru_run(Context* c, pfem_solver_t* s, MultiWork& w, const Pattern& 
     double* pb = w.part.p + (size_t)2 * G * kMGrid;
     float *xs = w.x32.p, *rs = w.r32.p, *ps = w.p32.p, *qs = w.q32.p;
     const dim3 gr(kMGrid), bl(kBlock);
-    if (jac) hipLaunchKernelGGL((k_mr_init<G, true>), gr, bl, 0, c->stream, w.ntiles, w.tiles.p, P.rowptr.p, P.col.p, a->val.p, (const double*)w.x.p, (const double*)w.b.p, invd, invd32, rs, ps, xs, pa);
-    else hipLaunchKernelGGL((k_mr_init<G, false>), gr, bl, 0, c->stream, w.ntiles, w.tiles.p, P.rowptr.p, P.col.p, a->val.p, (const double*)w.x.p, (const double*)w.b.p, invd, invd32, rs, ps, xs, pa);
-    hipLaunchKernelGGL((k_mr_begin<G>), dim3(1), bl, 0, c->stream, w.st.p, (const double*)pa);
+    const int ns = jac ? resident_grid(k_mr_spmv<G, true>) : resident_grid(k_mr_spmv<G, false>);
+    const dim3 gs(ns);
+    if (jac) hipLaunchKernelGGL((k_mr_init<G, true>), gs, bl, 0, c->stream, w.ntiles, w.tiles.p, P.rowptr.p, P.col.p, a->val.p, (const double*)w.x.p, (const double*)w.b.p, invd, invd32, rs, ps, xs, pa);
+    else hipLaunchKernelGGL((k_mr_init<G, false>), gs, bl, 0, c->stream, w.ntiles, w.tiles.p, P.rowptr.p, P.col.p, a->val.p, (const double*)w.x.p, (const double*)w.b.p, invd, invd32, rs, ps, xs, pa);
+    hipLaunchKernelGGL((k_mr_begin<G>), dim3(1), bl, 0, c->stream, w.st.p, (const double*)pa, ns);
     PFEM_HIP(hipGetLastError());
     const int batch = s->poll;
     int launched = 0, pend = -1, slot = 0;
@@ -722,12 +743,12 @@ static void m_ru_run(Context* c, pfem_solver_t* s, MultiWork& w, const Pattern& 
     const int slots = s->maxit + s->maxit / 4 + 64;
     while (true) {
         for (int jb = 0; jb < batch; ++jb) {
-            if (jac) hipLaunchKernelGGL((k_mr_spmv<G, true>), gr, bl, 0, c->stream, w.ntiles, w.tiles.p, P.rowptr.p, P.col.p, (const float*)a->val32.p, (const double*)a->val.p, (const float*)ps, qs, (const double*)w.x.p, (const double*)w.b.p, invd, rs, (const MState*)w.st.p, pa);
-            else hipLaunchKernelGGL((k_mr_spmv<G, false>), gr, bl, 0, c->stream, w.ntiles, w.tiles.p, P.rowptr.p, P.col.p, (const float*)a->val32.p, (const double*)a->val.p, (const float*)ps, qs, (const double*)w.x.p, (const double*)w.b.p, invd, rs, (const MState*)w.st.p, pa);
-            hipLaunchKernelGGL((k_mr_alpha<G>), dim3(1), bl, 0, c->stream, w.st.p, (const double*)pa);
+            if (jac) hipLaunchKernelGGL((k_mr_spmv<G, true>), gs, bl, 0, c->stream, w.ntiles, w.tiles.p, P.rowptr.p, P.col.p, (const float*)a->val32.p, (const double*)a->val.p, (const float*)ps, qs, (const double*)w.x.p, (const double*)w.b.p, invd, rs, (const MState*)w.st.p, pa);
+            else hipLaunchKernelGGL((k_mr_spmv<G, false>), gs, bl, 0, c->stream, w.ntiles, w.tiles.p, P.rowptr.p, P.col.p, (const float*)a->val32.p, (const double*)a->val.p, (const float*)ps, qs, (const double*)w.x.p, (const double*)w.b.p, invd, rs, (const MState*)w.st.p, pa);
+            hipLaunchKernelGGL((k_mr_alpha<G>), dim3(1), bl, 0, c->stream, w.st.p, (const double*)pa, ns);
             if (jac) hipLaunchKernelGGL((k_mr_update<G, true>), gr, bl, 0, c->stream, n, xs, rs, (const float*)ps, (const float*)qs, invd32, (const MState*)w.st.p, pb);
             else hipLaunchKernelGGL((k_mr_update<G, false>), gr, bl, 0, c->stream, n, xs, rs, (const float*)ps, (const float*)qs, invd32, (const MState*)w.st.p, pb);
-            hipLaunchKernelGGL((k_mr_beta<G>), dim3(1), bl, 0, c->stream, w.st.p, (const double*)pb);
+            hipLaunchKernelGGL((k_mr_beta<G>), dim3(1), bl, 0, c->stream, w.st.p, (const double*)pb, kMGrid);
             if (jac) hipLaunchKernelGGL((k_mr_p<G, true>), gr, bl, 0, c->stream, n, ps, (const float*)rs, invd32, w.x.p, xs, (const MState*)w.st.p);
             else hipLaunchKernelGGL((k_mr_p<G, false>), gr, bl, 0, c->stream, n, ps, (const float*)rs, invd32, w.x.p, xs, (const MState*)w.st.p);
         }
