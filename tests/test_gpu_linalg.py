@@ -555,3 +555,12 @@ def test_half_factor(ctx, oracle):
             assert rel_l2(x.download(), exact) < 1e-9, (scale, bits)
             its[(scale, bits)] = st.iterations
     assert all(abs(v - its[(1.0, 32)]) <= 3 for v in its.values()), its
+    # 16 bits are for IC0 factors, and are chosen before the reduced-precision copy exists
+    for kind in (capi.PRECOND_JACOBI, capi.PRECOND_ILU0):
+        with pytest.raises(capi.PfemError):
+            capi.Preconditioner(a, kind, capi.ORDER_MULTICOLOR).set_factor_bits(16)
+    pc, s = solver(a, 32, 10)
+    with pytest.raises(capi.PfemError):
+        pc.set_factor_bits(16)
+    with pytest.raises(capi.PfemError):
+        capi.Preconditioner(a, capi.PRECOND_IC0, capi.ORDER_MULTICOLOR).set_factor_bits(8)
