@@ -149,3 +149,46 @@ def test_step_mixed_precision(ctx, oracle, ordering):
     for k in ("u0", "u1", "p"):
         assert rel_l2(g[k], o[k]) < 1e-8, (k, rel_l2(g[k], o[k]))
     assert np.allclose(fg, fo, rtol=1e-6, atol=1e-9)
+
+
+def test_particles_full_size(ctx, oracle):
+    """The particle kernels at the 2M-triangle cylinder's size (40.4M particles, 20 per cell, 16
+    subcells): seed, velocity initialisation, one advect cycle (3 substeps with relocation, deletion at
+    the outlet, refill and reordering), projection and correction, bit for bit against the oracle. The
+    oracle walks the library's neighbour lists (pinned against brute force on small meshes)."""
+    from gpupfem2_amd import capi, cases
+    case = cases.cylinder_case()
+    xy, tri = case.xy, case.tri
+    nv = xy.shape[0]
+    print("particles full size", tri.shape[0], "triangles", flush=True)
+    m = capi.Mesh2D(ctx, xy, tri)
+    ph = capi.Particles(m, case.level)
+    ph.seed(case.seed_mode, case.ppc, case.seed)
+    area, invJ = oracle.geometry2d(xy, tri)
+    oph = oracle.Particles(xy, tri, invJ, m.download(4), m.download(5), case.level)
+    oph.seed(case.seed_mode, case.ppc, case.seed)
+
+    def same():
+        g, o = ph.get(), oph.get()
+        for k in ("cell", "id", "lam", "pos", "vel"):
+            assert np.array_equal(g[k], o[k]), k
+
+    rng = np.random.default_rng(11)
+    # about a cell per substep near the cylinder, and particles leaving through the outlet (x = 20)
+    u0 = 1.5 + 0.5 * rng.standard_normal(nv)
+    u1 = 0.5 * rng.standard_normal(nv)
+    du0, du1 = ctx.to_device(u0), ctx.to_device(u1)
+    ph.init_velocity(du0, du1)
+    oph.init_velocity(u0, u1)
+    ph.advect(du0, du1, 0.01, 3)
+    oph.advect(u0, u1, 0.01, 3)
+    assert ph.stats() == oph.stats()
+    assert oph.stats()[0] > 0 and oph.stats()[1] > 0   # deletions and refills happened
+    same()
+    p0, p1 = ctx.empty(nv), ctx.empty(nv)
+    ph.project(p0, p1)
+    q0, q1 = oph.project(nv)
+    assert np.array_equal(p0.download(), q0) and np.array_equal(p1.download(), q1)
+    ph.correct(p0, p1, du0, du1)
+    oph.correct(q0, q1, u0, u1)
+    same()
