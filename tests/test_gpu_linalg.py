@@ -564,3 +564,41 @@ def test_half_factor(ctx, oracle):
         pc.set_factor_bits(16)
     with pytest.raises(capi.PfemError):
         capi.Preconditioner(a, capi.PRECOND_IC0, capi.ORDER_MULTICOLOR).set_factor_bits(8)
+
+
+def neighbors_by_incidence(nv, tri, self_included):
+    """Cells sharing at least one vertex, ascending, from the pattern of C C^T (C the cell-vertex
+    incidence): an independent construction for meshes too large for the brute-force oracle."""
+    import scipy.sparse as sp
+    nt = tri.shape[0]
+    C = sp.csr_matrix((np.ones(3 * nt, np.int8), tri.ravel(), np.arange(0, 3 * nt + 1, 3)), shape=(nt, nv))
+    N = (C.astype(np.int32) @ C.T.astype(np.int32)).tocsr()
+    if not self_included:
+        N.setdiag(0)
+        N.eliminate_zeros()
+    N.sort_indices()
+    return N.indptr.astype(np.int64), N.indices
+
+
+def test_mesh_setup_full_size(ctx, oracle):
+    """Geometry, cell neighbours and the CSR pattern of the 2M-triangle cylinder mesh: geometry and CSR
+    bit for bit against the oracle, neighbours against the pattern of C C^T (the construction is first
+    checked against the brute-force oracle on a fixture)."""
+    from gpupfem2_amd import capi, cases
+    v, c, ndev = oracle.load_dat(golden("CylinderMesh.dat"), 2)
+    off, idx = oracle.neighbors_bruteforce(c)
+    selfin = any(i in idx[off[i]:off[i + 1]] for i in range(10))
+    o2, i2 = neighbors_by_incidence(v.shape[0], c, selfin)
+    assert np.array_equal(o2, off) and np.array_equal(i2, idx)
+    case = cases.cylinder_case()
+    xy, tri = case.xy, case.tri
+    print("mesh setup full size", tri.shape[0], "triangles", flush=True)
+    m = capi.Mesh2D(ctx, xy, tri)
+    area, invJ = oracle.geometry2d(xy, tri)
+    assert np.array_equal(m.download(2), area)
+    assert np.array_equal(m.download(3).reshape(-1, 4), invJ)
+    on, ni = neighbors_by_incidence(xy.shape[0], tri, selfin)
+    assert np.array_equal(m.download(4), on) and np.array_equal(m.download(5), ni)
+    rp, col = oracle.csr_pattern(xy.shape[0], tri)
+    grp, gcol, _ = capi.CSR(ctx, mesh=m).download()
+    assert np.array_equal(grp, rp) and np.array_equal(gcol, col)
