@@ -98,9 +98,13 @@ def test_poisson2d_assembly(ctx, oracle, rhs_kind):
         assert rel_l2(rhs.download(), r) < 1e-15
 
 
-def test_poisson3d_assembly(ctx, oracle):
+@pytest.mark.parametrize("n", [6, 128])
+def test_poisson3d_assembly(ctx, oracle, n):
+    """Geometry, CSR pattern and the P1 Poisson system of the box of n^3 cubes (6 tetrahedra each; n = 128
+    is the 12.6M-tetrahedron, 2.15M-node configuration) bit for bit against the oracle."""
     from gpupfem2_amd import capi, meshgen
-    xyz, tet = meshgen.box3d(6)
+    print("poisson3d assembly", n, flush=True)
+    xyz, tet = meshgen.box3d(n)
     m = capi.Mesh3D(ctx, xyz, tet)
     a = capi.CSR(ctx, mesh=m)
     rhs = ctx.zeros(xyz.shape[0])
@@ -108,7 +112,10 @@ def test_poisson3d_assembly(ctx, oracle):
     vol, invJ = oracle.geometry3d(xyz, tet)
     rp, col = oracle.csr_pattern(xyz.shape[0], tet)
     val, r = oracle.assemble_poisson3d(xyz, tet, vol, invJ, rp, col, 1.0, 2, 3.0, 0)
-    assert np.array_equal(a.download()[2], val) and np.array_equal(rhs.download(), r)
+    assert np.array_equal(m.download(2), vol) and np.array_equal(m.download(3).reshape(-1, 9), invJ)
+    grp, gcol, gval = a.download()
+    assert np.array_equal(grp, rp) and np.array_equal(gcol, col)
+    assert np.array_equal(gval, val) and np.array_equal(rhs.download(), r)
 
 
 def test_spmv_bitexact(ctx, oracle):
