@@ -19,6 +19,7 @@ SOLVER_CG, SOLVER_GMRES = 0, 1
 TOL_ABSOLUTE, TOL_RELATIVE = 0, 1
 PRECISION_FP64, PRECISION_MIXED, PRECISION_MIXED_PRECOND = 0, 1, 2
 SOLVER_OPT_FACTOR_ORDER, SOLVER_OPT_POLL, SOLVER_OPT_FUSE, SOLVER_OPT_REPLACE, SOLVER_OPT_GRAPH = 1, 2, 3, 4, 5
+SOLVER_OPT_GS_PASSES = 6
 
 
 class SolveStats(C.Structure):
@@ -469,7 +470,7 @@ class Solver(_Handle):
 
     def set_option(self, option, value):
         """SOLVER_OPT_FACTOR_ORDER (0/1), SOLVER_OPT_POLL (iterations between convergence polls) or
-        SOLVER_OPT_FUSE (0/1/2/3, default 3), SOLVER_OPT_REPLACE or SOLVER_OPT_GRAPH (0/1, default 0)."""
+        SOLVER_OPT_FUSE (0/1/2/3, default 3), SOLVER_OPT_REPLACE, SOLVER_OPT_GRAPH (0/1, default 0) or SOLVER_OPT_GS_PASSES (GMRES: 1/2, default 2)."""
         self.ctx.check(lib().pfem_solver_set_option(self.h, option, int(value)))
 
     def set_precision(self, precision, inner_tol=0.0, max_refinements=0):

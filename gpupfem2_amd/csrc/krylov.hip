@@ -1211,6 +1211,10 @@ int pfem_solver_set_option(pfem_solver s, int option, int value)
             s->replace_digits = value;
             break;
         case PFEM_SOLVER_OPT_GRAPH: s->graph = value ? 1 : 0; break;
+        case PFEM_SOLVER_OPT_GS_PASSES:
+            PFEM_REQUIRE(value == 1 || value == 2, "Gram-Schmidt passes must be 1 or 2");
+            s->reorth = value == 2 ? 1 : 0;
+            break;
         case PFEM_SOLVER_OPT_POLL:
             PFEM_REQUIRE(value >= 1 && value <= 4096, "poll interval out of range");
             s->poll = value;

@@ -134,6 +134,10 @@ int pfem_pfem2_create(pfem_mesh2d m, pfem_particles ph, const pfem_step_config* 
         check(pfem_dirichlet_create(s->ctx, s->n, nbcu, bcu_node, bcu_val1, &s->bcpred[1]), s->ctx);
         check(pfem_dirichlet_create(s->ctx, s->n, nbcp, bcp_node, bcp_val, &s->bcp), s->ctx);
         make_solver(s.get(), s->A0, cfg->pred_solver, cfg->pred_precond, PFEM_ORDER_NATURAL, &s->pc_pred, &s->s_pred);
+        // the prediction systems are mass-dominated (rho M + mu dt K): one classical Gram-Schmidt pass per
+        // Arnoldi step keeps their iteration counts (cylinder, 2M triangles: prediction 9.0 -> 5.9 ms per
+        // step in FP64)
+        if (cfg->pred_solver == PFEM_SOLVER_GMRES) check(pfem_solver_set_option(s->s_pred, PFEM_SOLVER_OPT_GS_PASSES, 1), s->ctx);
         PFEM_REQUIRE(cfg->pres_factor_bits == 0 || cfg->pres_factor_bits == 16 || cfg->pres_factor_bits == 32,
                      "pres_factor_bits must be 0, 16 or 32");
         make_solver(s.get(), s->Ap, cfg->pres_solver, cfg->pres_precond, cfg->pres_ordering, &s->pc_pres, &s->s_pres,

@@ -207,6 +207,9 @@ int pfem_solver_set_precision(pfem_solver s, int precision, double inner_tol, in
 #define PFEM_SOLVER_OPT_POLL 2
 #define PFEM_SOLVER_OPT_REPLACE 4 /* multi-RHS / batched PFEM_PRECISION_MIXED, default 2: a residual replacement
                                     follows every fall of (r, M^-1 r) by 10^-value (value 1..8) */
+#define PFEM_SOLVER_OPT_GS_PASSES 6 /* GMRES Arnoldi: classical Gram-Schmidt passes, 2 (default: one
+                                       re-orthogonalisation) or 1 (for well-conditioned systems; the
+                                       reference's modified Gram-Schmidt lies between the two) */
 #define PFEM_SOLVER_OPT_GRAPH 5  /* fused PCG (fusion level 2 or more), default 0: 1 makes each solve capture one
                                     batch of iterations (the poll interval) as a HIP graph and replay it; 0
                                     launches the kernels one by one (the same bits either way) */

@@ -318,8 +318,10 @@ def test_pcg(ctx, oracle, precond, ordering, tol_mode):
     assert rel_l2(x.download(), exact) < 1e-7
 
 
-@pytest.mark.parametrize("restart", [30, 3000])
-def test_gmres(ctx, oracle, restart):
+@pytest.mark.parametrize("restart,passes", [(30, 2), (3000, 2), (30, 1)])
+def test_gmres(ctx, oracle, restart, passes):
+    """GMRES (classical Gram-Schmidt, passes = 2: one re-orthogonalisation, or a single pass) against the
+    oracle's modified Gram-Schmidt GMRES on a nonsymmetric system."""
     from gpupfem2_amd import capi
     xy, m, a, rhs, rp, col, val = bc_system(ctx, oracle, n=24)
     n = rp.size - 1
@@ -331,6 +333,7 @@ def test_gmres(ctx, oracle, restart):
     db = ctx.to_device(b)
     pc = capi.Preconditioner(a, capi.PRECOND_JACOBI)
     s = capi.Solver(a, capi.SOLVER_GMRES, 1e-12, 3000, restart, capi.TOL_RELATIVE, pc)
+    s.set_option(capi.SOLVER_OPT_GS_PASSES, passes)
     x = ctx.zeros(n)
     st = s.solve(a, x, db)
     assert st.converged
