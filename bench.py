@@ -119,6 +119,35 @@ def roofline_spmv(ctx, st):
                 frac=round(gbs / HBM_PEAK_GBS, 4), avg_us=round(ms * 1e3, 2), bytes_per_launch=b)
 
 
+def natural_over_multicolour(ctx):
+    """Iterations of IC0-PCG (oracle, relative 1e-12, random right-hand side) with the natural-order
+    factor over those with the library's multicolour factor, on the cylinder channel's pressure matrix at
+    h = 1/28 (65k nodes): the CPU runs the reference's natural ordering, which converges faster."""
+    import numpy as np
+    import scipy.sparse as sp
+    from gpupfem2_amd import capi, cases
+    from oracle import oracle as O
+    sc = cases.cylinder_case(h=1.0 / 28)
+    nv = sc.xy.shape[0]
+    area, invJ = O.geometry2d(sc.xy, sc.tri)
+    rp, col = O.csr_pattern(nv, sc.tri)
+    val, _ = O.assemble_poisson2d(sc.xy, sc.tri, area, invJ, rp, col, 1.0, 2, 0.0, 1)
+    b = np.random.default_rng(0).standard_normal(nv)
+    O.dirichlet_apply(rp, col, val, b, sc.bcp_node, sc.bcp_val, sc.dirichlet_mode)
+    nat = O.pcg(rp, col, val, np.zeros(nv), b, 1e-12, 20000, 1, "ic0").iterations
+    m = capi.Mesh2D(ctx, sc.xy, sc.tri)
+    a = capi.CSR(ctx, mesh=m)
+    a.upload_values(val)
+    pc = capi.Preconditioner(a, capi.PRECOND_IC0, capi.ORDER_MULTICOLOR)
+    pc.initialize(a)
+    _, _, perm = pc.download()
+    B = sp.csr_matrix((val, col, rp), shape=(nv, nv))[perm][:, perm].tocsr()
+    B.sort_indices()
+    mc = O.pcg(B.indptr.astype(np.int32), B.indices.astype(np.int32), B.data.copy(), np.zeros(nv),
+               np.ascontiguousarray(b[perm]), 1e-12, 20000, 1, "ic0").iterations
+    return nat / mc
+
+
 def cpu_baseline(case, st, mesh, gpu_its):
     """The oracle (1 thread, FP64) on the same full-size inputs: particle step, the 5 assemblies, and
     per-iteration Krylov costs measured with capped iteration counts, combined with the iteration
@@ -167,8 +196,9 @@ def cpu_baseline(case, st, mesh, gpu_its):
     corr_it, corr_setup = per_iter(lambda k: O.pcg(rp, col, vc, x * 0, c0, 1e-300, k, 1, "jacobi"), 10, 60)
     pred_it, pred_setup = per_iter(lambda k: O.gmres(rp, col, v0, x * 0, r0, 1e-300, k, 30, 1, "jacobi"), 10, 60)
     its = np.asarray(gpu_its[-1]).reshape(case.outer_iterations, 5)
+    ratio = natural_over_multicolour(mesh.ctx)
     t_solve = (its[:, 0:2].sum() * pred_it + 2 * case.outer_iterations * pred_setup +
-               its[:, 2].sum() * pres_it + case.outer_iterations * pres_setup +
+               ratio * its[:, 2].sum() * pres_it + case.outer_iterations * pres_setup +
                its[:, 3:5].sum() * corr_it + 2 * case.outer_iterations * corr_setup)
     t_step = t_part + case.outer_iterations * (2 * t_pred_asm + t_pres_asm + t_corr_asm) + t_solve
     return dict(value=round(1.0 / t_step, 6), unit="steps/s", cores=O.threads(), kind="port",
@@ -176,7 +206,9 @@ def cpu_baseline(case, st, mesh, gpu_its):
                         f"advect+project+correct ({t_part:.1f} s), one assembly of each system, and 50 "
                         f"Krylov iterations per solver type (IC0-PCG {pres_it * 1e3:.1f} ms/it, Jacobi-PCG "
                         f"{corr_it * 1e3:.1f} ms/it, GMRES(30) {pred_it * 1e3:.1f} ms/it), combined with the "
-                        f"GPU FP64 step's iteration counts; measured in {time.perf_counter() - T0:.0f} s"),
+                        f"GPU FP64 step's iteration counts, the pressure's scaled by {ratio:.3f} (the natural-order "
+                        f"IC0 of the reference against the GPU's multicolour factor, measured on the same channel "
+                        f"at h = 1/28); measured in {time.perf_counter() - T0:.0f} s"),
                 seconds_per_step=round(t_step, 2))
 
 
