@@ -612,3 +612,40 @@ def test_mesh_setup_full_size(ctx, oracle):
     rp, col = oracle.csr_pattern(xy.shape[0], tri)
     grp, gcol, _ = capi.CSR(ctx, mesh=m).download()
     assert np.array_equal(grp, rp) and np.array_equal(gcol, col)
+
+
+@pytest.mark.parametrize("bits", [64, 16])
+def test_pcg_full_size_repeatable(ctx, bits):
+    """The cylinder's pressure system (1.01M rows) solved by the fused multicolour IC0-PCG three times
+    over (FP64 factor, and the mixed step's 16-bit copy): identical iteration counts and bits each
+    time, whatever order the workgroups' publications happen to arrive in, and a true residual at the
+    requested tolerance."""
+    from gpupfem2_amd import capi, cases
+    case = cases.cylinder_case()
+    print("pcg full size repeatable", bits, flush=True)
+    m = capi.Mesh2D(ctx, case.xy, case.tri, fill_neighbors=False)
+    a = capi.CSR(ctx, mesh=m)
+    n = case.xy.shape[0]
+    rhs = ctx.zeros(n)
+    capi.assemble_poisson2d(m, a, rhs, coeff=1.0, rhs_kind=2, rhs_param=1.0)
+    capi.Dirichlet(ctx, n, case.bcp_node, case.bcp_val).apply(a, rhs, capi.DIRICHLET_MASK)
+    pc = capi.Preconditioner(a, capi.PRECOND_IC0, capi.ORDER_MULTICOLOR)
+    if bits == 16:
+        pc.set_factor_bits(16)
+    s = capi.Solver(a, capi.SOLVER_CG, 1e-10, 20000, 0, capi.TOL_RELATIVE, pc)
+    if bits == 16:
+        s.set_precision(capi.PRECISION_MIXED_PRECOND, 1e-5, 20)
+    ref = None
+    for rep in range(3):
+        x = ctx.zeros(n)
+        st = s.solve(a, x, rhs)
+        assert st.converged and not st.nan_detected
+        got = (st.iterations, x.download())
+        if ref is None:
+            ref = got
+            y = ctx.empty(n)
+            a.spmv(x, y)
+            b = rhs.download()
+            assert np.linalg.norm(b - y.download()) / np.linalg.norm(b) < 1e-7
+        else:
+            assert got[0] == ref[0] and np.array_equal(got[1], ref[1]), rep
