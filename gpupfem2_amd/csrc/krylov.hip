@@ -749,10 +749,16 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
         (void)hipGraphDestroy(graph);
         PFEM_HIP(e);
     }
-    struct ExecGuard {
+    struct ExecGuard {   // the replays queued on the stream finish before the executable graph goes
         hipGraphExec_t& e;
-        ~ExecGuard() { if (e) (void)hipGraphExecDestroy(e); }
-    } exec_guard{exec};
+        hipStream_t st;
+        ~ExecGuard()
+        {
+            if (!e) return;
+            (void)hipStreamSynchronize(st);
+            (void)hipGraphExecDestroy(e);
+        }
+    } exec_guard{exec, c->stream};
     // the state of batch b is copied to slot b & 1 and checked after batch b + 1 is queued, so the stream
     // never drains while the host waits; a converged solve runs at most one batch of no-op launches
     int pend = -1, pend_launched = 0, slot = 0;
