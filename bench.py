@@ -107,6 +107,47 @@ def roofline_ic0(ctx, st, precision):
                 avg_us=round(ms * 1e3, 2), bytes_per_launch=b)
 
 
+def pcg_iteration_bytes(n, nnz, factor_bytes):
+    """Algorithmic bytes of one fused pressure PCG iteration: the SpMV (A, p gathered, q written), the
+    forward sweep (the lower triangle with its diagonal; r read and written, q read, y written), the
+    backward sweep (the upper triangle with the diagonal; y read, z written, r read for (r, z)) and the
+    x/p update (x read and written, p read and written, z read)."""
+    tri = (nnz - n) // 2 * (factor_bytes + 4) + n * factor_bytes
+    return spmv_bytes(n, nnz, 8, 8) + (tri + 4 * 8 * n) + (tri + 3 * 8 * n) + 5 * 8 * n
+
+
+def roofline_pcg(ctx, st, precision, tol=1e-9):
+    """One pressure solve from x = 0 (relative tolerance 1e-9, a seeded right-hand side) with the step's
+    matrix and preconditioner, as the step runs it: time per iteration (the solve's wall time, which also
+    holds one factorization, over its iterations) against the algorithmic bytes of an iteration."""
+    import types
+    import numpy as np
+    from gpupfem2_amd import capi
+    a = types.SimpleNamespace(h=st.matrix_handle(2), ctx=ctx)
+    pc = types.SimpleNamespace(h=st.precond_handle(1))
+    n, nnz = csr_info(a.h)
+    s = capi.Solver(a, capi.SOLVER_CG, tol, 20000, 0, capi.TOL_RELATIVE, pc)
+    if precision == 1:
+        s.set_precision(capi.PRECISION_MIXED_PRECOND, 1e-5, 0)
+    b = ctx.to_device(np.random.default_rng(1).standard_normal(n))
+    best = None
+    for _ in range(2):
+        x = ctx.zeros(n)
+        ctx.sync()
+        t = time.perf_counter()
+        stt = s.solve(a, x, b)
+        ctx.sync()
+        dt = time.perf_counter() - t
+        best = dt if best is None else min(best, dt)
+    us = best / stt.iterations * 1e6
+    b_it = pcg_iteration_bytes(n, nnz, 2 if precision == 1 else 8)
+    gbs = b_it / (us * 1e-6) / 1e9
+    del s
+    return dict(iterations=stt.iterations, us_per_iteration=round(us, 2), bytes_per_iteration=b_it,
+                achieved=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 4),
+                kernels="k_cg_spmv_alpha_sliced + k_sweep_lower + k_sweep_upper + k_cg_xp_recorded")
+
+
 def roofline_spmv(ctx, st):
     """FP64 SpMV of the pressure matrix (the SpMV of every pressure PCG iteration)."""
     from gpupfem2_amd import capi
@@ -404,6 +445,7 @@ def main():
     res = {}
     roof = roof_spmv = None
     cpu = None
+    roof_pcg = {}
     for prec in ([0, 1] if args.mode == "both" else [0] if args.mode == "fp64" else [1]):
         print(f"[bench] precision {'mixed' if prec else 'fp64'}: {case.tri.shape[0]} triangles, "
               f"{args.warmup} warm-up + {args.steps} timed steps", file=sys.stderr, flush=True)
@@ -411,6 +453,9 @@ def main():
         res[prec] = r
         print(f"[bench] {r['steps_per_s']:.3f} steps/s, iterations {r['iterations'][-1]}", file=sys.stderr,
               flush=True)
+        roof_pcg["mixed" if prec else "fp64"] = roofline_pcg(ctx, st, prec)
+        print(f"[bench] pressure PCG iteration {roof_pcg['mixed' if prec else 'fp64']['us_per_iteration']} us",
+              file=sys.stderr, flush=True)
         if prec == (1 if args.mode != "fp64" else 0):
             roof = roofline_ic0(ctx, st, prec)
             roof_spmv = roofline_spmv(ctx, st)
@@ -445,6 +490,7 @@ def main():
         "cd_cl_last_step": {("mixed" if k else "fp64"): v["cd_cl"] for k, v in res.items()},
         "roofline": roof,
         "roofline_spmv": roof_spmv,
+        "roofline_pcg_iteration": roof_pcg,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
