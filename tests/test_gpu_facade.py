@@ -98,3 +98,50 @@ def test_cylinder2d_driver(ctx, tmp_path, oracle):
     ref = np.array([stepper.step() for _ in range(steps)])
     # Forces.csv holds 6 significant digits, as the reference writes it
     assert np.allclose(got[:, 1:], ref, rtol=1e-5, atol=1e-12), (got, ref)
+
+
+def vtu_field(path, name):
+    import re
+    text = open(path).read()
+    m = re.search(r'<DataArray type="Float32" Name="%s" Format="ascii">(.*?)</DataArray>' % name, text, re.S)
+    return np.array(m.group(1).split(), dtype=np.float64)
+
+
+def test_poiseuille2d_driver(ctx, tmp_path, oracle):
+    """examples/poiseuille2d.cpp, the PoiseuilleFlow2D case driver (ChannelMesh.dat, pressure-driven: p = 10
+    at x = -5 and 0 at x = 5, no slip on the walls; Scheme A, dt 0.01, level-2 particles, traction on every
+    boundary edge), run for 3 steps: the exported fields equal, bit for bit, those of the same case built
+    and stepped from Python."""
+    from gpupfem2_amd import cases
+    steps = 3
+    rc, out = run([os.path.join(EX, "poiseuille2d"), golden("ChannelMesh.dat"), str(steps), str(tmp_path), "3"],
+                  tmp_path)
+    print(out[-1500:])
+    assert rc == 0, out[-1500:]
+    v, c, ndev = oracle.load_dat(golden("ChannelMesh.dat"), 2)
+    ids = cases.classify_edges(v, c, -5.0, 5.0, 0.0, 1.0, ids=(0, 1, 2, 3))
+    bu, bp, bpv = [], [], []
+    for i, (x, y) in enumerate(v):       # the pressure-driven option, in the reference's if/else order
+        if abs(x + 5.0) < cases.TOL_BC:
+            bp.append(i); bpv.append(10.0)
+        elif abs(x - 5.0) < cases.TOL_BC:
+            bp.append(i); bpv.append(0.0)
+        elif abs(y) < cases.TOL_BC or abs(y - 1.0) < cases.TOL_BC:
+            bu.append(i)
+    bu, bp = np.asarray(bu, np.int32), np.asarray(bp, np.int32)
+    z = np.zeros(bu.size)
+    case = cases.Pfem2Case(
+        name="PoiseuilleFlow2D", xy=v, tri=c, edge_ids=ids, bcu_node=bu, bcu_val0=z, bcu_val1=z, bcp_node=bp,
+        bcp_val=np.asarray(bpv), rho=1.0, mu=1.0, dt=0.01, scheme=0, traction_mask=0b1111,
+        normals=[(-1.0, 0.0), (1.0, 0.0), (0.0, -1.0), (0.0, 1.0)], outer_iterations=1, substeps=3,
+        pred_solver=cases.GMRES, pred_precond=cases.JACOBI, pres_solver=cases.CG, pres_precond=cases.JACOBI,
+        corr_solver=cases.CG, corr_precond=cases.JACOBI, tol=1e-8, tol_mode=cases.ABS, maxit=1000, restart=0,
+        dirichlet_mode=cases.PARITY, level=2, seed_mode=0, ppc=0, seed=0)
+    mesh, ph, stepper = cases.build_gpu(ctx, case, precision=0, pres_ordering=0)
+    for _ in range(steps):
+        stepper.step()
+    f = stepper.fields()
+    for name, key in (("velX", "u0"), ("velY", "u1"), ("pressure", "p")):
+        # the exporter writes 17 significant digits, which round-trip a double
+        assert np.array_equal(vtu_field(tmp_path / f"solution{steps}.vtu", name), f[key]), name
+    assert np.abs(f["u0"]).max() > 0.0
