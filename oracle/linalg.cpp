@@ -273,12 +273,14 @@ int orc_pcg(int n, const int* rowptr, const int* col, const double* val, double*
 // SolverGMRES::solve (linear_solver.cu:421-521): left preconditioning, modified Gram-Schmidt,
 // Givens rotations (updateHcolumn :40-66), stop when |beta_{k+1}| < tol (tol_mode 0) or
 // < tol * ||M^-1 b|| (tol_mode 1). restart >= maxit reproduces the reference (no restart, x untouched
-// unless converged); restart < maxit is GMRES(m): x is updated at the end of every cycle.
+// unless converged), and so does restart <= 0 (the library's code for it); restart < maxit is
+// GMRES(m): x is updated at the end of every cycle.
 // GivensRotation uses 1/sqrt (the reference's rsqrt rounding is not reproducible).
 int orc_gmres(int n, const int* rowptr, const int* col, const double* val, double* x, const double* b,
               double tol, int maxit, int restart, int tol_mode, int precond, orc_solve_stats* st)
 {
     if (maxit > n) maxit = n;
+    if (restart <= 0) restart = maxit;
     const int m = restart < maxit ? restart : maxit;
     const bool restarted = restart < maxit;
     Precond M;

@@ -68,9 +68,9 @@ def test_cylinder2d_driver(ctx, tmp_path, oracle):
     particles, Jacobi CG and non-restarted GMRES at the reference's absolute tolerance 1e-8), run for 4
     steps: Forces.csv in the reference's "Time;Cx;Cy" format, VTU snapshots, and the loads of every step
     equal to those of the same case built in Python (edge IDs and Dirichlet lists from the reference's
-    rules in gpupfem2_amd.cases) and stepped through the Python binding. The step itself is held to the
-    oracle on smaller meshes (test_gpu_pfem2.py); the oracle's non-restarted GMRES is too slow here."""
+    rules in gpupfem2_amd.cases) and stepped through the Python binding, and to the oracle's loads."""
     from gpupfem2_amd import cases
+    from helpers import oracle_stepper
     steps = 4
     rc, out = run([os.path.join(EX, "cylinder2d"), golden("CylinderMesh3.dat"), str(steps), str(tmp_path), "2"],
                   tmp_path)
@@ -98,6 +98,10 @@ def test_cylinder2d_driver(ctx, tmp_path, oracle):
     ref = np.array([stepper.step() for _ in range(steps)])
     # Forces.csv holds 6 significant digits, as the reference writes it
     assert np.allclose(got[:, 1:], ref, rtol=1e-5, atol=1e-12), (got, ref)
+    _, ost = oracle_stepper(oracle, case)
+    orc = np.array([ost.step() for _ in range(steps)])
+    # two reduction orders under the absolute solver tolerance 1e-8 leave load differences near 1e-6
+    assert np.allclose(ref, orc, rtol=1e-5, atol=1e-8), (ref, orc)
 
 
 def vtu_field(path, name):
@@ -145,3 +149,36 @@ def test_poiseuille2d_driver(ctx, tmp_path, oracle):
         # the exporter writes 17 significant digits, which round-trip a double
         assert np.array_equal(vtu_field(tmp_path / f"solution{steps}.vtu", name), f[key]), name
     assert np.abs(f["u0"]).max() > 0.0
+
+
+def test_reference_cylinder_config_vs_oracle(ctx, oracle):
+    """The Cylinder2D driver's exact configuration (Scheme A, absolute tolerance 1e-8, non-restarted
+    GMRES and CG with Jacobi, the reference's Dirichlet elimination heuristic, level-2 centroid seeding)
+    on the smaller CylinderMesh.dat, 3 steps against the oracle: fields within 1e-6 relative L2 and loads
+    within 1e-5 (the absolute solver tolerance leaves differences of that order between two reduction
+    orders), particle cells identical."""
+    from gpupfem2_amd import cases
+    from helpers import oracle_stepper
+    v, c, ndev = oracle.load_dat(golden("CylinderMesh.dat"), 2)
+    circle = (0.2, 0.2, 0.05)
+    ids = cases.classify_edges(v, c, 0.0, 2.2, 0.0, 0.41, ids=(0, 1, 2, 2), circle=circle, circle_tol=1e-3,
+                               circle_id=3)
+    bu, bv0, bv1, bp, bpv = cases.channel_bcs(v, 0.0, 2.2, 0.0, 0.41, umax=1.5, circle=circle)
+    case = cases.Pfem2Case(
+        name="Cylinder2D", xy=v, tri=c, edge_ids=ids, bcu_node=bu, bcu_val0=bv0, bcu_val1=bv1, bcp_node=bp,
+        bcp_val=bpv, rho=1.0, mu=0.001, dt=0.001, scheme=0, traction_mask=0b10,
+        normals=[(0.0, 0.0), (1.0, 0.0)], outer_iterations=2, substeps=3, pred_solver=cases.GMRES,
+        pred_precond=cases.JACOBI, pres_solver=cases.CG, pres_precond=cases.JACOBI, corr_solver=cases.CG,
+        corr_precond=cases.JACOBI, tol=1e-8, tol_mode=cases.ABS, maxit=1000, restart=0,
+        dirichlet_mode=cases.PARITY, level=2, seed_mode=0, ppc=0, seed=0, body_id=3, body_center=(0.2, 0.2),
+        force_coeff=2.0 / (1.0 * 1.0 * 0.1))
+    print("reference cylinder config", c.shape[0], "cells", flush=True)
+    mesh, ph, stepper = cases.build_gpu(ctx, case, precision=0, pres_ordering=0)
+    oph, ost = oracle_stepper(oracle, case)
+    for s in range(3):
+        fg, fo = stepper.step(), ost.step()
+        assert np.allclose(fg, fo, rtol=1e-5, atol=1e-8), (s, fg, fo)
+    g, o = stepper.fields(), ost.fields()
+    for k in ("u0", "u1", "p"):
+        assert rel_l2(g[k], o[k]) < 1e-6, (k, rel_l2(g[k], o[k]))
+    assert np.array_equal(np.sort(ph.get()["cell"]), np.sort(oph.get()["cell"]))

@@ -223,3 +223,26 @@ def test_poisson3d_manufactured(oracle):
         exact = np.prod(np.sin(np.pi * xyz), axis=1)
         errs.append(np.abs(x - exact).max())
     assert errs[2] < errs[1] < errs[0] and errs[1] / errs[2] > 3.0, errs
+
+
+def test_gmres_restart_zero_is_unrestarted(oracle):
+    """restart = 0 (the library's code for the reference's non-restarted GMRES) runs the same iterations
+    as restart = maxit and leaves x untouched when it does not converge."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(3)
+    n = 60
+    A = sp.random(n, n, density=0.08, random_state=3) + sp.eye(n) * 4.0
+    A = A.tocsr()
+    A.sort_indices()
+    rp, col, val = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.copy()
+    b = rng.standard_normal(n)
+    xs = []
+    for restart in (0, 40):
+        x = np.zeros(n)
+        st = oracle.gmres(rp, col, val, x, b, 1e-10, 40, restart, 0, "jacobi")
+        assert st.converged
+        xs.append((st.iterations, x))
+    assert xs[0][0] == xs[1][0] and np.array_equal(xs[0][1], xs[1][1])
+    x = np.full(n, 0.5)
+    st = oracle.gmres(rp, col, val, x, b, 1e-30, 5, 0, 0, "jacobi")
+    assert not st.converged and st.iterations == 5 and np.all(x == 0.5)
