@@ -80,6 +80,8 @@ SIGNATURES = {
     "pfem_mesh3d_destroy": (_i, [_vp]),
     "pfem_mesh3d_info": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
     "pfem_mesh3d_download": (_i, [_vp, _i, _vp]),
+    "pfem_mesh3d_fill_neighbors": (_i, [_vp, C.POINTER(C.c_long)]),
+    "pfem_mesh3d_device_ptr": (_i, [_vp, _i, C.POINTER(_vp)]),
     "pfem_csr_create_from_mesh2d": (_i, [_vp, _pp]),
     "pfem_csr_create_from_mesh3d": (_i, [_vp, _pp]),
     "pfem_csr_create": (_i, [_vp, _i, _i, _vp, _vp, _vp, _pp]),
@@ -334,9 +336,17 @@ class Mesh3D(_Handle):
         lib().pfem_mesh3d_info(self.h, C.byref(nv), C.byref(nt))
         self.nv, self.nt = nv.value, nt.value
 
+    def fill_neighbors(self):
+        """Face neighbour lists (Mesh3D::fillCellNeighborIndices): (offsets, indices)."""
+        total = C.c_long()
+        self.ctx.check(lib().pfem_mesh3d_fill_neighbors(self.h, C.byref(total)))
+        self.n_neighbors = total.value
+        return self.download(4), self.download(5)
+
     def download(self, what):
         spec = {0: (3 * self.nv, np.float64), 1: (4 * self.nt, np.int32), 2: (self.nt, np.float64),
-                3: (9 * self.nt, np.float64)}[what]
+                3: (9 * self.nt, np.float64), 4: (self.nt + 1, np.int32),
+                5: (getattr(self, "n_neighbors", 0), np.int32)}[what]
         out = np.empty(spec[0], spec[1])
         self.ctx.check(lib().pfem_mesh3d_download(self.h, what, _ptr(out)))
         return out

@@ -140,6 +140,8 @@ struct pfem_mesh3d_t {
     std::vector<int> h_tet;
     pfem::DevArray<double> xyz, vol, invJ;
     pfem::DevArray<int> tet;
+    pfem::DevArray<int> nbr_off, nbr_idx;   // face neighbours, built on request
+    long n_nbr = 0;
     std::shared_ptr<pfem::Pattern> pat;
     Incidence inc;
 };

@@ -247,6 +247,64 @@ __global__ void k_geometry3d(int nt, const double* __restrict__ p, const int* __
     o[8] = id * (m[0] * m[4] - m[1] * m[3]);
 }
 
+// Face neighbours of Mesh3D::fillCellNeighborIndices (src/mesh_3d.cu:37-82, 198-227): the cells whose
+// vertex lists have exactly 3 equal pairs with this cell's, ascending. The reference tests all T^2 pairs;
+// here one lane per cell tests only the cells of its vertices' incidence lists (every cell that shares a
+// pair is in one of them), each candidate once: at the first vertex of this cell it contains, and at that
+// vertex's first slot in the candidate. Pass 1 counts, pass 2 writes and sorts the short segment.
+template <bool FILL>
+__global__ void k_face_neighbors3d(int nt, const int* __restrict__ tet, const int* __restrict__ ioff,
+                                   const int* __restrict__ icode, int* __restrict__ cnt_or_off, int* __restrict__ idx)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nt) return;
+    int a[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a[i] = tet[4 * (size_t)c + i];
+    const int base = FILL ? cnt_or_off[c] : 0;
+    int k = 0;
+    for (int i = 0; i < 4; ++i) {
+        bool repeat = false;
+        for (int j = 0; j < i; ++j) repeat |= a[j] == a[i];
+        if (repeat) continue;
+        for (int e = ioff[a[i]]; e < ioff[a[i] + 1]; ++e) {
+            const int code = icode[e];
+            const int d = code >> 2, slot = code & 3;
+            if (d == c) continue;
+            int b[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = tet[4 * (size_t)d + j];
+            bool skip = false;
+            for (int j = 0; j < slot; ++j) skip |= b[j] == a[i];      // d listed again for a repeated vertex
+            for (int j = 0; j < i; ++j)                              // d already met at an earlier vertex
+                skip |= a[j] == b[0] || a[j] == b[1] || a[j] == b[2] || a[j] == b[3];
+            if (skip) continue;
+            int pairs = 0;
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+                for (int y = 0; y < 4; ++y) pairs += a[x] == b[y];
+            if (pairs == 3) {
+                if (FILL) idx[base + k] = d;
+                ++k;
+            }
+        }
+    }
+    if (!FILL) {
+        cnt_or_off[c] = k;
+        return;
+    }
+    for (int x = 1; x < k; ++x) {   // insertion sort of the segment (at most 4 entries on a valid mesh)
+        const int v = idx[base + x];
+        int y = x - 1;
+        while (y >= 0 && idx[base + y] > v) {
+            idx[base + y + 1] = idx[base + y];
+            --y;
+        }
+        idx[base + y + 1] = v;
+    }
+}
+
 // ------------------------------------------------------------------ DAT import
 // SALOME DAT reader semantics of Mesh2D/Mesh3D::loadMeshFromFile (src/mesh_2d.cu:36-96,
 // src/mesh_3d.cu:84-147): header "nV nElems"; vertex records "id x y z"; element records
@@ -602,6 +660,49 @@ int pfem_mesh3d_info(pfem_mesh3d m, int* nv, int* nt)
     return PFEM_OK;
 }
 
+int pfem_mesh3d_fill_neighbors(pfem_mesh3d m, long* n_neighbors)
+{
+    PFEM_TRY(m ? m->ctx : nullptr, {
+        PFEM_REQUIRE(m, "null argument");
+        Context* c = C(m->ctx);
+        const int nt = m->nt;
+        if (m->nbr_off.n != (size_t)nt + 1) {
+            DevArray<int> cnt(std::max(nt, 1));
+            m->nbr_off.alloc((size_t)nt + 1);
+            const unsigned g = blocks_for(std::max(nt, 1));
+            hipLaunchKernelGGL((k_face_neighbors3d<false>), dim3(g), dim3(kBlock), 0, c->stream, nt, m->tet.p,
+                               m->inc.off.p, m->inc.code.p, cnt.p, (int*)nullptr);
+            PFEM_HIP(hipGetLastError());
+            exclusive_scan_int(c, cnt.p, m->nbr_off.p, nt, m->nbr_off.p + nt);
+            int total = 0;
+            d2h_sync(c, &total, m->nbr_off.p + nt, sizeof(int));
+            m->nbr_idx.alloc(std::max(total, 1));
+            m->n_nbr = total;
+            hipLaunchKernelGGL((k_face_neighbors3d<true>), dim3(g), dim3(kBlock), 0, c->stream, nt, m->tet.p,
+                               m->inc.off.p, m->inc.code.p, m->nbr_off.p, m->nbr_idx.p);
+            PFEM_HIP(hipGetLastError());
+            PFEM_HIP(hipStreamSynchronize(c->stream));
+        }
+        if (n_neighbors) *n_neighbors = m->n_nbr;
+    });
+}
+
+int pfem_mesh3d_device_ptr(pfem_mesh3d m, int what, void** d_ptr)
+{
+    PFEM_TRY(m ? m->ctx : nullptr, {
+        PFEM_REQUIRE(m && d_ptr, "null argument");
+        switch (what) {
+        case 0: *d_ptr = m->xyz.p; break;
+        case 1: *d_ptr = m->tet.p; break;
+        case 2: *d_ptr = m->vol.p; break;
+        case 3: *d_ptr = m->invJ.p; break;
+        case 4: *d_ptr = m->nbr_off.p; break;
+        case 5: *d_ptr = m->nbr_idx.p; break;
+        default: throw Error(PFEM_EINVAL, "unknown mesh array");
+        }
+    });
+}
+
 int pfem_mesh3d_download(pfem_mesh3d m, int what, void* dst)
 {
     PFEM_TRY(m ? m->ctx : nullptr, {
@@ -612,6 +713,12 @@ int pfem_mesh3d_download(pfem_mesh3d m, int what, void* dst)
         case 1: d2h_sync(c, dst, m->tet.p, m->tet.bytes()); break;
         case 2: d2h_sync(c, dst, m->vol.p, m->vol.bytes()); break;
         case 3: d2h_sync(c, dst, m->invJ.p, m->invJ.bytes()); break;
+        case 4:
+        case 5:
+            PFEM_REQUIRE(m->nbr_off.n == (size_t)m->nt + 1, "face neighbours not built (pfem_mesh3d_fill_neighbors)");
+            if (what == 4) d2h_sync(c, dst, m->nbr_off.p, m->nbr_off.bytes());
+            else d2h_sync(c, dst, m->nbr_idx.p, sizeof(int) * (size_t)m->n_nbr);
+            break;
         default: throw Error(PFEM_EINVAL, "unknown mesh array");
         }
     });

@@ -5,6 +5,7 @@
 // advect and project; SimulationParameters defaults and quadrature tables are the reference's.
 //
 //   facade_check mesh.dat
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -190,6 +191,33 @@ int main(int argc, char* argv[])
     report(q.size() == 3 && std::fabs(wsum - 1.0) < 1e-14, "face_quadrature_1", (double)q.size(), wsum);
     const auto e = createEdgeQuadratureFormula(1);
     report(e.size() == 2 && std::fabs(e[0].weight + e[1].weight - 2.0) < 1e-14, "edge_quadrature_1", (double)e.size());
+
+    if (argc > 2) {
+        // Mesh3D with the reference's default arguments builds the face-neighbour lists: every listed pair
+        // shares exactly three vertices, lists ascend, the relation is symmetric, at most 4 per cell
+        Mesh3D box;
+        const bool ok3 = box.loadMeshFromFile(argv[2]);
+        report(ok3, "load3d");
+        if (ok3) {
+            std::vector<int> off, idx, tet(4 * (size_t)box.numCells());
+            box.getHostCellNeighbors(off, idx);
+            gpupfem2::check(pfem_mesh3d_download(box.handle(), 1, tet.data()));
+            bool good = box.getCellNeighborsOffsets().size == box.numCells() + 1 &&
+                        box.getCellNeighborIndices().size == (int)idx.size() && off.back() == (int)idx.size();
+            for (int c = 0; c < box.numCells() && good; ++c) {
+                good = off[c + 1] - off[c] <= 4;
+                for (int k = off[c]; k < off[c + 1] && good; ++k) {
+                    const int d = idx[k];
+                    int pairs = 0;
+                    for (int i = 0; i < 4; ++i)
+                        for (int j = 0; j < 4; ++j) pairs += tet[4 * (size_t)c + i] == tet[4 * (size_t)d + j];
+                    good = pairs == 3 && (k == off[c] || idx[k - 1] < d) &&
+                           std::binary_search(idx.begin() + off[d], idx.begin() + off[d + 1], c);
+                }
+            }
+            report(good, "mesh3d_face_neighbors", (double)idx.size(), box.numCells());
+        }
+    }
 
     std::printf("%s %d failures\n", failures ? "FAIL" : "ok", failures);
     return failures ? EXIT_FAILURE : EXIT_SUCCESS;

@@ -281,9 +281,12 @@ public:
     {
         if (h_) pfem_mesh3d_destroy(h_);
     }
-    /* Mesh3D::loadMeshFromFile (src/mesh_3d.cu:84-147); zeroSmallDet keeps the reference's zeroing of inverse
-     * Jacobians with |det| < 2e-6 (matrix3x3.cuh:134-154) */
-    bool loadMeshFromFile(const std::string& filename, double scale = 1.0, bool zeroSmallDet = true)
+    /* Mesh3D::loadMeshFromFile (src/mesh_3d.cu:84-147), the reference's arguments: the face-neighbour lists
+     * are built unless fillNeighborLists is false (fillCellNeighborIndices, :198-227). The additive
+     * zeroSmallDet keeps the reference's zeroing of inverse Jacobians with |det| < 2e-6
+     * (matrix3x3.cuh:134-154). False when the file cannot be read. */
+    bool loadMeshFromFile(const std::string& filename, bool fillNeighborLists = true, double scale = 1.0,
+                          bool zeroSmallDet = true)
     {
         pfem_mesh3d m = nullptr;
         if (pfem_mesh3d_load_dat(gpupfem2::context(), filename.c_str(), scale, zeroSmallDet ? 1 : 0, &m) != PFEM_OK) {
@@ -292,16 +295,53 @@ public:
         }
         if (h_) pfem_mesh3d_destroy(h_);
         h_ = m;
+        nnb_ = 0;
+        nbrBuilt_ = false;
         gpupfem2::check(pfem_mesh3d_info(h_, &nv_, &nt_));
+        if (fillNeighborLists) fillCellNeighborIndices();
         return true;
+    }
+    /* Mesh3D::fillCellNeighborIndices (src/mesh_3d.cu:198-227): cells sharing a face, ascending */
+    void fillCellNeighborIndices()
+    {
+        gpupfem2::check(pfem_mesh3d_fill_neighbors(handleChecked(), &nnb_));
+        nbrBuilt_ = true;
     }
     int numVertices() const { return nv_; }
     int numCells() const { return nt_; }
+    deviceSpan<Point3> getVertices() const { return {static_cast<Point3*>(ptr(0)), nv_}; }
+    deviceSpan<int> getCells() const { return {static_cast<int*>(ptr(1)), 4 * nt_}; }
+    deviceSpan<double> getCellVolume() const { return {static_cast<double*>(ptr(2)), nt_}; }
+    deviceSpan<double> getInvJacobi() const { return {static_cast<double*>(ptr(3)), 9 * nt_}; }
+    deviceSpan<int> getCellNeighborsOffsets() const { return {static_cast<int*>(ptr(4)), nbrBuilt_ ? nt_ + 1 : 0}; }
+    deviceSpan<int> getCellNeighborIndices() const { return {static_cast<int*>(ptr(5)), (int)nnb_}; }
+    /* additive: the face-neighbour lists on the host, as (offsets, indices) */
+    void getHostCellNeighbors(std::vector<int>& offsets, std::vector<int>& indices) const
+    {
+        if (!nbrBuilt_) throw gpupfem2::Error(PFEM_EINVAL, "face neighbours not built");
+        offsets.resize((size_t)nt_ + 1);
+        indices.resize((size_t)nnb_);
+        gpupfem2::check(pfem_mesh3d_download(handleChecked(), 4, offsets.data()));
+        if (nnb_) gpupfem2::check(pfem_mesh3d_download(handleChecked(), 5, indices.data()));
+    }
     pfem_mesh3d handle() const { return h_; }
 
 private:
+    pfem_mesh3d handleChecked() const
+    {
+        if (!h_) throw gpupfem2::Error(PFEM_EINVAL, "Mesh3D used before loadMeshFromFile");
+        return h_;
+    }
+    void* ptr(int what) const
+    {
+        void* p = nullptr;
+        gpupfem2::check(pfem_mesh3d_device_ptr(handleChecked(), what, &p));
+        return p;
+    }
     pfem_mesh3d h_ = nullptr;
     int nv_ = 0, nt_ = 0;
+    long nnb_ = 0;
+    bool nbrBuilt_ = false;
 };
 
 /* ------------------------------------------------------------------ CSR matrix */

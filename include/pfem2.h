@@ -82,16 +82,26 @@ int pfem_mesh2d_classify_edges(pfem_mesh2d m, double x0, double x1, double y0, d
 int pfem_mesh2d_set_edge_ids(pfem_mesh2d m, const int* ids);
 
 /* ------------------------------------------------------------------ 3D mesh (src/mesh_3d.cuh) */
-/* Mesh3D::loadMeshFromFile (src/mesh_3d.cu:84-147); face neighbours are not built (no 3D particles).
- * zero_small_det = 1 keeps the reference's zero inverse Jacobian for |det| < 2e-6
+/* Mesh3D::loadMeshFromFile (src/mesh_3d.cu:84-147) without the neighbour lists (pfem_mesh3d_fill_neighbors
+ * builds them). zero_small_det = 1 keeps the reference's zero inverse Jacobian for |det| < 2e-6
  * (src/common/matrix3x3.cuh:139-140); 0 keeps the true inverse. */
 int pfem_mesh3d_load_dat(pfem_ctx ctx, const char* path, double scale, int zero_small_det, pfem_mesh3d* out);
 int pfem_mesh3d_create(pfem_ctx ctx, int nv, const double* xyz, int nt, const int* tet, int zero_small_det,
                        pfem_mesh3d* out);
 int pfem_mesh3d_destroy(pfem_mesh3d m);
 int pfem_mesh3d_info(pfem_mesh3d m, int* nv, int* nt);
-/* what: 0 vertices (3 dbl), 1 cells (4 int), 2 volume (1 dbl), 3 inverse Jacobian (9 dbl, row-major) */
+/* Mesh3D::fillCellNeighborIndices (src/mesh_3d.cu:37-82, 198-227; run by loadMeshFromFile when
+ * fillNeighborLists is set): per cell, the cells sharing a face (exactly 3 equal vertex pairs),
+ * ascending, as offsets (nt + 1) and indices. Built on the GPU from the vertex -> cell incidence in O(T)
+ * instead of the reference's all-pairs O(T^2) test; a second call returns the lists already built.
+ * n_neighbors (may be null) receives the total. */
+int pfem_mesh3d_fill_neighbors(pfem_mesh3d m, long* n_neighbors);
+/* what: 0 vertices (3 dbl), 1 cells (4 int), 2 volume (1 dbl), 3 inverse Jacobian (9 dbl, row-major),
+ * 4 face-neighbour offsets (nt + 1 int), 5 face-neighbour indices (int; after pfem_mesh3d_fill_neighbors) */
 int pfem_mesh3d_download(pfem_mesh3d m, int what, void* dst);
+/* the same arrays in device memory (what 0..5; 4 and 5 null until the neighbours are built), owned by the
+ * mesh: the deviceVector members behind Mesh3D's getters (getVertices().data, getCellNeighborIndices().data) */
+int pfem_mesh3d_device_ptr(pfem_mesh3d m, int what, void** d_ptr);
 
 /* ------------------------------------------------------------------ CSR (src/linear_algebra/sparse_matrix.cuh) */
 /* SparseMatrixCSR(const Mesh2D&) / (const Mesh3D&) (sparse_matrix.cu:8-99): pattern {i} U {vertices

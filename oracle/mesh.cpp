@@ -135,6 +135,39 @@ void orc_neighbors_bruteforce_fill(int nt, const int* tri, int* offsets, int* id
     }
 }
 
+// src/mesh_3d.cu:37-82 (kFindNeighbors, the path loadMeshFromFile takes by default) and :149-188 (the CPU
+// variant): tetrahedra whose vertex lists have exactly 3 equal pairs (a shared face), every other cell
+// tested in ascending order. The GPU kernel counts all equal (i, j) pairs; the CPU variant stops at the
+// first match per vertex of the first cell; the two agree on tetrahedra with distinct vertices, and the
+// pair count is what is restated here. O(T^2): only for the small fixtures.
+static int tet_common(const int* a, const int* b)
+{
+    int n = 0;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) n += a[i] == b[j];
+    return n;
+}
+
+long orc_face_neighbors3d_count(int nt, const int* tet)
+{
+    long total = 0;
+    for (int i = 0; i < nt; ++i)
+        for (int j = 0; j < nt; ++j)
+            if (j != i && tet_common(tet + 4 * (size_t)i, tet + 4 * (size_t)j) == 3) ++total;
+    return total;
+}
+
+void orc_face_neighbors3d_fill(int nt, const int* tet, int* offsets, int* idx)
+{
+    offsets[0] = 0;
+    int k = 0;
+    for (int i = 0; i < nt; ++i) {
+        for (int j = 0; j < nt; ++j)
+            if (j != i && tet_common(tet + 4 * (size_t)i, tet + 4 * (size_t)j) == 3) idx[k++] = j;
+        offsets[i + 1] = k;
+    }
+}
+
 // Quadrature tables: the decimal literals of src/integration/quadrature_formula_1d.cu:3-44,
 // quadrature_formula_2d.cu:3-126 and quadrature_formula_3d.cu:3-171, kept digit for digit because the
 // assembled values depend on them. The last barycentric coordinate is 1 - sum(others), as there.

@@ -30,6 +30,8 @@ long orc_csr_count(int nv, int nt, int nper, const int* cells);
 void orc_csr_fill(int nv, int nt, int nper, const int* cells, int* rowptr, int* col);
 long orc_neighbors_bruteforce_count(int nt, const int* tri);
 void orc_neighbors_bruteforce_fill(int nt, const int* tri, int* offsets, int* idx);
+long orc_face_neighbors3d_count(int nt, const int* tet);
+void orc_face_neighbors3d_fill(int nt, const int* tet, int* offsets, int* idx);
 
 /* ---------------- quadrature (src/integration/quadrature_formula_*.cu) ---------------- */
 /* kind 1 = edge (coord, weight), 2 = face (l1, l2, l3, weight), 3 = cell (l1..l4, weight).

@@ -69,6 +69,8 @@ def _declare(L):
         "orc_csr_fill": (None, [i, i, i, vp, vp, vp]),
         "orc_neighbors_bruteforce_count": (l, [i, vp]),
         "orc_neighbors_bruteforce_fill": (None, [i, vp, vp, vp]),
+        "orc_face_neighbors3d_count": (l, [i, vp]),
+        "orc_face_neighbors3d_fill": (None, [i, vp, vp, vp]),
         "orc_quadrature": (i, [i, i, vp]),
         "orc_spmv": (None, [i, vp, vp, vp, vp, vp]),
         "orc_dirichlet_apply": (None, [i, vp, vp, vp, vp, i, vp, vp, i]),
@@ -188,6 +190,17 @@ def neighbors_bruteforce(tri):
     off = np.empty(nt + 1, I)
     idx = np.empty(total, I)
     lib().orc_neighbors_bruteforce_fill(nt, _p(tri), _p(off), _p(idx))
+    return off, idx
+
+
+def face_neighbors3d(tet):
+    """Face neighbours of tetrahedra by the all-pairs test of src/mesh_3d.cu:37-82 (small meshes only)."""
+    tet = np.ascontiguousarray(tet, I)
+    nt = tet.shape[0]
+    total = lib().orc_face_neighbors3d_count(nt, _p(tet))
+    off = np.empty(nt + 1, I)
+    idx = np.empty(total, I)
+    lib().orc_face_neighbors3d_fill(nt, _p(tet), _p(off), _p(idx))
     return off, idx
 
 

@@ -34,3 +34,22 @@ def oracle_stepper(O, case, gpu_mesh=None):
     st = O.Pfem2(case.xy, case.tri, case.edge_ids, cfg, ph, case.bcu_node, case.bcu_val0, case.bcu_val1,
                  case.bcp_node, case.bcp_val)
     return ph, st
+
+
+def face_neighbors_by_hashing(tet):
+    """Face neighbours found independently of the all-pairs test: the four sorted vertex triples of every
+    tetrahedron, sorted together, pair up the two cells of each interior face."""
+    T = tet.shape[0]
+    faces = np.sort(tet[:, [[1, 2, 3], [0, 2, 3], [0, 1, 3], [0, 1, 2]]], axis=2).reshape(-1, 3)
+    owner = np.repeat(np.arange(T), 4)
+    order = np.lexsort((faces[:, 2], faces[:, 1], faces[:, 0]))
+    f, o = faces[order], owner[order]
+    same = np.all(f[1:] == f[:-1], axis=1)
+    assert not np.any(same[1:] & same[:-1]), "a face shared by three cells"
+    a, b = o[:-1][same], o[1:][same]
+    src, dst = np.concatenate([a, b]), np.concatenate([b, a])
+    k = np.lexsort((dst, src))
+    off = np.zeros(T + 1, np.int64)
+    np.add.at(off, src + 1, 1)
+    return np.cumsum(off), dst[k], int(same.sum())
+

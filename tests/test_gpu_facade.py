@@ -23,9 +23,10 @@ def run(args, cwd):
 
 
 def test_facade_checks(tmp_path):
-    rc, out = run([os.path.join(EX, "facade_check"), golden("ChannelMesh.dat")], tmp_path)
+    rc, out = run([os.path.join(EX, "facade_check"), golden("ChannelMesh.dat"), golden("BoxMesh.dat")], tmp_path)
     print(out)
     assert rc == 0 and "FAIL" not in out and out.strip().endswith("ok 0 failures")
+    assert "ok mesh3d_face_neighbors 19898 5393" in out   # the oracle's count on BoxMesh.dat
 
 
 def test_poisson2d_driver_matches_oracle(tmp_path, oracle):

@@ -63,6 +63,50 @@ def test_mesh3d_import(ctx, oracle):
     assert np.array_equal(grp, rp) and np.array_equal(gcol, col)
 
 
+def test_mesh3d_face_neighbors(ctx, oracle):
+    """Mesh3D::fillCellNeighborIndices (src/mesh_3d.cu:37-82, 198-227) built from the incidence lists:
+    BoxMesh.dat, and its cells followed by cells with a repeated vertex, equal the oracle's all-pairs
+    test; the box of 64^3 cubes (1.57M tetrahedra) equals the face hashing; the 128^3 box (12.6M) has
+    4T - 12 n^2 entries (12 n^2 boundary triangles), at most 4 per cell, ascending, and sampled entries
+    share exactly one face and are listed both ways. A second call returns the same lists."""
+    from gpupfem2_amd import capi, meshgen
+    from helpers import face_neighbors_by_hashing
+    v, c, ndev = oracle.load_dat(golden("BoxMesh.dat"), 3)
+    m = capi.Mesh3D(ctx, path=golden("BoxMesh.dat"))
+    off, idx = m.fill_neighbors()
+    roff, ridx = oracle.face_neighbors3d(c[:ndev])
+    assert np.array_equal(off, roff) and np.array_equal(idx, ridx)
+    off2, idx2 = m.fill_neighbors()
+    assert np.array_equal(off2, off) and np.array_equal(idx2, idx)
+    tet = c[:ndev]
+    degenerate = np.array([[t[0], t[0], t[1], t[2]] for t in tet[:40:3]] +
+                          [[t[1], t[1], t[2], t[2]] for t in tet[5:60:7]], np.int32)
+    tet2 = np.concatenate([tet, degenerate])
+    m2 = capi.Mesh3D(ctx, v, tet2, zero_small_det=True)
+    off, idx = m2.fill_neighbors()
+    roff, ridx = oracle.face_neighbors3d(tet2)
+    assert np.array_equal(off, roff) and np.array_equal(idx, ridx)
+    for n in (64, 128):
+        print("face neighbours box", n, flush=True)
+        xyz, tet = meshgen.box3d(n)
+        m = capi.Mesh3D(ctx, xyz, tet)
+        off, idx = m.fill_neighbors()
+        T = tet.shape[0]
+        assert off[-1] == idx.shape[0] == 4 * T - 12 * n * n
+        assert np.diff(off).max() == 4
+        if n == 64:
+            hoff, hidx, _ = face_neighbors_by_hashing(tet)
+            assert np.array_equal(off, hoff) and np.array_equal(idx, hidx)
+            continue
+        seg = np.repeat(np.arange(T), np.diff(off))
+        assert np.all((idx[1:] > idx[:-1]) | (seg[1:] != seg[:-1]))
+        rng = np.random.default_rng(7)
+        for cc in rng.integers(0, T, 2000):
+            for d in idx[off[cc]:off[cc + 1]]:
+                assert np.sum(tet[cc][:, None] == tet[d][None, :]) == 3
+                assert cc in idx[off[d]:off[d + 1]]
+
+
 def test_generated_cylinder_setup(ctx, oracle):
     from gpupfem2_amd import capi, cases
     case = cases.cylinder_case(h=1.0 / 8)

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from conftest import golden
+from helpers import face_neighbors_by_hashing
 
 # (file, dim, V, T, CSR nnz, neighbour entries, max neighbours) -- counts derived from the fixtures
 FIXTURES = [
@@ -132,3 +133,28 @@ def test_generated_meshes_closed_forms(oracle):
     rp, col = oracle.csr_pattern(xyz.shape[0], tet)
     E = 3 * 8 * 9 * 9 + 3 * 8 * 8 * 9 + 8 ** 3
     assert col.shape[0] == 9 ** 3 + 2 * E
+
+
+def test_face_neighbors3d_pinned(oracle):
+    """The oracle's all-pairs face test (src/mesh_3d.cu:37-82) on BoxMesh.dat equals the face-hashing
+    construction, and the count of boundary faces (4T minus the neighbour entries) equals Euler's 2V_b - 4
+    for the closed triangulated surface of the box, V_b its vertices on the bounding planes."""
+    v, c, ndev = oracle.load_dat(golden("BoxMesh.dat"), 3)
+    tet = c[:ndev]
+    off, idx = oracle.face_neighbors3d(tet)
+    hoff, hidx, interior = face_neighbors_by_hashing(tet)
+    assert np.array_equal(off, hoff) and np.array_equal(idx, hidx)
+    lo, hi = v.min(axis=0), v.max(axis=0)
+    vb = int(np.sum(np.any((np.abs(v - lo) < 1e-12) | (np.abs(v - hi) < 1e-12), axis=1)))
+    assert 4 * tet.shape[0] - idx.shape[0] == 2 * vb - 4
+    assert idx.shape[0] == 2 * interior and np.diff(off).max() == 4
+
+
+def test_face_neighbors3d_degenerate(oracle):
+    """Cells with a repeated vertex follow the pair count of the reference's kernel: (a, a, b, c) against
+    a cell with face (a, b, c) has 4 equal pairs and is not a neighbour; against a cell holding only a and b
+    it has 2 + 1 = 3 and is one."""
+    tet = np.array([[0, 1, 2, 3], [0, 1, 2, 4], [0, 0, 1, 5], [0, 0, 1, 2]], np.int32)
+    off, idx = oracle.face_neighbors3d(tet)
+    got = [sorted(idx[off[i]:off[i + 1]].tolist()) for i in range(4)]
+    assert got == [[1, 2], [0, 2], [0, 1], []]
