@@ -290,35 +290,78 @@ __device__ __forceinline__ TF sweep_sum(const SweepArgs<TV, TS, TF>& a, TF s, in
 // TAGGED (sliced layout, FP64 published values): sy and sw are one buffer; the forward sweep publishes y
 // with tag 1, the backward sweep replaces it by z with tag 0, and neither re-arms anything. z keeps tag 0
 // so that an exact zero (a Dirichlet row's) stays exactly zero.
+//
+// In the fused PCG with an IC0 factor (TAGGED, XR, not UNIT, a.cg set) this sweep also ends the iteration:
+// with M = L L^T, gamma_{k+1} = (r_{k+1}, M^-1 r_{k+1}) = (y, y) for y = L^-1 r_{k+1}, so the chunks
+// publish their partials of (y, y) and the last workgroup adds them and records the iteration
+// (cg_record), which the backward sweep would otherwise do from (r, z) after reading r once more. A
+// 16-bit factor stored as c L gives c y here: the sum is divided by c^2 (a power of two, exact).
 template <typename TV, typename TS, typename TF, bool UNIT, int LAYOUT, bool XR, bool TAGGED = false>
 __global__ __launch_bounds__(kBlock) void k_sweep_lower(SweepArgs<TV, TS, TF> a)
 {
+    constexpr bool kYY = TAGGED && XR && !UNIT;
+    __shared__ double wsum[kWaves];
+    __shared__ unsigned arrived;
     if (a.done && *a.done) return;
-    int i, e0, stride, width;
-    if (!sweep_row<LAYOUT>(a, i, e0, stride, width)) return;
-    TV b;
-    if constexpr (XR) {
-        // updateXR (linear_solver.cu:68-77) for this row, as the separate vector kernel computes it (the
-        // x half is left to the p-update kernel when a.x is null)
-        const TV al = (TV)(a.acg ? a.acg->alpha[a.acg->it & 1] : *a.alpha);
-        if (a.x) a.x[i] += al * a.p[i];
-        b = a.r[i] - al * a.q[i];
-        a.r[i] = b;
-    } else {
-        b = a.src[a.perm ? a.perm[i] : i];
+    const bool yy = kYY && a.cg;
+    if (yy) {
+        if (threadIdx.x == 0) arrived = 0u;
+        __syncthreads();
     }
-    if constexpr (TAGGED) {
-        static_assert(LAYOUT == kSliced && sizeof(TF) == 8, "tagged sweeps: sliced layout, FP64 values");
-        TF b0 = (TF)b;
-        if constexpr (sizeof(TS) == 2) b0 *= (TF)a.in_scale[1];
-        const TF s = sweep_sum<LAYOUT, 1>(a, b0, i, e0, stride, width, a.sy);
-        publish_tagged<1>(a.sy + i, UNIT ? s : s / (TF)a.d[i]);
-    } else {
-        TF b0 = (TF)b;
-        if constexpr (sizeof(TS) == 2) b0 *= (TF)a.in_scale[1];
-        const TF s = sweep_sum<LAYOUT, -1>(a, b0, i, e0, stride, width, a.sy);
-        publish(a.sy + i, UNIT ? s : s / (TF)a.d[i]);
-        a.sw[i] = sentinel_value<TF>();
+    int i, e0, stride, width;
+    const bool has = sweep_row<LAYOUT>(a, i, e0, stride, width);
+    if (!yy && !has) return;
+    double v = 0.0;
+    if (has) {
+        TV b;
+        if constexpr (XR) {
+            // updateXR (linear_solver.cu:68-77) for this row, as the separate vector kernel computes it
+            // (the x half is left to the p-update kernel when a.x is null)
+            const TV al = (TV)(a.acg ? a.acg->alpha[a.acg->it & 1] : *a.alpha);
+            if (a.x) a.x[i] += al * a.p[i];
+            b = a.r[i] - al * a.q[i];
+            a.r[i] = b;
+        } else {
+            b = a.src[a.perm ? a.perm[i] : i];
+        }
+        if constexpr (TAGGED) {
+            static_assert(LAYOUT == kSliced && sizeof(TF) == 8, "tagged sweeps: sliced layout, FP64 values");
+            TF b0 = (TF)b;
+            if constexpr (sizeof(TS) == 2) b0 *= (TF)a.in_scale[1];
+            const TF s = sweep_sum<LAYOUT, 1>(a, b0, i, e0, stride, width, a.sy);
+            const TF y = UNIT ? s : s / (TF)a.d[i];
+            publish_tagged<1>(a.sy + i, y);
+            v = (double)y * (double)y;
+        } else {
+            TF b0 = (TF)b;
+            if constexpr (sizeof(TS) == 2) b0 *= (TF)a.in_scale[1];
+            const TF s = sweep_sum<LAYOUT, -1>(a, b0, i, e0, stride, width, a.sy);
+            publish(a.sy + i, UNIT ? s : s / (TF)a.d[i]);
+            a.sw[i] = sentinel_value<TF>();
+        }
+    }
+    if constexpr (kYY) {
+        if (!yy) return;
+        // the backward sweep's reduction: per-wave sums in LDS, the last wave to arrive adds them in wave
+        // order and publishes the chunk's partial into its sentinel-armed slot
+        v = wave_sum(v);
+        if ((threadIdx.x & 63) == 0) {
+            wsum[threadIdx.x >> 6] = v;
+            __threadfence_block();
+            if (atomicAdd(&arrived, 1u) == kWaves - 1) {
+                __threadfence_block();
+                double t = wsum[0];
+                for (int w = 1; w < kWaves; ++w) t += wsum[w];
+                publish(a.part + blockIdx.x, t);
+            }
+        }
+        if (blockIdx.x == gridDim.x - 1) {
+            __syncthreads();
+            double g = collect_partials(a.part, gridDim.x, a.err, a.limit);
+            if constexpr (sizeof(TS) == 2) g /= a.in_scale[1];
+            // iteration k = it (it counts the iterations ended so far)
+            if (threadIdx.x == 0) cg_record(a.cg, g, a.cg->it);
+        }
     }
 }
 
@@ -1062,6 +1105,14 @@ static void sweeps(Context* c, pfem_precond_t* pc, const TV* src, TV* out, const
                 PFEM_REQUIRE(o.factor_order && (const void*)out != (const void*)src && sizeof(TV) == 8,
                              "tagged sweeps need vectors in the factor's ordering and z distinct from r");
                 lo.sy = lo.sw = up.sy = up.sw = reinterpret_cast<TF*>(out);
+                if (xr && ic && up.cg) {
+                    // IC0: the forward sweep forms gamma = (y, y) and ends the iteration; the backward
+                    // sweep only solves (its partial slots and r are not touched)
+                    up.part = nullptr;
+                    up.cg = nullptr;
+                } else {
+                    lo.cg = nullptr;
+                }
                 if (xr) {
                     if (ic) hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, false, kSliced, true, true>), gl, bl, 0, c->stream, lo);
                     else hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, true, kSliced, true, true>), gl, bl, 0, c->stream, lo);
@@ -1108,7 +1159,9 @@ int precond_apply_t(Context* c, pfem_precond_t* pc, TV* z, const TV* r, bool fp3
     constexpr bool f64 = sizeof(TV) == 8;
     const int n = pc->n;
     // the partials need r intact, so an in-place apply (z == r) does not fuse them
-    const bool fuse_ok = opt.part && z != r && pc->sweep && pc->nuchunks <= opt.cap;
+    // (the slots serve one partial per backward chunk, or per forward chunk when the forward sweep ends a
+    // PCG iteration over an IC0 factor)
+    const bool fuse_ok = opt.part && z != r && pc->sweep && std::max(pc->nuchunks, pc->nlchunks) <= opt.cap;
     double* part = fuse_ok ? opt.part : nullptr;
     PFEM_REQUIRE(!opt.factor_order || pc->sweep, "factor-order apply needs the single-pass sweeps");
     switch (pc->kind) {
