@@ -658,6 +658,9 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
     // (two 8-byte passes less per iteration; y and z change by at most one unit in the last place)
     bool tagged = false;
     if constexpr (sizeof(T) == 8) tagged = fused2 && s->fuse >= 3 && A.pc->sell;
+    // with IC0 at level 3 the forward sweep ends the iteration (gamma = (y, y)) and the backward sweep
+    // applies the x and p updates row by row: three launches per iteration
+    const bool xp_in_sweep = tagged && A.kind == PFEM_PRECOND_IC0;
     CgState init{};
     init.tol = A.tol;
     init.tol_mode = A.tol_mode;
@@ -728,10 +731,15 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
         o.alpha_cg = st;
         o.cg = st;
         o.tagged = tagged;
+        if (xp_in_sweep) {
+            o.xpx = A.x;
+            o.xpp = A.p;
+        }
         PFEM_REQUIRE(precond_apply_t<T>(c, A.pc, A.z, A.r, A.pc_fp32, o) > 0,
                      "fused PCG iteration without its partials");
-        hipLaunchKernelGGL((k_cg_xp_recorded<T>), dim3(blocks_for((n + 1) / 2)), dim3(kBlock), 0, c->stream, n, A.x,
-                           A.p, (const T*)A.z, st);
+        if (!xp_in_sweep)
+            hipLaunchKernelGGL((k_cg_xp_recorded<T>), dim3(blocks_for((n + 1) / 2)), dim3(kBlock), 0, c->stream, n,
+                               A.x, A.p, (const T*)A.z, st);
     };
     // a batch of fused iterations captured once per solve as a HIP graph and replayed
     hipGraphExec_t exec = nullptr;

@@ -244,6 +244,9 @@ struct SweepArgs {
     double* part;
     CgState* cg;
     const CgState* acg;   // non-null: alpha = acg->alpha[acg->it & 1] (the device's iteration index)
+    // backward, tagged, fused IC0-PCG: the x and p updates of the iteration (null: a separate kernel)
+    TV* xu;
+    TV* pu;
     // 16-bit factors are stored as c L (c a power of two that puts the largest entry near 2^14); the
     // forward sweep then solves for c^2 b, which leaves z unchanged. in_scale[1] = c^2.
     const double* in_scale;
@@ -367,6 +370,12 @@ __global__ __launch_bounds__(kBlock) void k_sweep_lower(SweepArgs<TV, TS, TF> a)
 
 // backward solve, all levels: z_i = (y_i - sum_{k>i} U_ik z_k) / U_ii, out[perm[i]] = z_i; re-arms sy.
 // part != null: the chunk's partial of (r, z) (r = rdot, in the vectors' order) goes to part[chunk].
+//
+// TAGGED with xu set (the fused IC0-PCG, whose forward sweep has already recorded gamma and beta): each row
+// also gets the end-of-iteration vector updates as soon as its z is known, x += alpha_k p_k (the x half of
+// updateXR, linear_solver.cu:68-77) and p = z + beta p (updateP, :79-87), with the operations of
+// k_cg_xp_recorded, so no separate vector pass runs. Iteration k = it - 1; when it stopped the solve only
+// the x update runs, and a launch finding it equal to the xp_seen its SpMV left does nothing.
 template <typename TV, typename TS, typename TF, int LAYOUT, bool TAGGED = false>
 __global__ __launch_bounds__(kBlock) void k_sweep_upper(SweepArgs<TV, TS, TF> a)
 {
@@ -374,6 +383,27 @@ __global__ __launch_bounds__(kBlock) void k_sweep_upper(SweepArgs<TV, TS, TF> a)
     // last wave to arrive adds the four in wave order (the order block_sum uses)
     __shared__ double wsum[kWaves];
     __shared__ unsigned arrived;
+    if constexpr (TAGGED) {
+        if (a.xu) {
+            const CgState* st = a.acg;
+            const int it = st->it;
+            if (it == st->xp_seen) return;
+            const int k = it - 1;
+            const bool last = st->done;
+            const TV al = (TV)st->alpha[k & 1];
+            int i, e0, stride, width;
+            if (!sweep_row<LAYOUT>(a, i, e0, stride, width)) return;
+            const TV pi = a.pu[i];
+            a.xu[i] += al * pi;
+            if (last) return;
+            const TV beta = (TV)(st->gam[(k + 1) & 1] / st->gam[k & 1]);
+            TF z = sweep_sum<LAYOUT, 0>(a, a.sy[i], i, e0, stride, width, a.sw);
+            z = tagged<0>(z / (TF)a.d[i]);
+            publish_tagged<0>(a.sw + i, z);
+            a.pu[i] = (TV)z + beta * pi;
+            return;
+        }
+    }
     if (a.done && *a.done) return;
     if (a.part) {
         if (threadIdx.x == 0) arrived = 0u;
@@ -1107,9 +1137,12 @@ static void sweeps(Context* c, pfem_precond_t* pc, const TV* src, TV* out, const
                 lo.sy = lo.sw = up.sy = up.sw = reinterpret_cast<TF*>(out);
                 if (xr && ic && up.cg) {
                     // IC0: the forward sweep forms gamma = (y, y) and ends the iteration; the backward
-                    // sweep only solves (its partial slots and r are not touched)
+                    // sweep solves and, given o.xpx, applies the x and p updates (its partial slots and r
+                    // are not touched)
                     up.part = nullptr;
                     up.cg = nullptr;
+                    up.xu = o.xpx;
+                    up.pu = o.xpp;
                 } else {
                     lo.cg = nullptr;
                 }

@@ -280,7 +280,7 @@ struct ApplyOpts {
     const double* alpha = nullptr;
     // with part: the backward sweep publishes its partials (sentinel-armed slots) and its last workgroup
     // adds them and ends the PCG iteration cg is in (iteration cg->it, cg_record); nothing further is
-    // needed for beta
+    // needed for beta. With an IC0 factor and tagged sweeps the forward sweep does this from (y, y).
     CgState* cg = nullptr;
     // with alpha pointing at alpha_cg->alpha: the forward sweep takes alpha[it & 1] of the iteration the
     // state is in (the iteration index lives on the device, so a batch of iterations is always the same
@@ -289,6 +289,10 @@ struct ApplyOpts {
     // single-pass sliced sweeps in factor order with FP64 values: y and z share z's storage and are told
     // apart by tags (no sentinel re-arming); z must carry tag 0 on entry (+0.0 does) and leaves with tag 0
     bool tagged = false;
+    // tagged, IC0, with cg: the backward sweep also applies x += alpha p and p = z + beta p (the fused
+    // PCG then launches no separate vector kernel)
+    TV* xpx = nullptr;
+    TV* xpp = nullptr;
 };
 // z = M^-1 r with vectors of type TV. fp32 (TV = double): IC0/ILU0 with the factor stored in FP32 and the
 // solves computed in FP64. Returns

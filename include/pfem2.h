@@ -230,7 +230,8 @@ int pfem_solver_set_precision(pfem_solver s, int precision, double inner_tol, in
                                     bit instead of re-armed sentinels (y and z change by at most one unit in
                                     the last place; levels 0 to 2 give the same bits as each other), and
                                     with IC0 the forward sweep ends the iteration with gamma = (y, y),
-                                    y = L^-1 r, equal to (r, z) in exact arithmetic */
+                                    y = L^-1 r, equal to (r, z) in exact arithmetic, and the backward
+                                    sweep applies the x and p updates (three launches per iteration) */
 int pfem_solver_set_option(pfem_solver s, int option, int value);
 int pfem_solver_destroy(pfem_solver s);
 /* LinearSolver::solve(A, x, b) -> bool (linear_solver.cu:146-154, 301-376, 421-521): warm start from x,
