@@ -108,12 +108,12 @@ def roofline_ic0(ctx, st, precision):
 
 
 def pcg_iteration_bytes(n, nnz, factor_bytes):
-    """Algorithmic bytes of one fused pressure PCG iteration: the SpMV (A, p gathered, q written), the
-    forward sweep (the lower triangle with its diagonal; r read and written, q read, y written), the
-    backward sweep (the upper triangle with the diagonal; y read, z written, r read for (r, z)) and the
-    x/p update (x read and written, p read and written, z read)."""
+    """Algorithmic bytes of one fused pressure PCG iteration (three launches): the SpMV (A, p gathered, q
+    written), the forward sweep (the lower triangle with its diagonal; r read and written, q read, y
+    written; gamma = (y, y) from registers) and the backward sweep with the x/p updates (the upper
+    triangle with the diagonal; y read, z written, x and p read and written)."""
     tri = (nnz - n) // 2 * (factor_bytes + 4) + n * factor_bytes
-    return spmv_bytes(n, nnz, 8, 8) + (tri + 4 * 8 * n) + (tri + 3 * 8 * n) + 5 * 8 * n
+    return spmv_bytes(n, nnz, 8, 8) + (tri + 4 * 8 * n) + (tri + 6 * 8 * n)
 
 
 def roofline_pcg(ctx, st, precision, tol=1e-9):
@@ -145,7 +145,7 @@ def roofline_pcg(ctx, st, precision, tol=1e-9):
     del s
     return dict(iterations=stt.iterations, us_per_iteration=round(us, 2), bytes_per_iteration=b_it,
                 achieved=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 4),
-                kernels="k_cg_spmv_alpha_sliced + k_sweep_lower + k_sweep_upper + k_cg_xp_recorded")
+                kernels="k_cg_spmv_alpha_sliced + k_sweep_lower (r update, gamma) + k_sweep_upper (x, p updates)")
 
 
 def roofline_spmv(ctx, st):
