@@ -651,8 +651,8 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
     // x/r update fused into the forward sweep: multicolour IC0/ILU0, single-pass sweeps, factor ordering
     const bool fused = A.factor_order && A.pc && A.pc->sweep && A.pc->contiguous &&
                        std::max(A.pc->nuchunks, A.pc->nlchunks) <= s->pb_cap && s->fuse >= 1;
-    // four launches per iteration, the reductions finished inside the launches that produce them: SpMV
-    // (+ alpha), forward sweep (+ x/r), backward sweep (+ beta and the stopping test), p update
+    // level 2: four launches per iteration, the reductions finished inside the launches that produce
+    // them: SpMV (+ alpha), forward sweep (+ x/r), backward sweep (+ beta and the stopping test), p update
     const bool fused2 = fused && s->fuse >= 2 && P.ntiles <= s->pa_cap;
     // level 3: the sweeps share z's storage for y and z, told apart by tags instead of re-armed sentinels
     // (two 8-byte passes less per iteration; y and z change by at most one unit in the last place)
@@ -707,8 +707,8 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
     const int batch = s->poll;
     const int na = P.ntiles, nb = (int)g;
     int launched = 0;
-    // one fused iteration (four launches); the iteration index lives on the device, so every iteration is
-    // the same sequence of launches
+    // one fused iteration (four launches, three with xp_in_sweep); the iteration index lives on the
+    // device, so every iteration is the same sequence of launches
     auto fused_iteration = [&]() {
         bool sliced = false;
         if constexpr (sizeof(T) == 8) {
