@@ -143,8 +143,10 @@ def roofline_pcg(ctx, st, precision, tol=1e-9):
     b_it = pcg_iteration_bytes(n, nnz, 2 if precision == 1 else 8)
     gbs = b_it / (us * 1e-6) / 1e9
     del s
+    traffic = measured_traffic("traffic_bytes_pcg_iteration_fp16_factor" if precision == 1
+                               else "traffic_bytes_pcg_iteration_fp64")
     return dict(iterations=stt.iterations, us_per_iteration=round(us, 2), bytes_per_iteration=b_it,
-                achieved=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 4),
+                achieved=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 4), traffic_per_iteration=traffic,
                 kernels="k_cg_spmv_alpha_sliced + k_sweep_lower (r update, gamma) + k_sweep_upper (x, p updates)")
 
 
@@ -257,8 +259,8 @@ def kernels_only(h, launches):
     """Pressure-system kernels alone: the cylinder mesh's P1 stiffness matrix (the pressure matrix:
     it depends on the geometry only) with the pressure Dirichlet rows, its multicolour IC0 factor, then
     `launches` IC0 applies on vectors in the factor's ordering (16-bit, FP32 and FP64 factor), FP64 SpMVs
-    in the natural and in the factor's ordering, and device-to-device copies of 256 MiB (the calibration
-    of the byte counters)."""
+    in the natural and in the factor's ordering, `launches` iterations of the fused PCG with each factor
+    (16-bit and FP64), and device-to-device copies of 256 MiB (the calibration of the byte counters)."""
     import numpy as np
     from gpupfem2_amd import capi, cases
     case = cases.cylinder_case(h=h)
@@ -282,6 +284,20 @@ def kernels_only(h, launches):
         ms = capi.time_kernel(ctx, op, a.h, p.h if p else None, launches=launches)
         res[name] = round(ms * 1e3, 2)
         print(f"[kernels] {name}: {ms * 1e3:.2f} us", file=sys.stderr, flush=True)
+    # the fused PCG's own kernels: `launches` iterations of a pressure solve per factor (16-bit copy with
+    # the mixed step's settings, FP64), on a seeded right-hand side that does not converge in so few
+    b = ctx.to_device(np.random.default_rng(1).standard_normal(n))
+    for name, p, mixed in (("pcg_fp16_factor", pc16, True), ("pcg_fp64", pc, False)):
+        s = capi.Solver(a, capi.SOLVER_CG, 1e-30, launches, 0, capi.TOL_RELATIVE, p)
+        if mixed:
+            s.set_precision(capi.PRECISION_MIXED_PRECOND, 1e-5, 0)
+        x = ctx.zeros(n)
+        st = s.solve(a, x, b)
+        res[name + "_iterations"] = st.iterations
+        print(f"[kernels] {name}: {st.iterations} iterations", file=sys.stderr, flush=True)
+        del s
+    res["pcg_iteration_bytes_fp16_factor"] = pcg_iteration_bytes(n, nnz, 2)
+    res["pcg_iteration_bytes_fp64"] = pcg_iteration_bytes(n, nnz, 8)
     mib = 256
     src, dst = ctx.zeros(mib << 17), ctx.empty(mib << 17)
     for _ in range(launches):
