@@ -148,6 +148,7 @@ SIGNATURES = {
     "pfem_pfem2_matrix": (_i, [_vp, _i, _pp]),
     "pfem_pfem2_precond": (_i, [_vp, _i, _pp]),
     "pfem_time_kernel": (_i, [_i, _vp, _vp, _i, C.POINTER(_d)]),
+    "pfem_pfem2_sweep_fallbacks": (_i, [_vp]),
     "pfem_export_vtk": (_i, [_vp, C.c_char_p, _i, _vp, _vp]),
 }
 
@@ -669,6 +670,10 @@ class Pfem2(_Handle):
         ms = np.zeros(8, np.float64)
         lib().pfem_pfem2_phase_times(self.h, _ptr(ms), 8)
         return ms
+
+    def sweep_fallbacks(self):
+        """solves re-run with per-level sweeps after a single-pass wait exceeded its spin limit"""
+        return lib().pfem_pfem2_sweep_fallbacks(self.h)
 
     def matrix_handle(self, which):
         """raw pfem_csr of the stepper: 0 prediction x, 1 prediction y, 2 pressure, 3 correction"""

@@ -3,6 +3,7 @@
 // All systems share the mesh's CSR pattern; the two correction matrices of the reference are one matrix
 // with two right-hand sides (they are identical, :248-251), solved together as a multi-RHS PCG in mixed
 // precision when the correction solver is Jacobi PCG.
+#include <cstdio>
 #include <cstring>
 
 #include "device.hpp"
@@ -36,6 +37,8 @@ struct pfem_pfem2_t {
     pfem_precond pc_pred = nullptr, pc_pres = nullptr, pc_corr = nullptr;
     pfem_solver s_pred = nullptr, s_pres = nullptr, s_corr = nullptr;
     pfem::DevArray<double> f;   // u0 u1 p vp0 vp1 uo0 uo1 pold rhs0 rhs1 (n each) + 4 forces
+    pfem::DevArray<double> xsave;   // a single-pass solve's start vector, kept for a per-level re-solve
+    int sweep_fallbacks = 0;
     int n = 0;
     std::vector<int> its;
     bool timing = false;
@@ -99,10 +102,35 @@ void make_solver(pfem_pfem2_t* s, pfem_csr a, int kind, int pc_kind, int orderin
     }
 }
 
+// A solve whose IC0/ILU0 preconditioner runs single-pass sweeps is solved again from the same start with
+// per-level sweeps (FP64 factor) when a sweep's wait exceeded its spin limit, instead of failing the
+// step; the fallback is counted and reported on stderr.
 void solve(pfem_pfem2_t* s, pfem_solver sv, pfem_csr a, double* x, const double* b)
 {
     pfem_solve_stats st{};
-    check(pfem_solver_solve(sv, a, x, b, &st), s->ctx);
+    pfem_precond_t* pc = sv->pc;
+    const bool sweeps = pc && (pc->kind == PFEM_PRECOND_IC0 || pc->kind == PFEM_PRECOND_ILU0) && pc->sweep;
+    if (sweeps) {
+        if (s->xsave.n < (size_t)s->n) s->xsave.alloc(s->n);
+        launch_copy(C(s->ctx), s->xsave.p, x, s->n);
+    }
+    const int failures = pc ? pc->sweep_failures : 0;
+    const int rc = pfem_solver_solve(sv, a, x, b, &st);
+    if (rc != PFEM_OK && sweeps && pc->sweep_failures > failures) {
+        ++s->sweep_fallbacks;
+        std::fprintf(stderr, "pfem_pfem2_step: a single-pass sweep wait exceeded its spin limit; solving again "
+                             "with per-level sweeps (fallback %d)\n", s->sweep_fallbacks);
+        launch_copy(C(s->ctx), x, s->xsave.p, s->n);
+        const int precision = sv->precision;
+        check(pfem_precond_set_sweep(pc, 0), s->ctx);
+        if (precision == PFEM_PRECISION_MIXED_PRECOND) check(pfem_solver_set_precision(sv, PFEM_PRECISION_FP64, 0, 0), s->ctx);
+        const int rc2 = pfem_solver_solve(sv, a, x, b, &st);
+        check(pfem_precond_set_sweep(pc, 1), s->ctx);
+        if (precision == PFEM_PRECISION_MIXED_PRECOND) check(pfem_solver_set_precision(sv, precision, 0, 0), s->ctx);
+        check(rc2, s->ctx);
+    } else {
+        check(rc, s->ctx);
+    }
     s->its.push_back(st.iterations);
 }
 }  // namespace
@@ -281,6 +309,11 @@ int pfem_pfem2_phase_times(pfem_pfem2 s, double* ms, int cap)
     if (!s || !ms) return PFEM_EINVAL;
     for (int k = 0; k < std::min(cap, 8); ++k) ms[k] = s->phase_ms[k];
     return PFEM_OK;
+}
+
+int pfem_pfem2_sweep_fallbacks(pfem_pfem2 s)
+{
+    return s ? s->sweep_fallbacks : PFEM_EINVAL;
 }
 
 int pfem_pfem2_matrix(pfem_pfem2 s, int which, pfem_csr* out)

@@ -1292,6 +1292,7 @@ void precond_check(Context* c, pfem_precond_t* pc)
         hipLaunchKernelGGL((k_fill_sentinel<float>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, pc->sw32.p);
     }
     PFEM_HIP(hipStreamSynchronize(c->stream));
+    ++pc->sweep_failures;
     throw Error(PFEM_ENUMERIC, "triangular sweep: a wait exceeded its spin limit");
 }
 

@@ -200,6 +200,7 @@ struct pfem_precond_t {
     pfem::DevArray<double> sy, sw;
     pfem::DevArray<float> sy32, sw32;
     pfem::DevArray<int> sweep_err;        // set when a wait exceeded its spin limit
+    int sweep_failures = 0;               // solves reported failed by a wait past its spin limit
     // the sweeps' copy of the factor: all strictly lower entries (row by row), then all strictly upper
     // entries, then the n diagonal values, each row's entries in the factor's order; split[k] takes
     // fac[split_pos[k]]. split_ptr holds the lower row offsets (n + 1), then the upper ones (n + 1).
@@ -242,7 +243,8 @@ struct pfem_solver_t {
     int pb_cap = 0, pa_cap = 0;
     int factor_order = 1;   // run multicolour IC0/ILU0 solves in the factor's ordering
     // PCG kernel fusion in the factor's ordering: 0 none; 1 x/r update inside the forward sweep; 2 also
-    // alpha and beta finished by the last workgroups of the SpMV and of the backward sweep
+    // alpha and beta finished by the last workgroups of the SpMV and of the backward sweep; 3 tagged
+    // sweeps (IC0: three launches per iteration)
     int fuse = 3;
     int graph = 0;                        // fused PCG: replay a captured batch of iterations
     int replace_digits = 2;   // reliable updates (multi/batched MIXED): replace after gamma fell 10^-digits

@@ -377,6 +377,10 @@ int pfem_pfem2_last_iterations(pfem_pfem2 s, int* its, int cap);
  * 6 particle correction + loads. Timing is off unless enabled (it synchronises at phase ends). */
 int pfem_pfem2_enable_timing(pfem_pfem2 s, int on);
 int pfem_pfem2_phase_times(pfem_pfem2 s, double* ms, int cap);
+/* Solves re-run since creation because a single-pass triangular sweep's wait exceeded its spin limit (no
+ * counterpart in the reference, whose SpSV has no such wait): such a solve is repeated from the same start
+ * with per-level sweeps and the FP64 factor instead of failing the step, and a line goes to stderr. */
+int pfem_pfem2_sweep_fallbacks(pfem_pfem2 s);
 /* the stepper's matrices (borrowed, owned by s): 0 prediction x, 1 prediction y, 2 pressure,
  * 3 correction; and its preconditioners: 0 prediction, 1 pressure, 2 correction */
 int pfem_pfem2_matrix(pfem_pfem2 s, int which, pfem_csr* out);
