@@ -40,8 +40,12 @@ void add_local(const int* rowptr, const int* col, double* val, double* rhs, cons
     for (int i = 0; i < 3; ++i) {
         for (int j = 0; j < 3; ++j) {
             const int s = slot(rowptr, col, v[i], v[j]);
-            if (s >= 0) val[s] += M[i][j] * area;
+            if (s >= 0) {
+#pragma omp atomic
+                val[s] += M[i][j] * area;
+            }
         }
+#pragma omp atomic
         rhs[v[i]] += R[i] * area;
     }
 }
@@ -91,6 +95,7 @@ void orc_assemble_poisson2d(int nv, const double* xy, int nt, const int* tri, co
     (void)nv;
     const Quad2 q = face(quad_index);
     const double PI = 3.14159265358979323846;
+#pragma omp parallel for schedule(static)
     for (int c = 0; c < nt; ++c) {
         const int* v = tri + 3 * c;
         const double* J = invJ + 4 * c;
@@ -132,6 +137,7 @@ void orc_assemble_poisson3d(int nv, const double* p, int nt, const int* tet, con
     const int nq = orc_quadrature(3, quad_index, buf);
     const double PI = 3.14159265358979323846;
     const double g3[4][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, -1, -1}};
+#pragma omp parallel for schedule(static)
     for (int c = 0; c < nt; ++c) {
         const int* v = tet + 4 * c;
         const double* J = invJ + 9 * c;
@@ -162,8 +168,12 @@ void orc_assemble_poisson3d(int nv, const double* p, int nt, const int* tet, con
         for (int i = 0; i < 4; ++i) {
             for (int j = 0; j < 4; ++j) {
                 const int s = slot(rowptr, col, v[i], v[j]);
-                if (s >= 0) val[s] += M[i][j] * vol[c];
+                if (s >= 0) {
+#pragma omp atomic
+                    val[s] += M[i][j] * vol[c];
+                }
             }
+#pragma omp atomic
             rhs[v[i]] += R[i] * vol[c];
         }
     }
@@ -184,6 +194,7 @@ void orc_assemble_prediction(int nv, const double* xy, int nt, const int* tri, c
     const Quad1 e = edge(prm->edge_quad);
     const double rho = prm->rho, mu = prm->mu, dt = prm->dt;
     const double* uold[2] = {uo0, uo1};
+#pragma omp parallel for schedule(static)
     for (int c = 0; c < nt; ++c) {
         const int* v = tri + 3 * c;
         const double* J = invJ + 4 * c;
@@ -255,6 +266,7 @@ void orc_assemble_pressure(int nv, int nt, const int* tri, const double* area, c
 {
     (void)nv;
     const Quad2 q = face(prm->face_quad);
+#pragma omp parallel for schedule(static)
     for (int c = 0; c < nt; ++c) {
         const int* v = tri + 3 * c;
         const double* J = invJ + 4 * c;
@@ -292,6 +304,7 @@ void orc_assemble_correction(int nv, int nt, const int* tri, const double* area,
     const Quad2 q = face(prm->face_quad);
     const double* vp[2] = {vp0, vp1};
     std::vector<double> dummy(nv > 0 ? nv : 1);
+#pragma omp parallel for schedule(static)
     for (int c = 0; c < nt; ++c) {
         const int* v = tri + 3 * c;
         const double* J = invJ + 4 * c;
@@ -318,9 +331,14 @@ void orc_assemble_correction(int nv, int nt, const int* tri, const double* area,
         for (int i = 0; i < 3; ++i) {
             for (int j = 0; j < 3; ++j) {
                 const int s = slot(rowptr, col, v[i], v[j]);
-                if (s >= 0) val[s] += M[i][j] * area[c];
+                if (s >= 0) {
+#pragma omp atomic
+                    val[s] += M[i][j] * area[c];
+                }
             }
+#pragma omp atomic
             rhs0[v[i]] += R[0][i] * area[c];
+#pragma omp atomic
             rhs1[v[i]] += R[1][i] * area[c];
         }
     }
@@ -337,6 +355,7 @@ void orc_prediction_bc_values(int nv, int nt, const int* tri, const double* area
     std::vector<int> map(nv, -1);
     for (int k = 0; k < nbc; ++k) map[bc_node[k]] = k;
     std::vector<double> num(nbc, 0.0), den(nbc, 0.0);
+#pragma omp parallel for schedule(static)
     for (int c = 0; c < nt; ++c) {
         const int* v = tri + 3 * c;
         if (map[v[0]] == -1 && map[v[1]] == -1 && map[v[2]] == -1) continue;
@@ -350,7 +369,12 @@ void orc_prediction_bc_values(int nv, int nt, const int* tri, const double* area
         const double gc = component == 0 ? g.x : g.y;
         for (int i = 0; i < 3; ++i) {
             const int k = map[v[i]];
-            if (k != -1) { num[k] += area[c] * gc; den[k] += area[c]; }
+            if (k != -1) {
+#pragma omp atomic
+                num[k] += area[c] * gc;
+#pragma omp atomic
+                den[k] += area[c];
+            }
         }
     }
     for (int k = 0; k < nbc; ++k) bc_out[k] = bc_src[k] + prm->dt / prm->rho * num[k] / den[k];

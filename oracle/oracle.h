@@ -58,6 +58,10 @@ typedef struct {
 /* precond: 0 none, 1 Jacobi, 2 IC0, 3 ILU0; tol_mode: 0 reference absolute, 1 relative */
 int orc_pcg(int n, const int* rowptr, const int* col, const double* val, double* x, const double* b,
             double tol, int maxit, int tol_mode, int precond, orc_solve_stats* st);
+/* PCG + IC0 on a multicolour-ordered matrix (rows of colour c: [cptr[c], cptr[c+1]), no entries between two
+ * rows of one colour), colours processed in sequence and the rows of one colour in parallel (OpenMP build) */
+int orc_pcg_coloured(int n, const int* rowptr, const int* col, const double* val, double* x, const double* b,
+                     double tol, int maxit, int tol_mode, int ncol, const int* cptr, orc_solve_stats* st);
 int orc_gmres(int n, const int* rowptr, const int* col, const double* val, double* x, const double* b,
               double tol, int maxit, int restart, int tol_mode, int precond, orc_solve_stats* st);
 
@@ -139,6 +143,9 @@ orc_pfem2* orc_pfem2_create(int nv, const double* xy, int nt, const int* tri, co
                             int nbcp, const int* bcp_node, const double* bcp_val);
 void orc_pfem2_destroy(orc_pfem2* s);
 void orc_pfem2_step(orc_pfem2* s, double forces[2]);
+/* on != 0: an IC0-PCG pressure solve runs on the pressure system permuted by a greedy colouring of the
+ * mesh's CSR pattern (natural visiting order), with orc_pcg_coloured (the CPU baseline's form) */
+void orc_pfem2_set_coloured_pressure(orc_pfem2* s, int on);
 void orc_pfem2_fields(const orc_pfem2* s, double* u0, double* u1, double* p, double* vp0, double* vp1);
 void orc_pfem2_set_fields(orc_pfem2* s, const double* u0, const double* u1, const double* p,
                           const double* vp0, const double* vp1);

@@ -14,6 +14,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
+_BUILD = "liboracle.so"
 
 D = np.float64
 I = np.int32
@@ -22,12 +23,22 @@ I = np.int32
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        path = os.path.join(_HERE, _BUILD)
         if not os.path.exists(path):
-            raise RuntimeError("oracle/liboracle.so is not built (run `make -C oracle`)")
+            raise RuntimeError(f"oracle/{_BUILD} is not built (run `make -C oracle`)")
         _LIB = C.CDLL(path)
         _declare(_LIB)
     return _LIB
+
+
+def use_openmp_build():
+    """Load oracle/liboracle_omp.so in this process instead of the parity build: the same sources with
+    -O3 -fopenmp (per-thread-chunk reductions, atomic scatters), for bench.py's CPU baseline only. Must be
+    called before the first lib()."""
+    global _BUILD
+    if _LIB is not None and _BUILD != "liboracle_omp.so":
+        raise RuntimeError("the parity oracle is already loaded in this process")
+    _BUILD = "liboracle_omp.so"
 
 
 def _p(a):
@@ -64,6 +75,8 @@ def _declare(L):
     vp, i, d, l, u64 = C.c_void_p, C.c_int, C.c_double, C.c_long, C.c_uint64
     sig = {
         "orc_geometry2d": (None, [i, vp, i, vp, vp, vp]),
+        "orc_pcg_coloured": (i, [i, vp, vp, vp, vp, vp, d, i, i, i, vp, vp]),
+        "orc_pfem2_set_coloured_pressure": (None, [vp, i]),
         "orc_geometry3d": (None, [i, vp, i, vp, vp, vp]),
         "orc_csr_count": (l, [i, i, i, vp]),
         "orc_csr_fill": (None, [i, i, i, vp, vp, vp]),
@@ -466,6 +479,9 @@ class Pfem2:
         out = [np.empty(self.nv, D) for _ in range(5)]
         lib().orc_pfem2_fields(self.h, *[_p(a) for a in out])
         return dict(zip(["u0", "u1", "p", "vp0", "vp1"], out))
+
+    def set_coloured_pressure(self, on=True):
+        lib().orc_pfem2_set_coloured_pressure(self.h, int(on))
 
     def set_fields(self, u0=None, u1=None, p=None, vp0=None, vp1=None):
         lib().orc_pfem2_set_fields(self.h, _p(u0), _p(u1), _p(p), _p(vp0), _p(vp1))

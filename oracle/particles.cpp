@@ -182,6 +182,7 @@ void orc_ph_seed(orc_particles* ph, int mode, int ppc, uint64_t seed)
 void orc_ph_init_velocity(orc_particles* ph, const double* u0, const double* u1)
 {
     const size_t P = ph->cell.size();
+#pragma omp parallel for schedule(static)
     for (size_t p = 0; p < P; ++p) {
         const int* t = &ph->tri[3 * (size_t)ph->cell[p]];
         double ix = 0.0, iy = 0.0;
@@ -237,6 +238,7 @@ void orc_ph_advect(orc_particles* ph, const double* u0, const double* u1, double
     ph->deleted_last = 0;
     for (int s = 0; s < substeps; ++s) {
         const size_t P = ph->cell.size();
+#pragma omp parallel for schedule(static)
         for (size_t p = 0; p < P; ++p) {
             const int* t = &ph->tri[3 * (size_t)ph->cell[p]];
             double ax = 0.0, ay = 0.0;
@@ -248,6 +250,8 @@ void orc_ph_advect(orc_particles* ph, const double* u0, const double* u1, double
             ph->pos[2 * p + 1] = ph->pos[2 * p + 1] + dts * ay;
         }
         std::vector<char> keep(P, 1);
+        long deleted = 0;
+#pragma omp parallel for schedule(static) reduction(+ : deleted)
         for (size_t p = 0; p < P; ++p) {
             double L[3];
             const int c = ph->cell[p];
@@ -265,16 +269,21 @@ void orc_ph_advect(orc_particles* ph, const double* u0, const double* u1, double
                     break;
                 }
             }
-            if (!found) { keep[p] = 0; ph->deleted_last++; }
+            if (!found) { keep[p] = 0; ++deleted; }
         }
+        ph->deleted_last += deleted;
         compact(ph, keep);
     }
     // census and refill
     const int L = ph->level, ns = L * L;
     std::vector<int> count((size_t)ph->nt * ns, 0);
     const size_t P = ph->cell.size();
-    for (size_t p = 0; p < P; ++p)
-        count[(size_t)ph->cell[p] * ns + orc_determine_subcell(L, ph->lam[3 * p], ph->lam[3 * p + 1], ph->lam[3 * p + 2])]++;
+#pragma omp parallel for schedule(static)
+    for (size_t p = 0; p < P; ++p) {
+        const size_t k = (size_t)ph->cell[p] * ns + orc_determine_subcell(L, ph->lam[3 * p], ph->lam[3 * p + 1], ph->lam[3 * p + 2]);
+#pragma omp atomic
+        count[k]++;
+    }
     ph->added_last = 0;
     for (int c = 0; c < ph->nt; ++c) {
         const int* t = &ph->tri[3 * (size_t)c];
@@ -293,12 +302,14 @@ void orc_ph_advect(orc_particles* ph, const double* u0, const double* u1, double
             ph->added_last++;
         }
     }
-    // stable order by cell
+    // stable order by cell (a counting sort)
     const size_t Q = ph->cell.size();
-    std::vector<size_t> perm(Q);
-    std::iota(perm.begin(), perm.end(), 0);
-    std::stable_sort(perm.begin(), perm.end(), [&](size_t a, size_t b) { return ph->cell[a] < ph->cell[b]; });
+    std::vector<size_t> perm(Q), start((size_t)ph->nt + 1, 0);
+    for (size_t p = 0; p < Q; ++p) start[(size_t)ph->cell[p] + 1]++;
+    for (int c = 0; c < ph->nt; ++c) start[c + 1] += start[c];
+    for (size_t p = 0; p < Q; ++p) perm[start[ph->cell[p]]++] = p;
     std::vector<int> c2(Q); std::vector<double> l2(3 * Q), x2(2 * Q), v2(2 * Q); std::vector<uint32_t> i2(Q);
+#pragma omp parallel for schedule(static)
     for (size_t k = 0; k < Q; ++k) {
         const size_t p = perm[k];
         c2[k] = ph->cell[p]; i2[k] = ph->id[p];
@@ -314,15 +325,20 @@ void orc_ph_project(orc_particles* ph, double* u0, double* u1)
 {
     const size_t P = ph->cell.size();
     std::vector<double> part(9 * (size_t)ph->nt, 0.0);
+#pragma omp parallel for schedule(static)
     for (size_t p = 0; p < P; ++p) {
         double* s = &part[9 * (size_t)ph->cell[p]];
         for (int i = 0; i < 3; ++i) {
             const double l = ph->lam[3 * p + i];
+#pragma omp atomic
             s[3 * i + 0] += l * ph->vel[2 * p];
+#pragma omp atomic
             s[3 * i + 1] += l * ph->vel[2 * p + 1];
+#pragma omp atomic
             s[3 * i + 2] += l;
         }
     }
+#pragma omp parallel for schedule(static)
     for (int v = 0; v < ph->nv; ++v) {
         double sx = 0.0, sy = 0.0, w = 0.0;
         for (int k = ph->inc_off[v]; k < ph->inc_off[v + 1]; ++k) {

@@ -3,7 +3,9 @@
 // parameters.
 #include "oracle.h"
 
+#include <algorithm>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #ifdef _OPENMP
@@ -21,7 +23,69 @@ struct orc_pfem2 {
     std::vector<double> u[2], uo[2], vp[2], p, pold;
     std::vector<int> its;
     int nbody = 0;
+    // multicolour pressure solve (orc_pfem2_set_coloured_pressure): new -> old rows, colour offsets, the
+    // permuted pattern and the old CSR slot of each permuted slot
+    bool coloured = false;
+    std::vector<int> cperm, cptr, crowptr, ccol, csrc;
 };
+
+// Greedy colouring of the pattern's graph in natural order, rows ordered by (colour, index).
+static void build_colouring(orc_pfem2* s)
+{
+    const int n = s->nv;
+    std::vector<int> colour(n, -1), mark;
+    int ncol = 0;
+    for (int i = 0; i < n; ++i) {
+        mark.assign(ncol + 1, 0);
+        for (int k = s->rowptr[i]; k < s->rowptr[i + 1]; ++k)
+            if (s->col[k] != i && colour[s->col[k]] >= 0) mark[colour[s->col[k]]] = 1;
+        int c = 0;
+        while (c < ncol && mark[c]) ++c;
+        colour[i] = c;
+        if (c == ncol) ++ncol;
+    }
+    s->cptr.assign(ncol + 1, 0);
+    for (int i = 0; i < n; ++i) s->cptr[colour[i] + 1]++;
+    for (int c = 0; c < ncol; ++c) s->cptr[c + 1] += s->cptr[c];
+    std::vector<int> fill(s->cptr.begin(), s->cptr.end() - 1), iperm(n);
+    s->cperm.resize(n);
+    for (int i = 0; i < n; ++i) {
+        const int q = fill[colour[i]]++;
+        s->cperm[q] = i;
+        iperm[i] = q;
+    }
+    s->crowptr.assign(n + 1, 0);
+    s->ccol.clear();
+    s->csrc.clear();
+    std::vector<std::pair<int, int>> row;
+    for (int q = 0; q < n; ++q) {
+        const int o = s->cperm[q];
+        row.clear();
+        for (int k = s->rowptr[o]; k < s->rowptr[o + 1]; ++k) row.emplace_back(iperm[s->col[k]], k);
+        std::sort(row.begin(), row.end());
+        for (const auto& e : row) { s->ccol.push_back(e.first); s->csrc.push_back(e.second); }
+        s->crowptr[q + 1] = (int)s->ccol.size();
+    }
+}
+
+static int solve_coloured(orc_pfem2* s, const std::vector<double>& val, std::vector<double>& x,
+                          const std::vector<double>& b)
+{
+    if (s->cperm.empty()) build_colouring(s);
+    const int n = s->nv, nnz = (int)s->ccol.size();
+    std::vector<double> v2(nnz), x2(n), b2(n);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nnz; ++k) v2[k] = val[s->csrc[k]];
+#pragma omp parallel for schedule(static)
+    for (int q = 0; q < n; ++q) { x2[q] = x[s->cperm[q]]; b2[q] = b[s->cperm[q]]; }
+    orc_solve_stats st;
+    const orc_step_config& c = s->cfg;
+    orc_pcg_coloured(n, s->crowptr.data(), s->ccol.data(), v2.data(), x2.data(), b2.data(), c.tol, c.maxit,
+                     c.tol_mode, (int)s->cptr.size() - 1, s->cptr.data(), &st);
+#pragma omp parallel for schedule(static)
+    for (int q = 0; q < n; ++q) x[s->cperm[q]] = x2[q];
+    return st.iterations;
+}
 
 static int solve(const orc_pfem2* s, int solver, int precond, const std::vector<double>& val,
                  std::vector<double>& x, const std::vector<double>& b)
@@ -73,6 +137,8 @@ orc_pfem2* orc_pfem2_create(int nv, const double* xy, int nt, const int* tri, co
 
 void orc_pfem2_destroy(orc_pfem2* s) { delete s; }
 
+void orc_pfem2_set_coloured_pressure(orc_pfem2* s, int on) { s->coloured = on != 0; }
+
 void orc_pfem2_step(orc_pfem2* s, double forces[2])
 {
     const orc_step_config& c = s->cfg;
@@ -112,7 +178,10 @@ void orc_pfem2_step(orc_pfem2* s, double forces[2])
                               s->col.data(), val0.data(), rhs0.data(), s->vp[0].data(), s->vp[1].data(), pold, prm);
         orc_dirichlet_apply(nv, s->rowptr.data(), s->col.data(), val0.data(), rhs0.data(),
                             (int)s->bcp_node.size(), s->bcp_node.data(), s->bcp_val.data(), c.dirichlet_mode);
-        s->its.push_back(solve(s, c.pres_solver, c.pres_precond, val0, s->p, rhs0));
+        if (s->coloured && c.pres_solver == 0 && c.pres_precond == 2)
+            s->its.push_back(solve_coloured(s, val0, s->p, rhs0));
+        else
+            s->its.push_back(solve(s, c.pres_solver, c.pres_precond, val0, s->p, rhs0));
         // correction: one matrix, two right-hand sides; each component applies its own BC values to its
         // own copy of the matrix exactly as the reference does with its two matrices
         std::fill(val0.begin(), val0.end(), 0.0);

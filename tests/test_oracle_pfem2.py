@@ -219,3 +219,44 @@ def test_poiseuille_steady_state(oracle):
     prof = np.mean(prof, axis=0)
     assert np.abs(prof - 4.0 * levels * (1.0 - levels)).max() < 0.02, prof
     assert abs(np.mean(slopes) - (-8 * case.mu * 1.0)) < 0.2 * 8 * case.mu, np.mean(slopes)
+
+
+_OMP_STEP = r"""
+import sys, numpy as np
+sys.path.insert(0, sys.argv[1])
+from oracle import oracle as O
+O.use_openmp_build()
+from gpupfem2_amd import cases
+sys.path.insert(0, sys.argv[1] + "/tests")
+from helpers import oracle_stepper
+case = cases.cylinder_case(h=1.0 / 16, ppc=6, dt=2e-3)
+ph, st = oracle_stepper(O, case)
+st.set_coloured_pressure(True)
+f = [st.step() for _ in range(2)]
+np.savez(sys.argv[2], forces=np.array(f), its=st.iterations(), **st.fields())
+"""
+
+
+def test_openmp_build_coloured_step(oracle, tmp_path):
+    """The OpenMP perf build of the oracle (bench.py's CPU baseline), with its multicolour IC0 pressure
+    solve, run in a child process on the h = 1/16 cylinder: two steps agree with the parity build's
+    natural-order steps to the solver tolerance (fields within relative L2 1e-8, C_D and C_L within 1e-6)."""
+    import os
+    import subprocess
+    import sys
+    from gpupfem2_amd import cases
+    from helpers import oracle_stepper
+    out = tmp_path / "omp.npz"
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    subprocess.run([sys.executable, "-c", _OMP_STEP, root, str(out)], check=True, timeout=600,
+                   env=dict(os.environ, OMP_NUM_THREADS="4"))
+    omp = np.load(out)
+    case = cases.cylinder_case(h=1.0 / 16, ppc=6, dt=2e-3)
+    ph, st = oracle_stepper(oracle, case)
+    f = np.array([st.step() for _ in range(2)])
+    ref = st.fields()
+    for k in ("u0", "u1", "p"):
+        err = np.linalg.norm(omp[k] - ref[k]) / np.linalg.norm(ref[k])
+        assert err < 1e-8, (k, err)
+    assert np.all(np.abs(omp["forces"] - f) <= 1e-6 * np.abs(f) + 1e-12), (omp["forces"], f)
+    assert omp["its"][2] > 0
