@@ -127,6 +127,7 @@ SIGNATURES = {
     "pfem_assemble_correction": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(Pfem2Params)]),
     "pfem_prediction_bc_values": (_i, [_vp, _vp, _vp, _vp, _vp, _i, C.POINTER(Pfem2Params)]),
     "pfem_body_forces": (_i, [_vp, _i, _d, _d, _vp, _vp, _vp, C.POINTER(Pfem2Params), _vp]),
+    "pfem_body_forces_noalloc": (_i, [_vp, _i, _d, _d, _vp, _vp, _vp, C.POINTER(Pfem2Params), _vp]),
     "pfem_particles_create": (_i, [_vp, _i, _pp]),
     "pfem_particles_destroy": (_i, [_vp]),
     "pfem_particles_seed": (_i, [_vp, _i, _i, C.c_uint64]),
@@ -560,6 +561,14 @@ def body_forces(mesh, body_id, center, u0, u1, p, prm):
     out = np.zeros(4, np.float64)
     mesh.ctx.check(lib().pfem_body_forces(mesh.h, body_id, center[0], center[1], u0.ptr, u1.ptr, p.ptr,
                                           C.byref(prm), _ptr(out)))
+    return out
+
+
+def body_forces_noalloc(mesh, body_id, center, u0, u1, p, prm):
+    """pfem_body_forces without device allocation (one static scratch array per GPU)"""
+    out = np.zeros(4, np.float64)
+    mesh.ctx.check(lib().pfem_body_forces_noalloc(mesh.h, body_id, center[0], center[1], u0.ptr, u1.ptr, p.ptr,
+                                                  C.byref(prm), _ptr(out)))
     return out
 
 

@@ -10,6 +10,8 @@
 // order of a sequential element loop.
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <vector>
 
 #include "device.hpp"
 #include "internal.hpp"
@@ -479,15 +481,12 @@ __global__ void k_prediction_bc(int nbc, const int* __restrict__ node, const int
 }
 
 // ------------------------------------------------------------------ body loads (Cylinder2D/main.cu:322-376)
-__global__ void k_body_edge_loads(int ne, const int* __restrict__ cell_edge, const double* __restrict__ xy,
-                                  const int* __restrict__ tri, const double4* __restrict__ invJ,
-                                  const double* __restrict__ u0, const double* __restrict__ u1,
-                                  const double* __restrict__ p, double cx, double cy, double mu, Quad1 qe,
-                                  double* __restrict__ loads)
+// The loads {pressure x, pressure y, viscous x, viscous y} of edge ed of cell c into l.
+__device__ __forceinline__ void body_edge_load(int c, int ed, const double* __restrict__ xy, const int* __restrict__ tri,
+                                               const double4* __restrict__ invJ, const double* __restrict__ u0,
+                                               const double* __restrict__ u1, const double* __restrict__ p, double cx,
+                                               double cy, double mu, const Quad1& qe, double (&l)[4])
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= ne) return;
-    const int c = cell_edge[2 * k], ed = cell_edge[2 * k + 1];
     const int v[3] = {tri[3 * c], tri[3 * c + 1], tri[3 * c + 2]};
     const double4 J = invJ[c];
     const int a = v[ed], b = v[(ed + 1) % 3];
@@ -498,7 +497,7 @@ __global__ void k_body_edge_loads(int ne, const int* __restrict__ cell_edge, con
     const double halfLength = 0.5 * sqrt((sx - ex) * (sx - ex) + (sy - ey) * (sy - ey));
     const double tx = ny, ty = -nx;
     const double v3x = xy[2 * v[2]], v3y = xy[2 * v[2] + 1];
-    double l[4] = {0, 0, 0, 0};
+    l[0] = l[1] = l[2] = l[3] = 0.0;
     for (int qp = 0; qp < qe.n; ++qp) {
         double pq = 0.0, dvtdn = 0.0;
         const double cd = qe.x[qp];
@@ -520,6 +519,18 @@ __global__ void k_body_edge_loads(int ne, const int* __restrict__ cell_edge, con
         l[2] += mu * dvtdn * tx * weight;
         l[3] += mu * dvtdn * ty * weight;
     }
+}
+
+__global__ void k_body_edge_loads(int ne, const int* __restrict__ cell_edge, const double* __restrict__ xy,
+                                  const int* __restrict__ tri, const double4* __restrict__ invJ,
+                                  const double* __restrict__ u0, const double* __restrict__ u1,
+                                  const double* __restrict__ p, double cx, double cy, double mu, Quad1 qe,
+                                  double* __restrict__ loads)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ne) return;
+    double l[4];
+    body_edge_load(cell_edge[2 * k], cell_edge[2 * k + 1], xy, tri, invJ, u0, u1, p, cx, cy, mu, qe, l);
     for (int d = 0; d < 4; ++d) loads[4 * (size_t)k + d] = l[d];
 }
 
@@ -628,6 +639,124 @@ void body_forces(pfem_mesh2d m, int body_id, double cx, double cy, const double*
     PFEM_HIP(hipGetLastError());
 }
 
+// ---- allocation-free body forces (pfem_body_forces_noalloc)
+//
+// The body's edges are found on the device instead of from the host copy of the edge IDs: a count of body
+// cells per workgroup, an exclusive scan of the counts, and a pass in which each body cell writes its
+// edge's loads at its rank (ascending cells, as the cached edge list of body_forces orders them); one lane
+// then adds the loads in that order, so the totals are bit-identical to body_forces'. Everything lives in
+// one static device array per GPU (about 2.4 MB): block counts, {edge count, overflow flag}, the four
+// totals and the per-edge loads. Concurrent calls on one GPU share it and race.
+constexpr int kNaMaxBlocks = 65536;   // cells <= 65536 * 256
+constexpr int kNaMaxEdges = 65536;    // body edges
+constexpr long kNaStatus = kNaMaxBlocks / 2, kNaTotals = kNaStatus + 2, kNaLoads = kNaTotals + 4;
+static __device__ __align__(256) double scratch[kNaLoads + 4L * kNaMaxEdges];
+
+int noalloc_init(Context* c)
+{
+    // one lookup per device; every context on that device shares the address
+    static std::mutex mu;
+    static std::vector<double*> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)addr.size() <= c->device) addr.resize(c->device + 1, nullptr);
+    if (!addr[c->device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return (int)e;
+        addr[c->device] = static_cast<double*>(p);
+    }
+    c->scratch = addr[c->device];
+    return (int)hipSuccess;
+}
+
+// the edge of cell c on the body: the first of its edges with that ID (kCountBodyEdges,
+// Cylinder2D/main.cu:299-320), or -1
+__device__ __forceinline__ int body_edge_of(int c, const int* __restrict__ edge_ids, int body_id)
+{
+    for (int i = 0; i < 3; ++i)
+        if (edge_ids[3 * c + i] == body_id) return i;
+    return -1;
+}
+
+// body cells among the 256 cells of each workgroup
+__global__ void k_na_count(int nt, const int* __restrict__ edge_ids, int body_id, int* __restrict__ counts)
+{
+    __shared__ int wc[kWaves];
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    const bool on = c < nt && body_edge_of(c, edge_ids, body_id) >= 0;
+    const unsigned long long b = __ballot(on);
+    if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = __popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) counts[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
+}
+
+// exclusive scan of the nb counts in place, one workgroup of 1024 lanes (lane t: a contiguous run of
+// counts); status = {total, 0}
+__global__ __launch_bounds__(1024) void k_na_scan(int nb, int* __restrict__ counts, int* __restrict__ status)
+{
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int per = (nb + 1023) / 1024;
+    const int b0 = min(nb, t * per), b1 = min(nb, b0 + per);
+    int s = 0;
+    for (int b = b0; b < b1; ++b) s += counts[b];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = t > 0 ? part[t - 1] : 0;
+    for (int b = b0; b < b1; ++b) {
+        const int cnt = counts[b];
+        counts[b] = run;
+        run += cnt;
+    }
+    if (t == 1023) {
+        status[0] = part[1023];
+        status[1] = 0;
+    }
+}
+
+// each body cell's edge loads at its rank among the body cells; ranks past the capacity raise status[1]
+__global__ void k_na_loads(int nt, const int* __restrict__ edge_ids, int body_id, const int* __restrict__ offsets,
+                           const double* __restrict__ xy, const int* __restrict__ tri,
+                           const double4* __restrict__ invJ, const double* __restrict__ u0,
+                           const double* __restrict__ u1, const double* __restrict__ p, double cx, double cy,
+                           double mu, Quad1 qe, double* __restrict__ loads, int* __restrict__ status)
+{
+    __shared__ int wc[kWaves];
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    const int ed = c < nt ? body_edge_of(c, edge_ids, body_id) : -1;
+    const unsigned long long b = __ballot(ed >= 0);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) wc[w] = __popcll(b);
+    __syncthreads();
+    if (ed < 0) return;
+    int rank = offsets[blockIdx.x] + __popcll(b & ((1ull << lane) - 1ull));
+    for (int k = 0; k < w; ++k) rank += wc[k];
+    if (rank >= kNaMaxEdges) {
+        __hip_atomic_store(status + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    double l[4];
+    body_edge_load(c, ed, xy, tri, invJ, u0, u1, p, cx, cy, mu, qe, l);
+    for (int d = 0; d < 4; ++d) loads[4 * (size_t)rank + d] = l[d];
+}
+
+// the loads added in rank order by one lane (k_body_sum's order)
+__global__ void k_na_sum(const int* __restrict__ status, const double* __restrict__ loads, double* __restrict__ out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int ne = min(status[0], kNaMaxEdges);
+    double t[4] = {0, 0, 0, 0};
+    for (int k = 0; k < ne; ++k)
+        for (int d = 0; d < 4; ++d) t[d] += loads[4 * (size_t)k + d];
+    for (int d = 0; d < 4; ++d) out[d] = t[d];
+}
+
 }  // namespace pfem
 
 using namespace pfem;
@@ -717,6 +846,41 @@ int pfem_body_forces(pfem_mesh2d m, int body_id, double cx, double cy, const dou
         DevArray<double> d(4);
         body_forces(m, body_id, cx, cy, d_u0, d_u1, d_p, prm, d.p);
         d2h_sync(C(m->ctx), out, d.p, 4 * sizeof(double));
+    });
+}
+
+int pfem_body_forces_noalloc(pfem_mesh2d m, int body_id, double cx, double cy, const double* d_u0,
+                             const double* d_u1, const double* d_p, const pfem_pfem2_params* prm, double out[4])
+{
+    PFEM_TRY(m ? m->ctx : nullptr, {
+        PFEM_REQUIRE(m && d_u0 && d_u1 && d_p && prm && out, "null argument");
+        Context* c = C(m->ctx);
+        PFEM_REQUIRE(c->scratch, "allocation-free scratch not resolved on this device");
+        PFEM_REQUIRE(m->edge_ids.p, "edge boundary IDs are not set on the mesh");
+        const long nb = blocks_for(m->nt);
+        PFEM_REQUIRE(nb <= kNaMaxBlocks, "pfem_body_forces_noalloc: the mesh has more than 16777216 cells");
+        double* sc = c->scratch;
+        int* counts = reinterpret_cast<int*>(sc);
+        int* status = reinterpret_cast<int*>(sc + kNaStatus);
+        double* totals = sc + kNaTotals;
+        if (nb > 0) {
+            hipLaunchKernelGGL(k_na_count, dim3(nb), dim3(kBlock), 0, c->stream, m->nt, m->edge_ids.p, body_id, counts);
+            hipLaunchKernelGGL(k_na_scan, dim3(1), dim3(1024), 0, c->stream, (int)nb, counts, status);
+            hipLaunchKernelGGL(k_na_loads, dim3(nb), dim3(kBlock), 0, c->stream, m->nt, m->edge_ids.p, body_id,
+                               (const int*)counts, m->xy.p, m->tri.p, (const double4*)m->invJ.p, d_u0, d_u1, d_p, cx,
+                               cy, prm->mu, edge_rule(prm->edge_quad), sc + kNaLoads, status);
+            hipLaunchKernelGGL(k_na_sum, dim3(1), dim3(64), 0, c->stream, (const int*)status,
+                               (const double*)(sc + kNaLoads), totals);
+            PFEM_HIP(hipGetLastError());
+            // read back through the context's pinned buffers (no allocation)
+            PFEM_HIP(hipMemcpyAsync(c->pinned_d, totals, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            PFEM_HIP(hipMemcpyAsync(c->pinned_i, status, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            PFEM_HIP(hipStreamSynchronize(c->stream));
+            PFEM_REQUIRE(c->pinned_i[1] == 0, "pfem_body_forces_noalloc: the body has more than 65536 edges");
+            for (int d = 0; d < 4; ++d) out[d] = c->pinned_d[d];
+        } else {
+            for (int d = 0; d < 4; ++d) out[d] = 0.0;
+        }
     });
 }
 

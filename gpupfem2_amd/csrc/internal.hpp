@@ -91,11 +91,15 @@ struct pfem_context {
     std::string err;
     int* pinned_i = nullptr;      // small pinned host buffers for scalar read-backs
     double* pinned_d = nullptr;
+    double* scratch = nullptr;    // device address of this device's copy of the allocation-free scratch array
 };
 
 namespace pfem {
 struct Context : pfem_context {};
 inline Context* C(pfem_ctx c) { return static_cast<Context*>(c); }
+// Resolves the device address of the allocation-free entry points' scratch array on the context's
+// device (once per device); returns a HIP error code.
+int noalloc_init(Context* c);
 }  // namespace pfem
 
 struct pfem_csr_t {

@@ -438,7 +438,8 @@ int pfem_ctx_create(int device, pfem_ctx* out)
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return PFEM_EHIP; }
     if (hipHostMalloc((void**)&c->pinned_i, 64 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&c->pinned_d, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void**)&c->pinned_d, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        noalloc_init(c) != hipSuccess) {
         delete c;
         return PFEM_EHIP;
     }

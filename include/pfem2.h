@@ -313,6 +313,14 @@ int pfem_prediction_bc_values(pfem_mesh2d m, pfem_dirichlet target, pfem_dirichl
  * edges with boundary ID body_id, outward normal taken from the centre (cx, cy) */
 int pfem_body_forces(pfem_mesh2d m, int body_id, double cx, double cy, const double* d_u0, const double* d_u1,
                      const double* d_p, const pfem_pfem2_params* prm, double out[4]);
+/* No counterpart in the reference: pfem_body_forces for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc). Same inputs and the same bits
+ * as pfem_body_forces; it calls no hipMalloc/hipFree: the body's edges are found on the device and the
+ * per-edge loads go to one static device array per GPU (about 2.4 MB, held until the process ends).
+ * Limits: at most 16777216 cells and 65536 body edges (PFEM_EINVAL beyond). Calls that overlap on one
+ * GPU share that array and race. Synchronises the context stream to return the totals. */
+int pfem_body_forces_noalloc(pfem_mesh2d m, int body_id, double cx, double cy, const double* d_u0,
+                             const double* d_u1, const double* d_p, const pfem_pfem2_params* prm, double out[4]);
 
 /* ------------------------------------------------------------------ particles (particle_handler_2d.cuh) */
 /* ParticleHandler2D(const Mesh2D*, int cellDivisionLevel) (particle_handler_2d.cu:238-294) */
