@@ -151,6 +151,7 @@ SIGNATURES = {
     "pfem_time_kernel": (_i, [_i, _vp, _vp, _i, C.POINTER(_d)]),
     "pfem_pfem2_sweep_fallbacks": (_i, [_vp]),
     "pfem_export_vtk": (_i, [_vp, C.c_char_p, _i, _vp, _vp]),
+    "pfem_export_vtk3d": (_i, [_vp, C.c_char_p, _i, _vp, _vp]),
 }
 
 _LIB = None
@@ -710,3 +711,9 @@ def export_vtk(mesh, path, fields):
     names = (C.c_char_p * len(fields))(*[k.encode() for k in fields])
     ptrs = (C.c_void_p * len(fields))(*[v.ptr for v in fields.values()])
     mesh.ctx.check(lib().pfem_export_vtk(mesh.h, path.encode(), len(fields), names, ptrs))
+
+
+def export_vtk3d(mesh, path, fields):
+    names = (C.c_char_p * len(fields))(*[k.encode() for k in fields])
+    ptrs = (C.c_void_p * len(fields))(*[v.ptr for v in fields.values()])
+    mesh.ctx.check(lib().pfem_export_vtk3d(mesh.h, path.encode(), len(fields), names, ptrs))

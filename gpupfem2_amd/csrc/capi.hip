@@ -279,4 +279,46 @@ int pfem_export_vtk(pfem_mesh2d m, const char* path, int nfields, const char* co
     });
 }
 
+// DataExport3D::exportToVTK (src/data_export_3d.cu:15-93): ASCII VTU, points, tetrahedra (type 10),
+// nodal scalar fields. Written with 17 significant digits (the reference streams 6).
+int pfem_export_vtk3d(pfem_mesh3d m, const char* path, int nfields, const char* const* names,
+                      const double* const* d_fields)
+{
+    PFEM_TRY(m ? m->ctx : nullptr, {
+        PFEM_REQUIRE(m && path && (nfields == 0 || (names && d_fields)), "null argument");
+        std::ofstream f(path);
+        if (!f.is_open()) throw Error(PFEM_EIO, std::string("cannot write ") + path);
+        f << std::setprecision(17);
+        f << "<?xml version=\"1.0\" ?> \n";
+        f << "<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\">\n";
+        f << "  <UnstructuredGrid>\n";
+        f << "    <Piece NumberOfPoints=\"" << m->nv << "\" NumberOfCells=\"" << m->nt << "\">\n";
+        f << "      <Points>\n        <DataArray type=\"Float32\" NumberOfComponents=\"3\" Format=\"ascii\">\n";
+        for (int i = 0; i < m->nv; ++i)
+            f << "        " << m->h_xyz[3 * (size_t)i] << " " << m->h_xyz[3 * (size_t)i + 1] << " "
+              << m->h_xyz[3 * (size_t)i + 2] << "\n";
+        f << "        </DataArray>\n      </Points>\n      <Cells>\n";
+        f << "        <DataArray type=\"Int32\" Name=\"connectivity\" Format=\"ascii\">\n          ";
+        for (int c = 0; c < m->nt; ++c)
+            for (int k = 0; k < 4; ++k) f << m->h_tet[4 * (size_t)c + k] << " ";
+        f << "\n        </DataArray>\n        <DataArray type=\"Int32\" Name=\"offsets\" Format=\"ascii\">\n          ";
+        for (int c = 0; c < m->nt; ++c) f << (c + 1) * 4 << " ";
+        f << "\n        </DataArray>\n        <DataArray type=\"UInt8\" Name=\"types\" Format=\"ascii\">\n          ";
+        for (int c = 0; c < m->nt; ++c) f << 10 << " ";
+        f << "\n        </DataArray>\n      </Cells>\n";
+        if (nfields > 0) {
+            f << "      <PointData Scalars=\"scalars\">\n";
+            std::vector<double> h(m->nv);
+            for (int k = 0; k < nfields; ++k) {
+                d2h_sync(C(m->ctx), h.data(), d_fields[k], sizeof(double) * m->nv);
+                f << "        <DataArray type=\"Float32\" Name=\"" << names[k] << "\" Format=\"ascii\">\n        ";
+                for (int i = 0; i < m->nv; ++i) f << h[i] << "  ";
+                f << "\n        </DataArray>\n";
+            }
+            f << "      </PointData>\n";
+        }
+        f << "    </Piece>\n  </UnstructuredGrid>\n</VTKFile>\n";
+    });
+}
+
 }  // extern "C"

@@ -309,6 +309,13 @@ public:
     }
     int numVertices() const { return nv_; }
     int numCells() const { return nt_; }
+    /* Mesh3D::getHostVertices (mesh_3d.cuh): the vertices as imported */
+    std::vector<Point3> getHostVertices() const
+    {
+        std::vector<Point3> v((size_t)nv_);
+        if (nv_) gpupfem2::check(pfem_mesh3d_download(handleChecked(), 0, v.data()));
+        return v;
+    }
     deviceSpan<Point3> getVertices() const { return {static_cast<Point3*>(ptr(0)), nv_}; }
     deviceSpan<int> getCells() const { return {static_cast<int*>(ptr(1)), 4 * nt_}; }
     deviceSpan<double> getCellVolume() const { return {static_cast<double*>(ptr(2)), nt_}; }
@@ -833,6 +840,27 @@ protected:
     const Mesh2D& mesh;
 };
 
+/* NumericalIntegrator3D (numerical_integrator_3d.cuh:46-57): as NumericalIntegrator2D, the base of a
+ * case's integrator; the Poisson3D integrand (cases/Poisson3D/main.cu:26-67) is the library operator
+ * behind assemblePoisson. */
+class NumericalIntegrator3D {
+public:
+    explicit NumericalIntegrator3D(const Mesh3D& mesh_) : mesh(mesh_) {}
+    virtual ~NumericalIntegrator3D() = default;
+
+protected:
+    /* coefficient 0.5 and f = exp(-(x^2 + 10 y^2 + z^2)) by default (rhsKind 0; 1: rhsParam sin sin sin,
+     * 2: constant rhsParam); cellQuadrature: createCellQuadratureFormula index (0 = 4 points, 1 = 6) */
+    void assemblePoisson(SparseMatrixCSR& a, deviceVector<double>& rhs, double coeff = 0.5, int rhsKind = 0,
+                         double rhsParam = 0.0, int cellQuadrature = 1)
+    {
+        gpupfem2::check(pfem_assemble_poisson3d(mesh.handle(), a.handle(), rhs.data, coeff, rhsKind, rhsParam,
+                                                cellQuadrature));
+    }
+
+    const Mesh3D& mesh;
+};
+
 /* ------------------------------------------------------------------ particles */
 class ParticleHandler2D {
 public:
@@ -953,6 +981,33 @@ public:
 private:
     const Mesh2D& mesh;
     const ParticleHandler2D* particleHandler;
+    std::map<std::string, const double*> scalarDataVectors;
+};
+
+/* DataExport3D (data_export_3d.cuh:9-25): nodal scalar fields of a tetrahedral mesh to a VTU file */
+class DataExport3D {
+public:
+    explicit DataExport3D(const Mesh3D& mesh_) : mesh(mesh_) {}
+    void addScalarDataVector(const deviceVector<double>& dataVector, const std::string& fieldname)
+    {
+        scalarDataVectors[fieldname] = dataVector.data;
+    }
+    /* DataExport3D::exportToVTK (data_export_3d.cu:15-93) */
+    void exportToVTK(const std::string& filename) const
+    {
+        std::vector<const char*> names;
+        std::vector<const double*> ptrs;
+        for (const auto& kv : scalarDataVectors) {
+            names.push_back(kv.first.c_str());
+            ptrs.push_back(kv.second);
+        }
+        const int rc = pfem_export_vtk3d(mesh.handle(), filename.c_str(), (int)names.size(), names.data(), ptrs.data());
+        if (rc == PFEM_OK) std::printf("Mesh solution saved to %s\n", filename.c_str());
+        else std::printf("Error while saving mesh solution to a file\n");
+    }
+
+private:
+    const Mesh3D& mesh;
     std::map<std::string, const double*> scalarDataVectors;
 };
 

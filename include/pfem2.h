@@ -402,13 +402,19 @@ int pfem_pfem2_precond(pfem_pfem2 s, int which, pfem_precond* out);
  * to FP64 vectors with the FP32 factor (PFEM_PRECISION_MIXED_PRECOND); ops 5 and 6: ops 2 and 4 on
  * vectors already in a multicolour factor's ordering (as the Krylov solvers run them); op 7: FP64 SpMV
  * of the matrix in that ordering (P a P^T, as the Krylov solvers run it); ops 8 and 9: ops 6 and 5 with
- * the tagged sweeps of PCG fusion level 3 (sliced layout). */
+ * the tagged sweeps of PCG fusion level 3 (sliced layout); ops 10 and 11: one iteration of the fused PCG
+ * as pfem_solver_solve runs it with pc (op 10 FP64 factor, op 11 its PFEM_PRECISION_MIXED_PRECOND copy),
+ * the device time of a solve of 2 launches iterations minus one of `launches` iterations, over launches. */
 int pfem_time_kernel(int op, pfem_csr a, pfem_precond pc, int launches, double* avg_ms);
 
 /* ------------------------------------------------------------------ export (src/data_export.cuh) */
 /* DataExport::exportToVTK (src/data_export.cu:17-95): ASCII VTU with nodal scalars (device arrays) */
 int pfem_export_vtk(pfem_mesh2d m, const char* path, int nfields, const char* const* names,
                     const double* const* d_fields);
+/* DataExport3D::exportToVTK (src/data_export_3d.cu:15-93): ASCII VTU of a tetrahedral mesh (cell type 10)
+ * with nodal scalars (device arrays) */
+int pfem_export_vtk3d(pfem_mesh3d m, const char* path, int nfields, const char* const* names,
+                      const double* const* d_fields);
 
 #ifdef __cplusplus
 }
