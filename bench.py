@@ -7,6 +7,7 @@ PCG + Jacobi, relative tolerance 1e-12), in the mixed-precision pipeline and in 
     python bench.py --kernels      # only the pressure-solve kernels (for counter collection)
     python bench.py --workload poisson3d   # 8-RHS 3D Poisson solves (secondary workload)
     python bench.py --workload batched     # 256 independent Poisson systems (secondary workload)
+    python bench.py --workload history --steps 500   # C_D / C_L histories, FP64 and mixed
 
 One step = advection + relocation + refill + reorder of all particles, projection, the two outer
 iterations of prediction / pressure / correction (5 assemblies + 5 solves each), particle velocity
@@ -14,8 +15,11 @@ correction and the drag/lift integral. The flow starts from rest; W warm-up step
 steps bracketed by device synchronisation. Prints one JSON line.
 
 Mixed precision: Jacobi-preconditioned solves run FP32 inner Krylov + FP64 refinement; the IC0
-pressure solve runs FP64 PCG with the IC0 factor stored in FP32 (FP32 refinement stalls on the
-pressure matrix, whose condition number is ~1/eps32).
+pressure solve runs FP64 PCG with the IC0 factor stored in 16 bits (IEEE half scaled by a power of two;
+FP32 refinement stalls on the pressure matrix, whose condition number is ~1/eps32).
+
+The line's `roofline` is the fused pressure PCG iteration (about 93% of a step), timed with HIP events;
+`cpu_baseline` is one full FP64 step of the OpenMP build of the oracle on the host cores.
 """
 import argparse
 import json
@@ -68,7 +72,8 @@ def spmv_bytes(n, nnz, val_bytes, vec_bytes):
     return nnz * (val_bytes + 4) + 4 * (n + 1) + 2 * n * vec_bytes
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "ic0_apply_cylinder2M.md")
+TRAFFIC_PROFILE_REL = "profiles/ic0_apply_cylinder2M.md"
+TRAFFIC_PROFILE = os.path.join(ROOT, TRAFFIC_PROFILE_REL)
 
 
 def measured_traffic(key):
@@ -85,9 +90,8 @@ def measured_traffic(key):
 
 
 def roofline_ic0(ctx, st, precision):
-    """The dominant kernels of a step: the two single-pass triangular sweeps of the pressure IC0 apply
-    (k_sweep_lower + k_sweep_upper, about 55% of the step), timed with HIP events on the context
-    stream over 200 applies."""
+    """Secondary: the standalone pressure IC0 apply (k_sweep_lower + k_sweep_upper, tagged, in the factor's
+    ordering), timed with HIP events on the context stream over 200 applies."""
     from gpupfem2_amd import capi
     a, pc = st.matrix_handle(2), st.precond_handle(1)
     n, nnz = csr_info(a)
@@ -116,38 +120,28 @@ def pcg_iteration_bytes(n, nnz, factor_bytes):
     return spmv_bytes(n, nnz, 8, 8) + (tri + 4 * 8 * n) + (tri + 6 * 8 * n)
 
 
-def roofline_pcg(ctx, st, precision, tol=1e-9):
-    """One pressure solve from x = 0 (relative tolerance 1e-9, a seeded right-hand side) with the step's
-    matrix and preconditioner, as the step runs it: time per iteration (the solve's wall time, which also
-    holds one factorization, over its iterations) against the algorithmic bytes of an iteration."""
-    import types
-    import numpy as np
+def roofline_pcg(ctx, st, precision, iterations=400):
+    """The step's dominant work: one fused pressure PCG iteration (k_cg_spmv_alpha_sliced + k_sweep_lower
+    + k_sweep_upper_xp) as the step runs it, on the step's own pressure matrix and preconditioner (16-bit
+    factor copy in the mixed pipeline, FP64 factor otherwise). Timed with HIP events on the context stream
+    (pfem_time_kernel ops 11 / 10: a solve of 2L iterations minus one of L, over L; the set-up cancels)
+    against the algorithmic bytes of one iteration; traffic from the committed counter runs."""
     from gpupfem2_amd import capi
-    a = types.SimpleNamespace(h=st.matrix_handle(2), ctx=ctx)
-    pc = types.SimpleNamespace(h=st.precond_handle(1))
-    n, nnz = csr_info(a.h)
-    s = capi.Solver(a, capi.SOLVER_CG, tol, 20000, 0, capi.TOL_RELATIVE, pc)
-    if precision == 1:
-        s.set_precision(capi.PRECISION_MIXED_PRECOND, 1e-5, 0)
-    b = ctx.to_device(np.random.default_rng(1).standard_normal(n))
-    best = None
-    for _ in range(2):
-        x = ctx.zeros(n)
-        ctx.sync()
-        t = time.perf_counter()
-        stt = s.solve(a, x, b)
-        ctx.sync()
-        dt = time.perf_counter() - t
-        best = dt if best is None else min(best, dt)
-    us = best / stt.iterations * 1e6
-    b_it = pcg_iteration_bytes(n, nnz, 2 if precision == 1 else 8)
+    a, pc = st.matrix_handle(2), st.precond_handle(1)
+    n, nnz = csr_info(a)
+    mixed = precision == 1
+    ms = capi.time_kernel(ctx, 11 if mixed else 10, a, pc, launches=iterations)
+    us = ms * 1e3
+    b_it = pcg_iteration_bytes(n, nnz, 2 if mixed else 8)
     gbs = b_it / (us * 1e-6) / 1e9
-    del s
-    traffic = measured_traffic("traffic_bytes_pcg_iteration_fp16_factor" if precision == 1
-                               else "traffic_bytes_pcg_iteration_fp64")
-    return dict(iterations=stt.iterations, us_per_iteration=round(us, 2), bytes_per_iteration=b_it,
-                achieved=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 4), traffic_per_iteration=traffic,
-                kernels="k_cg_spmv_alpha_sliced + k_sweep_lower (r update, gamma) + k_sweep_upper (x, p updates)")
+    traffic = measured_traffic("traffic_bytes_pcg_iteration_fp16_factor" if mixed else "traffic_bytes_pcg_iteration_fp64")
+    return dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(gbs / HBM_PEAK_GBS, 4),
+                traffic=traffic, traffic_source=TRAFFIC_PROFILE_REL if traffic else None,
+                kernel=(f"fused pressure PCG iteration = k_cg_spmv_alpha_sliced + k_sweep_lower (r update, gamma) + "
+                        f"k_sweep_upper_xp (x, p updates), n={n} nnz={nnz}, factor "
+                        f"{'FP16 (power-of-two scaled)' if mixed else 'FP64'}, vectors FP64"),
+                avg_us=round(us, 2), bytes_per_launch=b_it, timing="HIP events on the context stream, "
+                f"solves of {2 * iterations} and {iterations} iterations differenced")
 
 
 def roofline_spmv(ctx, st):
@@ -162,97 +156,62 @@ def roofline_spmv(ctx, st):
                 frac=round(gbs / HBM_PEAK_GBS, 4), avg_us=round(ms * 1e3, 2), bytes_per_launch=b)
 
 
-def natural_over_multicolour(ctx):
-    """Iterations of IC0-PCG (oracle, relative 1e-12, random right-hand side) with the natural-order
-    factor over those with the library's multicolour factor, on the cylinder channel's pressure matrix at
-    h = 1/28 (65k nodes): the CPU runs the reference's natural ordering, which converges faster."""
-    import numpy as np
-    import scipy.sparse as sp
-    from gpupfem2_amd import capi, cases
-    from oracle import oracle as O
-    sc = cases.cylinder_case(h=1.0 / 28)
-    nv = sc.xy.shape[0]
-    area, invJ = O.geometry2d(sc.xy, sc.tri)
-    rp, col = O.csr_pattern(nv, sc.tri)
-    val, _ = O.assemble_poisson2d(sc.xy, sc.tri, area, invJ, rp, col, 1.0, 2, 0.0, 1)
-    b = np.random.default_rng(0).standard_normal(nv)
-    O.dirichlet_apply(rp, col, val, b, sc.bcp_node, sc.bcp_val, sc.dirichlet_mode)
-    nat = O.pcg(rp, col, val, np.zeros(nv), b, 1e-12, 20000, 1, "ic0").iterations
-    m = capi.Mesh2D(ctx, sc.xy, sc.tri)
-    a = capi.CSR(ctx, mesh=m)
-    a.upload_values(val)
-    pc = capi.Preconditioner(a, capi.PRECOND_IC0, capi.ORDER_MULTICOLOR)
-    pc.initialize(a)
-    _, _, perm = pc.download()
-    B = sp.csr_matrix((val, col, rp), shape=(nv, nv))[perm][:, perm].tocsr()
-    B.sort_indices()
-    mc = O.pcg(B.indptr.astype(np.int32), B.indices.astype(np.int32), B.data.copy(), np.zeros(nv),
-               np.ascontiguousarray(b[perm]), 1e-12, 20000, 1, "ic0").iterations
-    return nat / mc
+def cpu_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model
 
 
-def cpu_baseline(case, st, mesh, gpu_its):
-    """The oracle (1 thread, FP64) on the same full-size inputs: particle step, the 5 assemblies, and
-    per-iteration Krylov costs measured with capped iteration counts, combined with the iteration
-    counts of the GPU FP64 step into a time per step."""
+def cpu_baseline(case, st, mesh, ph):
+    """One full FP64 time step of the same workload on the host cores: the oracle's OpenMP perf build
+    (oracle/liboracle_omp.so, -O3 -fopenmp: the same algorithm restated in C++, particles, projection, the
+    five assemblies, prediction GMRES(30) + Jacobi, pressure PCG + IC0 on a multicolour ordering (rows of
+    one colour in parallel), correction PCG + Jacobi), started from the GPU FP64 run's state after its timed
+    steps (fields and all particles copied to the host), timed as one step."""
     import numpy as np
     from oracle import oracle as O
+    O.use_openmp_build()
     T0 = time.perf_counter()
     f = st.fields()
-    nv = case.xy.shape[0]
-    area, invJ = O.geometry2d(case.xy, case.tri)
     off, idx = mesh.download(4), mesh.download(5)
-    prm = O.make_params(case.rho, case.mu, case.dt, case.scheme, case.traction_mask, case.normals)
-    rp, col = O.csr_pattern(nv, case.tri)
-    # particles: seed, one advect (3 substeps, relocation, refill, reorder), projection, correction
-    ph = O.Particles(case.xy, case.tri, invJ, off, idx, case.level)
-    ph.seed(case.seed_mode, case.ppc, case.seed)
-    ph.init_velocity(f["u0"], f["u1"])
+    area, invJ = O.geometry2d(case.xy, case.tri)
+    oph = O.Particles(case.xy, case.tri, invJ, off, idx, case.level)
+    pst = ph.get()
+    oph.set(pst, int(pst["id"].max()) + 1 if pst["id"].size else 0)
+    del pst
+    cfg = O.StepConfig()
+    cfg.prm = O.make_params(case.rho, case.mu, case.dt, case.scheme, case.traction_mask, case.normals)
+    for k in ("outer_iterations", "substeps", "pred_solver", "pred_precond", "pres_solver", "pres_precond",
+              "corr_solver", "corr_precond", "tol", "tol_mode", "maxit", "restart", "dirichlet_mode", "body_id",
+              "force_coeff"):
+        setattr(cfg, k, getattr(case, k))
+    cfg.body_cx, cfg.body_cy = case.body_center
+    ost = O.Pfem2(case.xy, case.tri, case.edge_ids, cfg, oph, case.bcu_node, case.bcu_val0, case.bcu_val1,
+                  case.bcp_node, case.bcp_val)
+    ost.set_fields(u0=f["u0"], u1=f["u1"], p=f["p"], vp0=f["vp0"], vp1=f["vp1"])
+    ost.set_coloured_pressure(True)
+    npart = oph.count()
+    print(f"[bench] cpu baseline: one FP64 step on {O.threads()} threads, {npart} particles", file=sys.stderr,
+          flush=True)
     t = time.perf_counter()
-    ph.advect(f["u0"], f["u1"], case.dt, case.substeps)
-    u0, u1 = ph.project(nv)
-    ph.correct(f["u0"], f["u1"], u0, u1)
-    t_part = time.perf_counter() - t
-    del ph
-    # assemblies (one of each system) + Dirichlet
-    t = time.perf_counter()
-    v0, v1, r0, r1 = O.assemble_prediction(case.xy, case.tri, area, invJ, case.edge_ids, rp, col, f["u0"], f["u1"],
-                                           f["u0"], f["u1"], f["p"], prm)
-    t_pred_asm = time.perf_counter() - t
-    t = time.perf_counter()
-    vp, rpv = O.assemble_pressure(case.tri, area, invJ, rp, col, f["vp0"], f["vp1"], f["p"], prm)
-    O.dirichlet_apply(rp, col, vp, rpv, case.bcp_node, case.bcp_val, case.dirichlet_mode)
-    t_pres_asm = time.perf_counter() - t
-    t = time.perf_counter()
-    vc, c0, c1 = O.assemble_correction(case.tri, area, invJ, rp, col, f["vp0"], f["vp1"], f["p"], f["p"], prm)
-    t_corr_asm = time.perf_counter() - t
-    O.dirichlet_apply(rp, col, v0, r0, case.bcu_node, case.bcu_val0, case.dirichlet_mode)
-    O.dirichlet_apply(rp, col, vc, c0, case.bcu_node, case.bcu_val0, case.dirichlet_mode)
-
-    def per_iter(fn, lo, hi):
-        ta = time.perf_counter(); fn(lo); tb = time.perf_counter(); fn(hi); tc = time.perf_counter()
-        it = ((tc - tb) - (tb - ta)) / (hi - lo)
-        return max(it, 1e-9), max((tb - ta) - lo * it, 0.0)
-
-    x = np.zeros(nv)
-    pres_it, pres_setup = per_iter(lambda k: O.pcg(rp, col, vp, x * 0, rpv, 1e-300, k, 1, "ic0"), 10, 60)
-    corr_it, corr_setup = per_iter(lambda k: O.pcg(rp, col, vc, x * 0, c0, 1e-300, k, 1, "jacobi"), 10, 60)
-    pred_it, pred_setup = per_iter(lambda k: O.gmres(rp, col, v0, x * 0, r0, 1e-300, k, 30, 1, "jacobi"), 10, 60)
-    its = np.asarray(gpu_its[-1]).reshape(case.outer_iterations, 5)
-    ratio = natural_over_multicolour(mesh.ctx)
-    t_solve = (its[:, 0:2].sum() * pred_it + 2 * case.outer_iterations * pred_setup +
-               ratio * its[:, 2].sum() * pres_it + case.outer_iterations * pres_setup +
-               its[:, 3:5].sum() * corr_it + 2 * case.outer_iterations * corr_setup)
-    t_step = t_part + case.outer_iterations * (2 * t_pred_asm + t_pres_asm + t_corr_asm) + t_solve
-    return dict(value=round(1.0 / t_step, 6), unit="steps/s", cores=O.threads(), kind="port",
-                sample=(f"oracle (1 thread, FP64) on the full {case.tri.shape[0]}-triangle case: one particle "
-                        f"advect+project+correct ({t_part:.1f} s), one assembly of each system, and 50 "
-                        f"Krylov iterations per solver type (IC0-PCG {pres_it * 1e3:.1f} ms/it, Jacobi-PCG "
-                        f"{corr_it * 1e3:.1f} ms/it, GMRES(30) {pred_it * 1e3:.1f} ms/it), combined with the "
-                        f"GPU FP64 step's iteration counts, the pressure's scaled by {ratio:.3f} (the natural-order "
-                        f"IC0 of the reference against the GPU's multicolour factor, measured on the same channel "
-                        f"at h = 1/28); measured in {time.perf_counter() - T0:.0f} s"),
-                seconds_per_step=round(t_step, 2))
+    forces = ost.step()
+    dt = time.perf_counter() - t
+    its = ost.iterations().tolist()
+    return dict(value=round(1.0 / dt, 6), unit="steps/s", cores=O.threads(), kind="port",
+                sample=(f"one full FP64 time step of the {case.tri.shape[0]}-triangle cylinder ({npart} particles) "
+                        f"by the oracle's OpenMP build (-O3 -fopenmp) on the host cores, from the GPU FP64 run's "
+                        f"state: {dt:.1f} s; Krylov iterations {its}; pressure IC0 in a greedy multicolour ordering; "
+                        f"set-up and transfers {t - T0:.0f} s (not timed)"),
+                seconds_per_step=round(dt, 2), cd_cl=[float(v) for v in forces],
+                host={"nproc": os.cpu_count(), "cpu_model": cpu_info(), "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"),
+                      "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"), "threads_used": O.threads()})
 
 
 def kernels_only(h, launches):
@@ -298,6 +257,12 @@ def kernels_only(h, launches):
         del s
     res["pcg_iteration_bytes_fp16_factor"] = pcg_iteration_bytes(n, nnz, 2)
     res["pcg_iteration_bytes_fp64"] = pcg_iteration_bytes(n, nnz, 8)
+    # the bench line's roofline measurement itself (pfem_time_kernel ops 11 / 10: solves of 2L and L
+    # iterations differenced), so that a kernel trace of this run holds the same launches
+    for name, p, op in (("pcg_iteration_fp16_factor_us", pc16, 11), ("pcg_iteration_fp64_us", pc, 10)):
+        ms = capi.time_kernel(ctx, op, a.h, p.h, launches=launches)
+        res[name] = round(ms * 1e3, 2)
+        print(f"[kernels] {name}: {ms * 1e3:.2f} us", file=sys.stderr, flush=True)
     mib = 256
     src, dst = ctx.zeros(mib << 17), ctx.empty(mib << 17)
     for _ in range(launches):
@@ -378,18 +343,20 @@ def batched_workload(args):
     triangles (log-uniform), one block-diagonal matrix, Jacobi PCG per system to relative 1e-10, FP64 and
     mixed (reliable updates). A step is one batched solve; value = systems solved per second."""
     import numpy as np
-    from gpupfem2_amd import capi, cases
+    from gpupfem2_amd import capi, cases, tune
     case = cases.batched_poisson_case()
     ctx = capi.Context(0)
     mesh, a, b = cases.build_batched_poisson(ctx, case)
     rows = case["sys_rows"]
     n = int(rows[-1])
+    tuning = None if args.untuned else tune.load()
     out = {}
     for prec in (0, 1):
         s = capi.Solver(a, capi.SOLVER_CG, 1e-10, 20000, 0, capi.TOL_RELATIVE,
                         capi.Preconditioner(a, capi.PRECOND_JACOBI))
         if prec:
             s.set_precision(prec, 1e-5, 30)
+        tune.apply(s, tuning)
         x = ctx.zeros(n)
         times, t2s, its = [], [], []
         for k in range(args.warmup + args.steps):
@@ -425,6 +392,53 @@ def batched_workload(args):
         "time_to_solution_ms": {("mixed" if k else "fp64"): {"p50": round(v["p50_ms"], 1), "p99": round(v["p99_ms"], 1)}
                                 for k, v in out.items()},
         "iterations": {("mixed" if k else "fp64"): v["iterations"] for k, v in out.items()},
+        "tuning": ({"buckets": tuning["buckets"], "poll": tuning["poll"], "source": "gpupfem2_amd/tuning_batched.json"}
+                   if tuning else "defaults (256-row, 2048-non-zero tiles, poll 8)"),
+    }
+    print(json.dumps(line))
+
+
+def history_workload(args):
+    """The cylinder configuration's drag and lift histories (Cylinder2D/main.cu:401-428, 872-874): `steps`
+    time steps from rest in plain FP64 and in the mixed pipeline, C_D and C_L after every step. Prints one
+    JSON line with the per-mode histories (every step), the largest relative difference between the modes
+    and the time per step; --out also writes the histories as CSV."""
+    import numpy as np
+    from gpupfem2_amd import capi, cases
+    case = cases.cylinder_case(h=args.h)
+    ctx = capi.Context(0)
+    hist, times = {}, {}
+    for prec in (0, 1):
+        mode = "mixed" if prec else "fp64"
+        mesh, ph, st = cases.build_gpu(ctx, case, precision=prec, inner_tol=args.inner_tol)
+        f = np.zeros((args.steps, 2))
+        ctx.sync()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            f[k] = st.step()
+            if (k + 1) % 50 == 0:
+                print(f"[bench] {mode} step {k + 1}: C_D {f[k, 0]:.6f} C_L {f[k, 1]:.6e} "
+                      f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+        ctx.sync()
+        times[mode] = (time.perf_counter() - t0) / args.steps
+        hist[mode] = f
+        del st, ph, mesh
+    d = np.abs(hist["mixed"] - hist["fp64"])
+    rel = d / np.maximum(np.abs(hist["fp64"]), 1e-300)
+    if args.out:
+        with open(args.out, "w") as fh:
+            fh.write("step,cd_fp64,cl_fp64,cd_mixed,cl_mixed\n")
+            for k in range(args.steps):
+                fh.write(f"{k + 1},{hist['fp64'][k, 0]:.17g},{hist['fp64'][k, 1]:.17g},"
+                         f"{hist['mixed'][k, 0]:.17g},{hist['mixed'][k, 1]:.17g}\n")
+    line = {
+        "metric": "cylinder drag/lift histories, FP64 and mixed", "steps": args.steps,
+        "config": {"workload": "cylinder Re=100, scheme B, dt=2e-4", "triangles": int(case.tri.shape[0])},
+        "s_per_step": {k: round(v, 4) for k, v in times.items()},
+        "max_rel_diff": {"cd": float(rel[:, 0].max()), "cl": float(rel[:, 1].max())},
+        "max_abs_diff": {"cd": float(d[:, 0].max()), "cl": float(d[:, 1].max())},
+        "final": {k: [float(v[-1, 0]), float(v[-1, 1])] for k, v in hist.items()},
+        "history": {k: [[round(float(a), 9), float(b)] for a, b in v] for k, v in hist.items()},
     }
     print(json.dumps(line))
 
@@ -440,9 +454,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernels", action="store_true", help="pressure-solve kernels only (counter runs)")
     ap.add_argument("--launches", type=int, default=20)
-    ap.add_argument("--workload", choices=["cylinder", "poisson3d", "batched"], default="cylinder",
+    ap.add_argument("--workload", choices=["cylinder", "poisson3d", "batched", "history"], default="cylinder",
                     help="cylinder: the headline PFEM-2 step; poisson3d: 8-RHS 3D Poisson solves; "
-                         "batched: 256 independent Poisson systems")
+                         "batched: 256 independent Poisson systems; history: C_D/C_L over --steps steps, "
+                         "FP64 and mixed")
+    ap.add_argument("--out", default=None, help="history: CSV of the per-step C_D and C_L")
+    ap.add_argument("--untuned", action="store_true", help="batched: ignore gpupfem2_amd/tuning_batched.json")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)")
@@ -452,6 +469,9 @@ def main():
     if args.workload == "batched":
         batched_workload(args)
         return
+    if args.workload == "history":
+        history_workload(args)
+        return
     if args.kernels:
         kernels_only(args.h, args.launches)
         return
@@ -459,7 +479,7 @@ def main():
     case = cases.cylinder_case(h=args.h)
     ctx = capi.Context(0)
     res = {}
-    roof = roof_spmv = None
+    roof_ic0 = roof_spmv = None
     cpu = None
     roof_pcg = {}
     for prec in ([0, 1] if args.mode == "both" else [0] if args.mode == "fp64" else [1]):
@@ -470,16 +490,18 @@ def main():
         print(f"[bench] {r['steps_per_s']:.3f} steps/s, iterations {r['iterations'][-1]}", file=sys.stderr,
               flush=True)
         roof_pcg["mixed" if prec else "fp64"] = roofline_pcg(ctx, st, prec)
-        print(f"[bench] pressure PCG iteration {roof_pcg['mixed' if prec else 'fp64']['us_per_iteration']} us",
+        print(f"[bench] pressure PCG iteration {roof_pcg['mixed' if prec else 'fp64']['avg_us']} us",
               file=sys.stderr, flush=True)
         if prec == (1 if args.mode != "fp64" else 0):
-            roof = roofline_ic0(ctx, st, prec)
+            roof_ic0 = roofline_ic0(ctx, st, prec)
             roof_spmv = roofline_spmv(ctx, st)
-            print(f"[bench] IC0 apply {roof['avg_us']} us ({roof['achieved']} GB/s), SpMV {roof_spmv['avg_us']} us",
+            print(f"[bench] IC0 apply {roof_ic0['avg_us']} us ({roof_ic0['achieved']} GB/s), SpMV {roof_spmv['avg_us']} us",
                   file=sys.stderr, flush=True)
         if prec == 0 and not args.no_cpu:
-            print("[bench] cpu baseline (oracle)", file=sys.stderr, flush=True)
-            cpu = cpu_baseline(case, st, mesh, r["iterations"])
+            print("[bench] cpu baseline (oracle, OpenMP build)", file=sys.stderr, flush=True)
+            cpu = cpu_baseline(case, st, mesh, ph)
+            # the GPU's own step from the same state, for the CPU step's C_D and C_L to be compared with
+            cpu["gpu_cd_cl_same_step"] = [float(v) for v in st.step()]
         del st, ph, mesh
     head = res.get(1, res.get(0))
     line = {
@@ -493,20 +515,25 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64+f32" if 1 in res else "f64",
+        "dtype": ("f64+f32+f16" if 1 in res else "f64"),
         "data": "synthetic (generated mesh, Philox-seeded particles, flow from rest)",
         "config": {"workload": "cylinder Re=100, scheme B, dt=2e-4", "triangles": int(case.tri.shape[0]),
                    "nodes": int(case.xy.shape[0]), "particles_per_cell": case.ppc,
                    "particles_after_steps": head["particles"], "parallelism": "1 GPU",
-                   "precision": "mixed" if 1 in res else "fp64"},
+                   "precision": "mixed" if 1 in res else "fp64",
+                   "mixed_precision": ("velocity solves: FP32 inner Krylov with FP64 refinement; pressure: FP64 "
+                                       "PCG with the IC0 factor stored in 16 bits (IEEE half, power-of-two scaled), "
+                                       "FP64 sweep arithmetic; particles, assembly, loads FP64"),
+                   "pres_factor_bits": 16 if 1 in res else 64},
         "fp64_steps_per_s": round(res[0]["steps_per_s"], 4) if 0 in res else None,
         "mixed_steps_per_s": round(res[1]["steps_per_s"], 4) if 1 in res else None,
         "mixed_over_fp64": round(res[1]["steps_per_s"] / res[0]["steps_per_s"], 3) if len(res) == 2 else None,
         "krylov_iterations_last_step": {("mixed" if k else "fp64"): v["iterations"][-1] for k, v in res.items()},
         "cd_cl_last_step": {("mixed" if k else "fp64"): v["cd_cl"] for k, v in res.items()},
-        "roofline": roof,
+        "roofline": roof_pcg.get("mixed", roof_pcg.get("fp64")),
+        "roofline_pcg_iteration_fp64": roof_pcg.get("fp64"),
+        "roofline_ic0_apply": roof_ic0,
         "roofline_spmv": roof_spmv,
-        "roofline_pcg_iteration": roof_pcg,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

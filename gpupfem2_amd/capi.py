@@ -40,7 +40,8 @@ class StepConfig(C.Structure):
                 ("tol", C.c_double), ("tol_mode", C.c_int), ("maxit", C.c_int), ("restart", C.c_int),
                 ("dirichlet_mode", C.c_int), ("body_id", C.c_int), ("body_cx", C.c_double),
                 ("body_cy", C.c_double), ("force_coeff", C.c_double), ("precision", C.c_int),
-                ("inner_tol", C.c_double), ("pres_ordering", C.c_int), ("pres_factor_bits", C.c_int)]
+                ("inner_tol", C.c_double), ("pres_ordering", C.c_int), ("pres_factor_bits", C.c_int),
+                ("pred_gs_passes", C.c_int)]
 
 
 class SimulationParameters(C.Structure):
@@ -149,6 +150,8 @@ SIGNATURES = {
     "pfem_pfem2_matrix": (_i, [_vp, _i, _pp]),
     "pfem_pfem2_precond": (_i, [_vp, _i, _pp]),
     "pfem_time_kernel": (_i, [_i, _vp, _vp, _i, C.POINTER(_d)]),
+    "pfem_solver_solve_pair": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "pfem_solver_set_batch_tiles": (_i, [_vp, _i, _vp, _vp, _vp]),
     "pfem_pfem2_sweep_fallbacks": (_i, [_vp]),
     "pfem_export_vtk": (_i, [_vp, C.c_char_p, _i, _vp, _vp]),
     "pfem_export_vtk3d": (_i, [_vp, C.c_char_p, _i, _vp, _vp]),
@@ -494,6 +497,17 @@ class Solver(_Handle):
         self.ctx.check(lib().pfem_solver_solve(self.h, a.h, x.ptr, b.ptr, C.byref(st)))
         return st
 
+    def solve_pair(self, a0, x0, b0, other, a1, x1, b1):
+        """this solver on a0 x0 = b0 and `other` on a1 x1 = b1, together (pfem_solver_solve_pair)"""
+        st = (SolveStats * 2)()
+        self.ctx.check(lib().pfem_solver_solve_pair(self.h, a0.h, x0.ptr, b0.ptr, other.h, a1.h, x1.ptr, b1.ptr, st))
+        return list(st)
+
+    def set_batch_tiles(self, buckets):
+        """buckets: [(max_rows, tile_rows, tile_nnz), ...] ascending in max_rows (pfem_solver_set_batch_tiles)"""
+        arr = [np.ascontiguousarray([bk[i] for bk in buckets], np.int32) for i in range(3)]
+        self.ctx.check(lib().pfem_solver_set_batch_tiles(self.h, len(buckets), _ptr(arr[0]), _ptr(arr[1]), _ptr(arr[2])))
+
     def solve_batched(self, a, sys_rows, x, b):
         """Independent systems of a block-diagonal matrix (system k: rows sys_rows[k]:sys_rows[k+1]);
         returns (stats list, per-system time to solution in ms)."""
@@ -701,7 +715,8 @@ def time_kernel(ctx, op, csr_handle, precond_handle=None, launches=50):
     """average ms per launch (HIP events on the context stream): 0 SpMV FP64, 1 SpMV FP32,
     2 preconditioner apply FP64, 3 preconditioner apply FP32, 4 apply with the FP32-stored factor,
     5 / 6 ops 2 / 4 in the factor's ordering, 7 SpMV FP64 in the factor's ordering, 8 / 9 ops 6 / 5 with
-    the tagged sweeps (PCG fusion level 3)"""
+    the tagged sweeps (PCG fusion level 3), 10 / 11 one fused PCG iteration with the FP64 / reduced-precision
+    factor (solves of 2L and L iterations differenced)"""
     ms = C.c_double()
     ctx.check(lib().pfem_time_kernel(op, csr_handle, precond_handle, launches, C.byref(ms)))
     return ms.value

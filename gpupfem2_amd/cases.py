@@ -228,7 +228,7 @@ def build_batched_poisson(ctx, case):
     return mesh, a, ctx.to_device(case["rhs"])
 
 
-def step_config(case, precision=0, inner_tol=1e-6, pres_ordering=1, pres_factor_bits=16):
+def step_config(case, precision=0, inner_tol=1e-6, pres_ordering=1, pres_factor_bits=16, pred_gs_passes=0):
     """pfem_step_config (include/pfem2.h) for a case; in mixed precision a multicolour IC0 pressure factor
     is applied from a 16-bit copy by default."""
     from .capi import StepConfig, make_params
@@ -248,17 +248,18 @@ def step_config(case, precision=0, inner_tol=1e-6, pres_ordering=1, pres_factor_
     cfg.inner_tol = inner_tol
     cfg.pres_ordering = pres_ordering
     cfg.pres_factor_bits = pres_factor_bits
+    cfg.pred_gs_passes = pred_gs_passes
     return cfg
 
 
-def build_gpu(ctx, case, precision=0, inner_tol=1e-6, pres_ordering=1):
+def build_gpu(ctx, case, precision=0, inner_tol=1e-6, pres_ordering=1, pred_gs_passes=0):
     """Mesh, particle handler (seeded, velocities initialised) and stepper on the GPU."""
     from . import capi
     mesh = capi.Mesh2D(ctx, case.xy, case.tri)
     mesh.set_edge_ids(case.edge_ids)
     ph = capi.Particles(mesh, case.level)
     ph.seed(case.seed_mode, case.ppc, case.seed)
-    cfg = step_config(case, precision, inner_tol, pres_ordering)
+    cfg = step_config(case, precision, inner_tol, pres_ordering, pred_gs_passes=pred_gs_passes)
     stepper = capi.Pfem2(mesh, ph, cfg, case.bcu_node, case.bcu_val0, case.bcu_val1, case.bcp_node, case.bcp_val)
     u0 = stepper.field_ptr("u0")
     u1 = stepper.field_ptr("u1")

@@ -42,6 +42,7 @@ struct BSys {
 struct BatchLayout {
     const Pattern* pat = nullptr;
     std::vector<int> rows, order, h_stile;
+    std::vector<int> tiling;   // the solver's tile parameters the layout was cut with
     int nsys = 0, ntiles = 0;
     DevArray<int> tiles, tsys, stile, psys;
 };
@@ -423,7 +424,22 @@ __global__ void k_b_to32(long n, float* __restrict__ d, const double* __restrict
 }
 
 // ---- host side
-static void build_layout(Context* c, const Pattern& P, const std::vector<int>& rows, BatchLayout& L)
+// Tile limits of a system of nr rows: the first bucket whose row bound holds it (tiling = {bound, rows,
+// non-zeros} per bucket); by default (no buckets) 256 rows and kTileNnz non-zeros.
+static void tile_limits(const std::vector<int>& tiling, int nr, int& max_rows, int& max_nnz)
+{
+    max_rows = kBlock;
+    max_nnz = kTileNnz;
+    for (size_t k = 0; k + 2 < tiling.size(); k += 3)
+        if (nr <= tiling[k]) {
+            max_rows = tiling[k + 1];
+            max_nnz = tiling[k + 2];
+            return;
+        }
+}
+
+static void build_layout(Context* c, const Pattern& P, const std::vector<int>& rows, const std::vector<int>& tiling,
+                         BatchLayout& L)
 {
     const int nsys = (int)rows.size() - 1;
     PFEM_REQUIRE(nsys >= 1 && rows[0] == 0 && rows[nsys] == P.n, "system row ranges must cover the matrix");
@@ -442,11 +458,13 @@ static void build_layout(Context* c, const Pattern& P, const std::vector<int>& r
     for (int q = 0; q < nsys; ++q) {
         const int s = order[q];
         stile[q] = (int)tsys.size();
+        int max_rows, max_nnz;
+        tile_limits(tiling, rows[s + 1] - rows[s], max_rows, max_nnz);
         int r = rows[s];
         while (r < rows[s + 1]) {
             int r1 = r;
             const int base = P.h_rowptr[r];
-            while (r1 < rows[s + 1] && r1 - r < kBlock && (P.h_rowptr[r1 + 1] - base <= kTileNnz || r1 == r)) ++r1;
+            while (r1 < rows[s + 1] && r1 - r < max_rows && (P.h_rowptr[r1 + 1] - base <= max_nnz || r1 == r)) ++r1;
             tiles.push_back(r);
             tiles.push_back(r1);
             tsys.push_back(s);
@@ -456,6 +474,7 @@ static void build_layout(Context* c, const Pattern& P, const std::vector<int>& r
     stile[nsys] = (int)tsys.size();
     L.pat = &P;
     L.rows = rows;
+    L.tiling = tiling;
     L.order = order;
     L.h_stile = stile;
     L.nsys = nsys;
@@ -571,10 +590,10 @@ static void solve_batched(Context* c, pfem_solver_t* s, pfem_csr_t* a, const std
 {
     const Pattern& P = *a->pat;
     const int n = P.n;
-    if (!s->batch || s->batch->L.pat != &P || s->batch->L.rows != rows) {
+    if (!s->batch || s->batch->L.pat != &P || s->batch->L.rows != rows || s->batch->L.tiling != s->batch_tiling) {
         s->batch.reset(new BatchWork());
         BatchWork& w = *s->batch;
-        build_layout(c, P, rows, w.L);
+        build_layout(c, P, rows, s->batch_tiling, w.L);
         w.n = n;
         w.r.alloc(n); w.p.alloc(n); w.q.alloc(n);
         w.part.alloc(2 * (size_t)w.L.ntiles);
@@ -621,6 +640,23 @@ void BatchWorkDeleter::operator()(BatchWork* w) const { delete w; }
 using namespace pfem;
 
 extern "C" {
+
+int pfem_solver_set_batch_tiles(pfem_solver s, int nbuckets, const int* max_rows, const int* tile_rows,
+                                const int* tile_nnz)
+{
+    PFEM_TRY(s ? s->ctx : nullptr, {
+        PFEM_REQUIRE(s && nbuckets >= 0 && nbuckets <= 16, "bad arguments");
+        PFEM_REQUIRE(nbuckets == 0 || (max_rows && tile_rows && tile_nnz), "null argument");
+        std::vector<int> t;
+        for (int k = 0; k < nbuckets; ++k) {
+            PFEM_REQUIRE(k == 0 || max_rows[k] > max_rows[k - 1], "bucket row bounds must ascend");
+            PFEM_REQUIRE(tile_rows[k] >= 1 && tile_rows[k] <= kBlock, "tile rows must lie in [1, 256]");
+            PFEM_REQUIRE(tile_nnz[k] >= 1 && tile_nnz[k] <= kTileNnz, "tile non-zeros must lie in [1, 2048]");
+            t.insert(t.end(), {max_rows[k], tile_rows[k], tile_nnz[k]});
+        }
+        s->batch_tiling = t;
+    });
+}
 
 int pfem_solver_solve_batched(pfem_solver s, pfem_csr a, int nsys, const int* h_sys_rows, double* d_x,
                               const double* d_b, pfem_solve_stats* stats, double* h_time_ms)

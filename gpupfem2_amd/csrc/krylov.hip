@@ -824,46 +824,64 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
     PFEM_HIP(hipStreamSynchronize(c->stream));
 }
 
+// One GMRES(m) system's launch sequence (SolverGMRES::solve, linear_solver.cu:421-521), split into the
+// pieces between the Arnoldi SpMVs so that gmres_run_pair can advance two systems in lockstep.
 template <typename T>
-static void gmres_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, GmState* st, GmState* hst)
-{
-    const Pattern& P = *A.P;
-    const int n = P.n;
-    const unsigned g = vec_grid(n);
-    const bool jac = A.kind == PFEM_PRECOND_JACOBI, none = A.kind == PFEM_PRECOND_NONE;
-    const int m = s->m;
-    const long ld = n;
-    T* V = A.V;
-    GmState init{};
-    init.tol = A.tol;
-    init.tol_mode = A.tol_mode;
-    init.maxit = A.maxit;
-    init.m = m;
-    init.restarted = s->restarted;
-    init.thr = -1.0;
-    init.H = s->H.p;
-    init.beta = s->beta.p;
-    init.cs = s->cs.p;
-    init.sn = s->sn.p;
-    init.y = s->y.p;
-    init.hc = s->hc.p;
-    PFEM_HIP(hipMemcpyAsync(st, &init, sizeof(GmState), hipMemcpyHostToDevice, c->stream));
+struct GmresSys {
+    Context* c;
+    pfem_solver_t* s;
+    KArgs<T> A;
+    GmState* st;
+    GmState* hst;
     bool first = true;
-    while (true) {
-        int nb0 = (int)g;
-        if (jac || none) {
+    bool done = false;
+
+    int n() const { return A.P->n; }
+    unsigned g() const { return vec_grid(A.P->n); }
+    bool jac() const { return A.kind == PFEM_PRECOND_JACOBI; }
+    bool none() const { return A.kind == PFEM_PRECOND_NONE; }
+    T* v(int j) const { return A.V + (size_t)j * A.P->n; }
+
+    void init()
+    {
+        GmState ini{};
+        ini.tol = A.tol;
+        ini.tol_mode = A.tol_mode;
+        ini.maxit = A.maxit;
+        ini.m = s->m;
+        ini.restarted = s->restarted;
+        ini.thr = -1.0;
+        ini.H = s->H.p;
+        ini.beta = s->beta.p;
+        ini.cs = s->cs.p;
+        ini.sn = s->sn.p;
+        ini.y = s->y.p;
+        ini.hc = s->hc.p;
+        PFEM_HIP(hipMemcpyAsync(st, &ini, sizeof(GmState), hipMemcpyHostToDevice, c->stream));
+        first = true;
+        done = false;
+    }
+    // v_0 = M^-1 (b - A x) / ||.||, with the stopping threshold on the first cycle
+    void cycle_start()
+    {
+        const Pattern& P = *A.P;
+        const int nn = n();
+        const unsigned gg = g();
+        T* V = A.V;
+        int nb0 = (int)gg;
+        if (jac() || none()) {
             if (first && A.x_zero) {
-                if (jac) hipLaunchKernelGGL((k_gm_init_zero<T, true>), dim3(g), dim3(kBlock), 0, c->stream, n, A.b, A.invd, A.x, V, s->pa);
-                else hipLaunchKernelGGL((k_gm_init_zero<T, false>), dim3(g), dim3(kBlock), 0, c->stream, n, A.b, A.invd, A.x, V, s->pa);
+                if (jac()) hipLaunchKernelGGL((k_gm_init_zero<T, true>), dim3(gg), dim3(kBlock), 0, c->stream, nn, A.b, A.invd, A.x, V, s->pa);
+                else hipLaunchKernelGGL((k_gm_init_zero<T, false>), dim3(gg), dim3(kBlock), 0, c->stream, nn, A.b, A.invd, A.x, V, s->pa);
             } else {
                 nb0 = P.ntiles;
-                if (jac) hipLaunchKernelGGL((k_gm_residual<T, true>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.x, A.b, A.invd, V, s->pa);
+                if (jac()) hipLaunchKernelGGL((k_gm_residual<T, true>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.x, A.b, A.invd, V, s->pa);
                 else hipLaunchKernelGGL((k_gm_residual<T, false>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.x, A.b, A.invd, V, s->pa);
             }
         } else {
             if (first && A.x_zero) {
-                PFEM_HIP(hipMemsetAsync(A.x, 0, sizeof(T) * n, c->stream));
-                PFEM_HIP(hipMemcpyAsync(V, A.b, sizeof(T) * n, hipMemcpyDeviceToDevice, c->stream));
+                PFEM_HIP(hipMemsetAsync(A.x, 0, sizeof(T) * nn, c->stream));
+                PFEM_HIP(hipMemcpyAsync(V, A.b, sizeof(T) * nn, hipMemcpyDeviceToDevice, c->stream));
             } else {
                 hipLaunchKernelGGL((k_residual<T>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, A.x, A.b, V);
             }
@@ -873,41 +891,161 @@ static void gmres_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, GmState* 
                 if (A.x_zero) t = V;
                 else { papply<T>(c, A, A.t, A.b); t = A.t; }
             }
-            hipLaunchKernelGGL((k_gm_norms<T>), dim3(g), dim3(kBlock), 0, c->stream, n, V, t, s->pa);
+            hipLaunchKernelGGL((k_gm_norms<T>), dim3(gg), dim3(kBlock), 0, c->stream, nn, V, t, s->pa);
         }
         hipLaunchKernelGGL(k_gm_begin, dim3(1), dim3(kBlock), 0, c->stream, st, (const double*)s->pa, nb0);
         first = false;
-        hipLaunchKernelGGL((k_gm_scale<T>), dim3(g), dim3(kBlock), 0, c->stream, n, V, st);
-        for (int j0 = 0; j0 < m; j0 += s->poll) {
-            const int j1 = std::min(m, j0 + s->poll);
+        hipLaunchKernelGGL((k_gm_scale<T>), dim3(gg), dim3(kBlock), 0, c->stream, nn, V, st);
+    }
+    // w = M^-1 A v_j into v_{j+1}
+    void spmv(int j)
+    {
+        const Pattern& P = *A.P;
+        if (jac()) hipLaunchKernelGGL((k_gm_arnoldi_spmv<T, true>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, v(j), A.invd, v(j + 1), st);
+        else hipLaunchKernelGGL((k_gm_arnoldi_spmv<T, false>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, v(j), A.invd, v(j + 1), st);
+    }
+    // the rest of Arnoldi step j: preconditioner (other than Jacobi), Gram-Schmidt, Givens, normalisation
+    void arnoldi_rest(int j)
+    {
+        const int nn = n();
+        const unsigned gg = g();
+        const long ld = nn;
+        T* w = v(j + 1);
+        if (!jac() && !none()) papply<T>(c, A, w, w);
+        const int passes = s->reorth ? 2 : 1;
+        for (int pass = 0; pass < passes; ++pass) {
+            for (int i0 = 0; i0 <= j; i0 += 8)
+                hipLaunchKernelGGL((k_gm_multidot<T>), dim3(gg), dim3(kBlock), 0, c->stream, nn, A.V, ld, i0, std::min(8, j + 1 - i0), w, st, s->pb);
+            hipLaunchKernelGGL(k_gm_reduce_h, dim3(j + 1), dim3(kBlock), 0, c->stream, st, (const double*)s->pb, (int)gg, pass);
+            hipLaunchKernelGGL((k_gm_orth<T>), dim3(gg), dim3(kBlock), 0, c->stream, nn, A.V, ld, w, st, s->pn, pass == passes - 1 ? 1 : 0);
+        }
+        hipLaunchKernelGGL(k_gm_finish, dim3(1), dim3(kBlock), 0, c->stream, st, (const double*)s->pn, (int)gg);
+        hipLaunchKernelGGL((k_gm_scale<T>), dim3(gg), dim3(kBlock), 0, c->stream, nn, w, st);
+    }
+    void read_state() { PFEM_HIP(hipMemcpyAsync(hst, st, sizeof(GmState), hipMemcpyDeviceToHost, c->stream)); }
+    // y = H^-1 beta, x += V y (when the cycle converged, or every cycle of GMRES(m))
+    void finish_cycle()
+    {
+        hipLaunchKernelGGL(k_gm_solve_y, dim3(1), dim3(64), 0, c->stream, st);
+        hipLaunchKernelGGL((k_gm_update_x<T>), dim3(g()), dim3(kBlock), 0, c->stream, n(), A.V, (long)n(), A.x, st);
+    }
+    // after finish_cycle and a state read: true when the solve is over, else the next cycle is armed
+    bool next_cycle()
+    {
+        if (hst->converged || hst->nan || hst->it >= A.maxit || !s->restarted) {
+            done = true;
+            return true;
+        }
+        hipLaunchKernelGGL(k_gm_rearm, dim3(1), dim3(64), 0, c->stream, st);
+        return false;
+    }
+};
+
+template <typename T>
+static void gmres_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, GmState* st, GmState* hst)
+{
+    GmresSys<T> g{c, s, A, st, hst};
+    g.init();
+    while (true) {
+        g.cycle_start();
+        for (int j0 = 0; j0 < s->m; j0 += s->poll) {
+            const int j1 = std::min(s->m, j0 + s->poll);
             for (int j = j0; j < j1; ++j) {
-                T* vj = V + (size_t)j * ld;
-                T* w = V + (size_t)(j + 1) * ld;
-                if (jac) hipLaunchKernelGGL((k_gm_arnoldi_spmv<T, true>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, vj, A.invd, w, st);
-                else hipLaunchKernelGGL((k_gm_arnoldi_spmv<T, false>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, A.val, vj, A.invd, w, st);
-                if (!jac && !none) papply<T>(c, A, w, w);
-                const int passes = s->reorth ? 2 : 1;
-                for (int pass = 0; pass < passes; ++pass) {
-                    for (int i0 = 0; i0 <= j; i0 += 8)
-                        hipLaunchKernelGGL((k_gm_multidot<T>), dim3(g), dim3(kBlock), 0, c->stream, n, V, ld, i0, std::min(8, j + 1 - i0), w, st, s->pb);
-                    hipLaunchKernelGGL(k_gm_reduce_h, dim3(j + 1), dim3(kBlock), 0, c->stream, st, (const double*)s->pb, (int)g, pass);
-                    hipLaunchKernelGGL((k_gm_orth<T>), dim3(g), dim3(kBlock), 0, c->stream, n, V, ld, w, st, s->pn, pass == passes - 1 ? 1 : 0);
-                }
-                hipLaunchKernelGGL(k_gm_finish, dim3(1), dim3(kBlock), 0, c->stream, st, (const double*)s->pn, (int)g);
-                hipLaunchKernelGGL((k_gm_scale<T>), dim3(g), dim3(kBlock), 0, c->stream, n, w, st);
+                g.spmv(j);
+                g.arnoldi_rest(j);
             }
             PFEM_HIP(hipGetLastError());
-            PFEM_HIP(hipMemcpyAsync(hst, st, sizeof(GmState), hipMemcpyDeviceToHost, c->stream));
+            g.read_state();
             PFEM_HIP(hipStreamSynchronize(c->stream));
             if (hst->cycle_done) break;
         }
-        hipLaunchKernelGGL(k_gm_solve_y, dim3(1), dim3(64), 0, c->stream, st);
-        hipLaunchKernelGGL((k_gm_update_x<T>), dim3(g), dim3(kBlock), 0, c->stream, n, V, ld, A.x, st);
+        g.finish_cycle();
         PFEM_HIP(hipGetLastError());
-        PFEM_HIP(hipMemcpyAsync(hst, st, sizeof(GmState), hipMemcpyDeviceToHost, c->stream));
+        g.read_state();
         PFEM_HIP(hipStreamSynchronize(c->stream));
-        if (hst->converged || hst->nan || hst->it >= A.maxit || !s->restarted) break;
-        hipLaunchKernelGGL(k_gm_rearm, dim3(1), dim3(64), 0, c->stream, st);
+        if (g.next_cycle()) break;
+    }
+}
+
+// w_k = M_k^-1 A_k v_k for two systems on one pattern (Jacobi or no preconditioner) in one launch: each
+// tile stages the first system's products in LDS, sums its rows, then does the same for the second, so
+// the tile's column indices come from HBM once (the second pass finds them in L2) with the LDS of a
+// single-system tile; a system whose cycle has ended is skipped. Each row is summed in the order of
+// k_gm_arnoldi_spmv (the same bits as two single launches).
+template <typename T, bool JAC>
+__global__ __launch_bounds__(kBlock) void k_gm_arnoldi_spmv_pair(const int* __restrict__ tiles, const int* __restrict__ rowptr,
+                                                                 const int* __restrict__ col, const T* __restrict__ val0,
+                                                                 const T* __restrict__ val1, const T* __restrict__ v0,
+                                                                 const T* __restrict__ v1, const T* __restrict__ invd0,
+                                                                 const T* __restrict__ invd1, T* __restrict__ w0,
+                                                                 T* __restrict__ w1, const GmState* st0, const GmState* st1)
+{
+    __shared__ T prod[kTileNnz];
+    const bool on[2] = {!st0->cycle_done, !st1->cycle_done};
+    if (!on[0] && !on[1]) return;
+    const T* val[2] = {val0, val1};
+    const T* v[2] = {v0, v1};
+    const T* invd[2] = {invd0, invd1};
+    T* w[2] = {w0, w1};
+    for (int q = 0; q < 2; ++q) {
+        if (!on[q]) continue;
+        int row, r1;
+        const T av = tile_row_product<T, T>(tiles, rowptr, col, val[q], v[q], prod, row, r1);
+        if (row < r1) w[q][row] = JAC ? av * invd[q][row] : av;
+        __syncthreads();   // the LDS tile is reused by the second system
+    }
+}
+
+// Two GMRES(m) solves with one pattern (the velocity prediction's two components, whose matrices share
+// the mesh's CSR structure; the reference solves them one after the other, Cylinder2D/main.cu:820-824):
+// both systems advance in lockstep, their Arnoldi SpMVs in one launch that streams the pattern once, and
+// one host poll serves both. Each system keeps its own recurrence, scalars and stopping test, so its
+// iterates are those of a single solve (the SpMV sums each row in the same order).
+template <typename T>
+static void gmres_run_pair(Context* c, GmresSys<T>& g0, GmresSys<T>& g1)
+{
+    GmresSys<T>* g[2] = {&g0, &g1};
+    const bool fused = g0.A.P == g1.A.P && g0.jac() == g1.jac() && (g0.jac() || g0.none()) && g0.s->m == g1.s->m;
+    for (auto* x : g) x->init();
+    while (!g0.done || !g1.done) {
+        for (auto* x : g)
+            if (!x->done) x->cycle_start();
+        const int m = std::max(g0.s->m, g1.s->m), poll = std::min(g0.s->poll, g1.s->poll);
+        for (int j0 = 0; j0 < m; j0 += poll) {
+            const int j1 = std::min(m, j0 + poll);
+            for (int j = j0; j < j1; ++j) {
+                // a finished system's state stays cycle_done, so its launches return at once
+                if (fused && !g0.done && !g1.done) {
+                    const Pattern& P = *g0.A.P;
+                    if (g0.jac())
+                        hipLaunchKernelGGL((k_gm_arnoldi_spmv_pair<T, true>), dim3(P.ntiles), dim3(kBlock), 0, c->stream,
+                                           P.tiles.p, P.rowptr.p, P.col.p, g0.A.val, g1.A.val, g0.v(j), g1.v(j),
+                                           g0.A.invd, g1.A.invd, g0.v(j + 1), g1.v(j + 1), g0.st, g1.st);
+                    else
+                        hipLaunchKernelGGL((k_gm_arnoldi_spmv_pair<T, false>), dim3(P.ntiles), dim3(kBlock), 0, c->stream,
+                                           P.tiles.p, P.rowptr.p, P.col.p, g0.A.val, g1.A.val, g0.v(j), g1.v(j),
+                                           g0.A.invd, g1.A.invd, g0.v(j + 1), g1.v(j + 1), g0.st, g1.st);
+                } else {
+                    for (auto* x : g)
+                        if (!x->done && j < x->s->m) x->spmv(j);
+                }
+                for (auto* x : g)
+                    if (!x->done && j < x->s->m) x->arnoldi_rest(j);
+            }
+            PFEM_HIP(hipGetLastError());
+            for (auto* x : g)
+                if (!x->done) x->read_state();
+            PFEM_HIP(hipStreamSynchronize(c->stream));
+            if ((g0.done || g0.hst->cycle_done) && (g1.done || g1.hst->cycle_done)) break;
+        }
+        for (auto* x : g)
+            if (!x->done) x->finish_cycle();
+        PFEM_HIP(hipGetLastError());
+        for (auto* x : g)
+            if (!x->done) x->read_state();
+        PFEM_HIP(hipStreamSynchronize(c->stream));
+        for (auto* x : g)
+            if (!x->done) x->next_cycle();
     }
 }
 
@@ -1018,44 +1156,61 @@ void gmres_solve_fp64(Context* c, pfem_solver_t* s, pfem_csr_t* a, double* x, co
 // FP64 iterative refinement around FP32 inner solves: r = b - A x and the stopping test in FP64
 // (the same criterion as the FP64 solver), the correction A d = r / ||r|| solved in FP32 to the
 // relative tolerance inner_tol, x += ||r|| d in FP64.
-void solve_mixed(Context* c, pfem_solver_t* s, pfem_csr_t* a, double* x, const double* b, pfem_solve_stats* out)
-{
-    const Pattern& P = *a->pat;
-    const int n = P.n;
-    const unsigned g = vec_grid(n);
-    if (!a->val32.p) a->val32.alloc(P.nnz);
-    hipLaunchKernelGGL(k_to32, dim3(blocks_for(P.nnz)), dim3(kBlock), 0, c->stream, (long)P.nnz, a->val32.p, a->val.p);
-    if (s->pc) precond_initialize(c, s->pc, a);   // also refreshes the FP32 copies of the factor
-    const int kind = s->pc ? s->pc->kind : PFEM_PRECOND_NONE;
-    const bool jac = kind == PFEM_PRECOND_JACOBI, none = kind == PFEM_PRECOND_NONE;
-    const bool gm = s->kind == PFEM_SOLVER_GMRES;
-    IrState* ist = reinterpret_cast<IrState*>(s->xr.p + 2 * (size_t)n);   // state lives after the two n-vectors
-    IrState init{};
-    init.first = 1;
-    PFEM_HIP(hipMemcpyAsync(ist, &init, sizeof(IrState), hipMemcpyHostToDevice, c->stream));
-    double* r = s->xr.p;
-    double* z = s->xr.p + n;
+// FP64 iterative refinement around FP32 inner solves of one system: r = b - A x and the stopping test in
+// FP64 (the same criterion as the FP64 solver), the correction A d = r / ||r|| solved in FP32 to the
+// relative tolerance inner_tol, x += ||r|| d in FP64. Split into pieces so that two systems can refine in
+// lockstep (solve_mixed_pair).
+struct MixedSys {
+    Context* c;
+    pfem_solver_t* s;
+    pfem_csr_t* a;
+    double* x;
+    const double* b;
+    IrState* ist = nullptr;
+    KArgs<float> A{};
     int total_inner = 0, refinements = 0, conv = 0, nan = 0;
     double res = 0.0;
-    KArgs<float> A{};
-    A.P = &P;
-    A.val = a->val32.p;
-    A.kind = kind;
-    A.invd = jac ? s->pc->invd32.p : nullptr;
-    A.pc = s->pc;
-    A.x = s->d32.p;
-    A.b = s->r32.p;
-    A.x_zero = true;
-    A.tol = s->inner_tol;
-    A.tol_mode = PFEM_TOL_RELATIVE;
-    A.r = s->z32.p; A.z = s->p32.p; A.p = s->q32.p; A.q = s->t32.p; A.t = nullptr; A.V = s->V32.p;
-    IrState h{};
-    while (true) {
+    bool active = true;
+
+    bool gm() const { return s->kind == PFEM_SOLVER_GMRES; }
+    void start()
+    {
+        const Pattern& P = *a->pat;
+        if (!a->val32.p) a->val32.alloc(P.nnz);
+        hipLaunchKernelGGL(k_to32, dim3(blocks_for(P.nnz)), dim3(kBlock), 0, c->stream, (long)P.nnz, a->val32.p, a->val.p);
+        if (s->pc) precond_initialize(c, s->pc, a);   // also refreshes the FP32 copies of the factor
+        ist = reinterpret_cast<IrState*>(s->xr.p + 2 * (size_t)P.n);   // state lives after the two n-vectors
+        IrState init{};
+        init.first = 1;
+        PFEM_HIP(hipMemcpyAsync(ist, &init, sizeof(IrState), hipMemcpyHostToDevice, c->stream));
+        const int kind = s->pc ? s->pc->kind : PFEM_PRECOND_NONE;
+        A.P = &P;
+        A.val = a->val32.p;
+        A.kind = kind;
+        A.invd = kind == PFEM_PRECOND_JACOBI ? s->pc->invd32.p : nullptr;
+        A.pc = s->pc;
+        A.x = s->d32.p;
+        A.b = s->r32.p;
+        A.x_zero = true;
+        A.tol = s->inner_tol;
+        A.tol_mode = PFEM_TOL_RELATIVE;
+        A.r = s->z32.p; A.z = s->p32.p; A.p = s->q32.p; A.q = s->t32.p; A.t = nullptr; A.V = s->V32.p;
+    }
+    // r = b - A x, its norms and the stopping test (launched; read with read_test)
+    void residual_test()
+    {
+        const Pattern& P = *a->pat;
+        const int n = P.n;
+        const unsigned g = vec_grid(n);
+        const int kind = A.kind;
+        const bool jac = kind == PFEM_PRECOND_JACOBI, none = kind == PFEM_PRECOND_NONE;
+        double* r = s->xr.p;
+        double* z = s->xr.p + n;
         int nb0 = (int)g;
         if (jac || none) {
             nb0 = P.ntiles;
             const double* invd = jac ? s->pc->invd.p : nullptr;
-            if (gm) {
+            if (gm()) {
                 if (jac) hipLaunchKernelGGL((k_ir_residual<true, true>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, a->val.p, x, b, invd, r, z, s->pa);
                 else hipLaunchKernelGGL((k_ir_residual<false, true>), dim3(P.ntiles), dim3(kBlock), 0, c->stream, P.tiles.p, P.rowptr.p, P.col.p, a->val.p, x, b, invd, r, z, s->pa);
             } else {
@@ -1070,37 +1225,118 @@ void solve_mixed(Context* c, pfem_solver_t* s, pfem_csr_t* a, double* x, const d
                 precond_apply(c, s->pc, s->t.p, b);
                 t = s->t.p;
             }
-            if (gm) hipLaunchKernelGGL((k_ir_norms<true>), dim3(g), dim3(kBlock), 0, c->stream, n, r, z, b, t, s->pa);
+            if (gm()) hipLaunchKernelGGL((k_ir_norms<true>), dim3(g), dim3(kBlock), 0, c->stream, n, r, z, b, t, s->pa);
             else hipLaunchKernelGGL((k_ir_norms<false>), dim3(g), dim3(kBlock), 0, c->stream, n, r, z, b, t, s->pa);
         }
-        if (gm) hipLaunchKernelGGL(k_ir_test<true>, dim3(1), dim3(kBlock), 0, c->stream, ist, (const double*)s->pa, nb0, s->tol, s->tol_mode);
+        if (gm()) hipLaunchKernelGGL(k_ir_test<true>, dim3(1), dim3(kBlock), 0, c->stream, ist, (const double*)s->pa, nb0, s->tol, s->tol_mode);
         else hipLaunchKernelGGL(k_ir_test<false>, dim3(1), dim3(kBlock), 0, c->stream, ist, (const double*)s->pa, nb0, s->tol, s->tol_mode);
         PFEM_HIP(hipMemcpyAsync(s->h_cg, ist, sizeof(IrState), hipMemcpyDeviceToHost, c->stream));
-        PFEM_HIP(hipStreamSynchronize(c->stream));
+    }
+    // after the stream is synchronised: false when the refinement is over
+    bool read_test()
+    {
+        IrState h{};
         std::memcpy(&h, s->h_cg, sizeof(IrState));
-        res = gm ? std::sqrt(h.rz) : std::sqrt(std::fabs(h.rz));
-        if (h.converged) { conv = 1; break; }
-        if (h.nan) { nan = 1; break; }
-        if (refinements >= s->max_refine || total_inner >= s->maxit) break;
-        hipLaunchKernelGGL(k_ir_down, dim3(g), dim3(kBlock), 0, c->stream, n, s->r32.p, r, ist);
+        res = gm() ? std::sqrt(h.rz) : std::sqrt(std::fabs(h.rz));
+        if (h.converged) conv = 1;
+        if (h.nan) nan = 1;
+        if (conv || nan || refinements >= s->max_refine || total_inner >= s->maxit) active = false;
+        return active;
+    }
+    // d's right-hand side r / ||r|| in FP32
+    void down()
+    {
+        const int n = a->pat->n;
+        hipLaunchKernelGGL(k_ir_down, dim3(vec_grid(n)), dim3(kBlock), 0, c->stream, n, s->r32.p, s->xr.p, ist);
         A.maxit = std::max(1, s->maxit - total_inner);
-        if (gm) {
-            gmres_run<float>(c, s, A, s->gm.p, s->h_gm);
-            total_inner += s->h_gm->it;
-        } else {
-            cg_run<float>(c, s, A, s->cg.p, s->h_cg);
-            total_inner += s->h_cg->it;
-        }
-        hipLaunchKernelGGL(k_ir_up, dim3(g), dim3(kBlock), 0, c->stream, n, x, s->d32.p, ist);
+    }
+    void up(int inner_its)
+    {
+        const int n = a->pat->n;
+        total_inner += inner_its;
+        hipLaunchKernelGGL(k_ir_up, dim3(vec_grid(n)), dim3(kBlock), 0, c->stream, n, x, s->d32.p, ist);
         ++refinements;
     }
-    precond_check(c, s->pc);
-    if (out) {
-        out->converged = conv;
-        out->iterations = total_inner;
-        out->final_residual = res;
-        out->nan_detected = nan;
-        out->refinements = refinements;
+    void finish(pfem_solve_stats* out)
+    {
+        precond_check(c, s->pc);
+        if (out) {
+            out->converged = conv;
+            out->iterations = total_inner;
+            out->final_residual = res;
+            out->nan_detected = nan;
+            out->refinements = refinements;
+        }
+    }
+};
+
+void solve_mixed(Context* c, pfem_solver_t* s, pfem_csr_t* a, double* x, const double* b, pfem_solve_stats* out)
+{
+    MixedSys m{c, s, a, x, b};
+    m.start();
+    while (true) {
+        m.residual_test();
+        PFEM_HIP(hipStreamSynchronize(c->stream));
+        if (!m.read_test()) break;
+        m.down();
+        if (m.gm()) {
+            gmres_run<float>(c, s, m.A, s->gm.p, s->h_gm);
+            m.up(s->h_gm->it);
+        } else {
+            cg_run<float>(c, s, m.A, s->cg.p, s->h_cg);
+            m.up(s->h_cg->it);
+        }
+    }
+    m.finish(out);
+}
+
+// Two GMRES systems on one pattern (the two velocity-prediction components): FP64 (the Arnoldi steps in
+// lockstep, gmres_run_pair), or mixed precision with both refinements in lockstep and their FP32 inner
+// GMRES solves run as a pair.
+static void gmres_solve_pair(Context* c, pfem_solver_t* s0, pfem_csr_t* a0, double* x0, const double* b0,
+                             pfem_solver_t* s1, pfem_csr_t* a1, double* x1, const double* b1, pfem_solve_stats* out)
+{
+    if (s0->precision == PFEM_PRECISION_MIXED) {
+        MixedSys m[2] = {{c, s0, a0, x0, b0}, {c, s1, a1, x1, b1}};
+        for (auto& q : m) q.start();
+        while (m[0].active || m[1].active) {
+            for (auto& q : m)
+                if (q.active) q.residual_test();
+            PFEM_HIP(hipStreamSynchronize(c->stream));
+            bool on[2];
+            for (int k = 0; k < 2; ++k) on[k] = m[k].active && m[k].read_test();
+            if (!on[0] && !on[1]) break;
+            for (int k = 0; k < 2; ++k)
+                if (on[k]) m[k].down();
+            if (on[0] && on[1]) {
+                GmresSys<float> g0{c, s0, m[0].A, s0->gm.p, s0->h_gm}, g1{c, s1, m[1].A, s1->gm.p, s1->h_gm};
+                gmres_run_pair<float>(c, g0, g1);
+            } else {
+                const int k = on[0] ? 0 : 1;
+                gmres_run<float>(c, m[k].s, m[k].A, m[k].s->gm.p, m[k].s->h_gm);
+            }
+            for (int k = 0; k < 2; ++k)
+                if (on[k]) m[k].up(m[k].s->h_gm->it);
+        }
+        m[0].finish(out);
+        m[1].finish(out ? out + 1 : nullptr);
+        return;
+    }
+    for (pfem_solver_t* s : {s0, s1})
+        if (s->pc) precond_initialize(c, s->pc, s == s0 ? a0 : a1);
+    GmresSys<double> g0{c, s0, args64(s0, a0, x0, b0), s0->gm.p, s0->h_gm};
+    GmresSys<double> g1{c, s1, args64(s1, a1, x1, b1), s1->gm.p, s1->h_gm};
+    gmres_run_pair<double>(c, g0, g1);
+    for (int k = 0; k < 2; ++k) {
+        pfem_solver_t* s = k ? s1 : s0;
+        precond_check(c, s->pc);
+        if (out) {
+            out[k].converged = s->h_gm->converged;
+            out[k].iterations = s->h_gm->it;
+            out[k].final_residual = s->h_gm->res;
+            out[k].nan_detected = s->h_gm->nan;
+            out[k].refinements = 0;
+        }
     }
 }
 
@@ -1239,10 +1475,53 @@ int pfem_solver_set_option(pfem_solver s, int option, int value)
     });
 }
 
+// ops 10 and 11: the fused PCG iteration as a solve runs it (FP64 factor, or the reduced-precision factor
+// copy of PFEM_PRECISION_MIXED_PRECOND), from x = 0 with b = 1 and a zero tolerance, so that every
+// iteration runs; the device time of a solve of 2L iterations minus that of L iterations, over L, on the
+// context stream (the solve's set-up, factorization included, cancels).
+static double time_pcg_iteration(Context* c, pfem_csr a, pfem_precond pc, bool reduced, int launches)
+{
+    const int n = a->pat->n;
+    pfem_solver s = nullptr;
+    if (pfem_solver_create(a, PFEM_SOLVER_CG, 0.0, launches, 0, PFEM_TOL_ABSOLUTE, pc, &s) != PFEM_OK)
+        throw Error(PFEM_EINVAL, c->err);
+    std::unique_ptr<pfem_solver_t, int (*)(pfem_solver)> guard(s, pfem_solver_destroy);
+    if (reduced && pfem_solver_set_precision(s, PFEM_PRECISION_MIXED_PRECOND, 0.0, 0) != PFEM_OK)
+        throw Error(PFEM_EINVAL, c->err);
+    DevArray<double> x(n), b(n);
+    launch_fill(c, b.p, 1.0, n);
+    hipEvent_t e0, e1;
+    PFEM_HIP(hipEventCreate(&e0));
+    PFEM_HIP(hipEventCreate(&e1));
+    float ms[2] = {0.f, 0.f};
+    for (int k = 0; k < 2; ++k) {
+        s->maxit = launches * (k + 1);
+        PFEM_HIP(hipEventRecord(e0, c->stream));
+        const int rc = pfem_solver_solve(s, a, x.p, b.p, nullptr);
+        PFEM_HIP(hipEventRecord(e1, c->stream));
+        PFEM_HIP(hipEventSynchronize(e1));
+        if (rc != PFEM_OK) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw Error(rc, c->err);
+        }
+        PFEM_HIP(hipEventElapsedTime(&ms[k], e0, e1));
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return (ms[1] - ms[0]) / launches;
+}
+
 int pfem_time_kernel(int op, pfem_csr a, pfem_precond pc, int launches, double* avg_ms)
 {
     PFEM_TRY(a ? a->ctx : nullptr, {
-        PFEM_REQUIRE(a && avg_ms && launches > 0 && op >= 0 && op <= 9, "bad arguments");
+        PFEM_REQUIRE(a && avg_ms && launches > 0 && op >= 0 && op <= 11, "bad arguments");
+        if (op >= 10) {
+            PFEM_REQUIRE(pc && (pc->kind == PFEM_PRECOND_IC0 || pc->kind == PFEM_PRECOND_ILU0),
+                         "ops 10 and 11 need an IC0 or ILU0 preconditioner");
+            *avg_ms = time_pcg_iteration(C(a->ctx), a, pc, op == 11, launches);
+            return PFEM_OK;
+        }
         PFEM_REQUIRE(op < 5 || (pc && pc->ordering != PFEM_ORDER_NATURAL && pc->sweep),
                      "ops 5 to 9 need a multicolour preconditioner with single-pass sweeps");
         PFEM_REQUIRE(op < 8 || pc->sell, "ops 8 and 9 need the sliced sweep layout");
@@ -1305,6 +1584,37 @@ int pfem_solver_destroy(pfem_solver s)
         if (e) (void)hipEventDestroy(e);
     delete s;
     return PFEM_OK;
+}
+
+int pfem_solver_solve_pair(pfem_solver s0, pfem_csr a0, double* d_x0, const double* d_b0, pfem_solver s1,
+                           pfem_csr a1, double* d_x1, const double* d_b1, pfem_solve_stats stats[2])
+{
+    PFEM_TRY(s0 ? s0->ctx : nullptr, {
+        PFEM_REQUIRE(s0 && s1 && a0 && a1 && d_x0 && d_b0 && d_x1 && d_b1, "null argument");
+        PFEM_REQUIRE(s0 != s1, "the pair needs two solver objects (each keeps its own Krylov state)");
+        PFEM_REQUIRE(s0->ctx == s1->ctx && a0->ctx == s0->ctx && a1->ctx == s0->ctx, "objects of different contexts");
+        PFEM_REQUIRE(a0->pat->n == s0->n && a1->pat->n == s1->n, "matrix size differs from the solver's");
+        Context* c = C(s0->ctx);
+        pfem_solve_stats local[2] = {};
+        const bool pairable = s0->kind == PFEM_SOLVER_GMRES && s1->kind == PFEM_SOLVER_GMRES &&
+                              s0->precision == s1->precision && s0->precision != PFEM_PRECISION_MIXED_PRECOND &&
+                              s0->m == s1->m && a0->pat.get() == a1->pat.get() &&
+                              (!s0->pc || s0->pc->kind == PFEM_PRECOND_JACOBI || s0->pc->kind == PFEM_PRECOND_NONE) &&
+                              (!s1->pc || s1->pc->kind == PFEM_PRECOND_JACOBI || s1->pc->kind == PFEM_PRECOND_NONE);
+        if (pairable) {
+            gmres_solve_pair(c, s0, a0, d_x0, d_b0, s1, a1, d_x1, d_b1, local);
+        } else {
+            // other combinations: the two solves one after the other
+            const int rc0 = pfem_solver_solve(s0, a0, d_x0, d_b0, &local[0]);
+            if (rc0 != PFEM_OK) throw Error(rc0, c->err);
+            const int rc1 = pfem_solver_solve(s1, a1, d_x1, d_b1, &local[1]);
+            if (rc1 != PFEM_OK) throw Error(rc1, c->err);
+        }
+        if (stats) {
+            stats[0] = local[0];
+            stats[1] = local[1];
+        }
+    });
 }
 
 int pfem_solver_solve(pfem_solver s, pfem_csr a, double* d_x, const double* d_b, pfem_solve_stats* stats)

@@ -34,8 +34,8 @@ struct pfem_pfem2_t {
     pfem_step_config cfg{};
     pfem_csr A0 = nullptr, A1 = nullptr, Ap = nullptr, Ac = nullptr;
     pfem_dirichlet bcu[2] = {nullptr, nullptr}, bcpred[2] = {nullptr, nullptr}, bcp = nullptr;
-    pfem_precond pc_pred = nullptr, pc_pres = nullptr, pc_corr = nullptr;
-    pfem_solver s_pred = nullptr, s_pres = nullptr, s_corr = nullptr;
+    pfem_precond pc_pred = nullptr, pc_pres = nullptr, pc_corr = nullptr, pc_pred1 = nullptr;
+    pfem_solver s_pred = nullptr, s_pres = nullptr, s_corr = nullptr, s_pred1 = nullptr;
     pfem::DevArray<double> f;   // u0 u1 p vp0 vp1 uo0 uo1 pold rhs0 rhs1 (n each) + 4 forces
     pfem::DevArray<double> xsave;   // a single-pass solve's start vector, kept for a per-level re-solve
     int sweep_fallbacks = 0;
@@ -48,7 +48,9 @@ struct pfem_pfem2_t {
     ~pfem_pfem2_t()
     {
         pfem_solver_destroy(s_pred); pfem_solver_destroy(s_pres); pfem_solver_destroy(s_corr);
+        pfem_solver_destroy(s_pred1);
         pfem_precond_destroy(pc_pred); pfem_precond_destroy(pc_pres); pfem_precond_destroy(pc_corr);
+        pfem_precond_destroy(pc_pred1);
         for (int k = 0; k < 2; ++k) { pfem_dirichlet_destroy(bcu[k]); pfem_dirichlet_destroy(bcpred[k]); }
         pfem_dirichlet_destroy(bcp);
         pfem_csr_destroy(A0); pfem_csr_destroy(A1); pfem_csr_destroy(Ap); pfem_csr_destroy(Ac);
@@ -164,8 +166,18 @@ int pfem_pfem2_create(pfem_mesh2d m, pfem_particles ph, const pfem_step_config* 
         make_solver(s.get(), s->A0, cfg->pred_solver, cfg->pred_precond, PFEM_ORDER_NATURAL, &s->pc_pred, &s->s_pred);
         // the prediction systems are mass-dominated (rho M + mu dt K): one classical Gram-Schmidt pass per
         // Arnoldi step keeps their iteration counts (cylinder, 2M triangles: prediction 9.0 -> 5.9 ms per
-        // step in FP64)
-        if (cfg->pred_solver == PFEM_SOLVER_GMRES) check(pfem_solver_set_option(s->s_pred, PFEM_SOLVER_OPT_GS_PASSES, 1), s->ctx);
+        // step in FP64); pred_gs_passes = 2 orthogonalises twice
+        PFEM_REQUIRE(cfg->pred_gs_passes >= 0 && cfg->pred_gs_passes <= 2, "pred_gs_passes must be 0, 1 or 2");
+        if (cfg->pred_solver == PFEM_SOLVER_GMRES) {
+            check(pfem_solver_set_option(s->s_pred, PFEM_SOLVER_OPT_GS_PASSES, cfg->pred_gs_passes == 2 ? 2 : 1),
+                  s->ctx);
+            // the second component's own solver: the two prediction systems are solved as a pair
+            // (pfem_solver_solve_pair: lockstep Arnoldi steps, one SpMV launch streaming the pattern once)
+            make_solver(s.get(), s->A1, cfg->pred_solver, cfg->pred_precond, PFEM_ORDER_NATURAL, &s->pc_pred1,
+                        &s->s_pred1);
+            check(pfem_solver_set_option(s->s_pred1, PFEM_SOLVER_OPT_GS_PASSES, cfg->pred_gs_passes == 2 ? 2 : 1),
+                  s->ctx);
+        }
         PFEM_REQUIRE(cfg->pres_factor_bits == 0 || cfg->pres_factor_bits == 16 || cfg->pres_factor_bits == 32,
                      "pres_factor_bits must be 0, 16 or 32");
         make_solver(s.get(), s->Ap, cfg->pres_solver, cfg->pres_precond, cfg->pres_ordering, &s->pc_pres, &s->s_pres,
@@ -221,8 +233,17 @@ int pfem_pfem2_step(pfem_pfem2 s, double forces[2])
             }
             {
                 Phase ph(s, 3);
-                solve(s, s->s_pred, s->A0, s->field(VP0), s->field(RHS0));
-                solve(s, s->s_pred, s->A1, s->field(VP1), s->field(RHS1));
+                if (s->s_pred1) {
+                    pfem_solve_stats st[2] = {};
+                    check(pfem_solver_solve_pair(s->s_pred, s->A0, s->field(VP0), s->field(RHS0), s->s_pred1, s->A1,
+                                                 s->field(VP1), s->field(RHS1), st),
+                          s->ctx);
+                    s->its.push_back(st[0].iterations);
+                    s->its.push_back(st[1].iterations);
+                } else {
+                    solve(s, s->s_pred, s->A0, s->field(VP0), s->field(RHS0));
+                    solve(s, s->s_pred, s->A1, s->field(VP1), s->field(RHS1));
+                }
             }
             {
                 Phase ph(s, 2);

@@ -138,6 +138,12 @@ __global__ void k_upper_level(int cnt, const int* __restrict__ rows, const int* 
 // column indices, factor values and first reads of up to kDeps dependencies are issued together, and
 // only dependencies that still read as the sentinel are polled
 constexpr int kDeps = 8;
+// Sliced layouts: each lane takes kLaneRows rows of its chunk (t, t + 256, ...), a chunk up to kSliceRows
+// rows of one level, the dependency loads of the lane's rows issued together. Two rows per lane (the
+// 1M-row pressure matrix's sweeps then fit the 256 CUs in one round of about 2,000 workgroups) measured
+// slower: the tagged 16-bit apply 22.7 -> 40.7 us, a PCG iteration 52.7 -> 69.1 us.
+constexpr int kLaneRows = 1;
+constexpr int kSliceRows = kLaneRows * kBlock;
 
 template <typename T, typename TS>
 __device__ __forceinline__ T dep_sum(T s, int p0, int p1, const int* __restrict__ col, const TS* __restrict__ f,
@@ -164,49 +170,6 @@ __device__ __forceinline__ T dep_sum(T s, int p0, int p1, const int* __restrict_
         for (int k = 0; k < kDeps; ++k)
             if (k < m) {
                 if (yv[k] == sent) yv[k] = __builtin_bit_cast(U, await_value(buf + cv[k], err, limit));
-                s -= fv[k] * __builtin_bit_cast(T, yv[k]);
-            }
-    }
-    return s;
-}
-
-// As dep_sum, over a row of a chunk-sliced triangle: entry w at slot e0 + w * stride (w < width), unused
-// slots marked by column -1 (they follow the row's entries, which keep their order). WAIT < 0: buf holds
-// the sentinel until published; WAIT = 0 or 1: buf is a tagged buffer (FP64) and a dependency is ready
-// once it carries tag WAIT.
-template <int WAIT, typename T, typename TS>
-__device__ __forceinline__ T dep_sum_sliced(T s, int e0, int stride, int width, const int* __restrict__ col32,
-                                            const unsigned short* __restrict__ col16, const ChunkBases& cb,
-                                            const TS* __restrict__ f, T* buf, int* err, int limit)
-{
-    using U = typename Sentinel<T>::U;
-    constexpr U sent = Sentinel<T>::bits;
-    for (int w0 = 0; w0 < width; w0 += kDeps) {
-        int cv[kDeps];
-        T fv[kDeps];
-        U yv[kDeps];
-#pragma unroll
-        for (int k = 0; k < kDeps; ++k) {
-            cv[k] = -1;
-            if (w0 + k < width) {
-                const int e = e0 + (w0 + k) * stride;
-                cv[k] = slot_col(e, col32, col16, cb);
-                fv[k] = (T)f[e];
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < kDeps; ++k)
-            if (cv[k] >= 0) yv[k] = __hip_atomic_load(reinterpret_cast<U*>(buf + cv[k]), __ATOMIC_RELAXED,
-                                                      __HIP_MEMORY_SCOPE_WORKGROUP);
-#pragma unroll
-        for (int k = 0; k < kDeps; ++k)
-            if (cv[k] >= 0) {
-                if constexpr (WAIT < 0) {
-                    if (yv[k] == sent) yv[k] = __builtin_bit_cast(U, await_value(buf + cv[k], err, limit));
-                } else {
-                    static_assert(sizeof(T) == 8, "tagged buffers hold FP64 values");
-                    if ((yv[k] & 1ull) != (U)WAIT) yv[k] = __builtin_bit_cast(U, await_tagged<WAIT>(buf + cv[k], err, limit));
-                }
                 s -= fv[k] * __builtin_bit_cast(T, yv[k]);
             }
     }
@@ -260,34 +223,97 @@ struct SweepArgs {
 // triangles), kSliced (a row range, the chunk-sliced triangle: header {first row, rows, base, width}).
 constexpr int kRows = 0, kRange = 1, kSliced = 2;
 
-// The calling lane's row and its dependency sum s - sum_k T_ik buf[k] over the sweep's triangle.
-template <int LAYOUT, typename TV, typename TS, typename TF>
-__device__ __forceinline__ bool sweep_row(const SweepArgs<TV, TS, TF>& a, int& i, int& e0, int& stride, int& width)
+// The calling lane's rows: one row (CSR layouts), or kLaneRows rows t, t + 256, ... of a sliced chunk
+// (header {first row, rows, base, width}); e0, stride and width locate a sliced row's entries.
+template <int LAYOUT, int R, typename TV, typename TS, typename TF>
+__device__ __forceinline__ bool sweep_rows(const SweepArgs<TV, TS, TF>& a, int (&i)[R], int (&e0)[R], bool (&has)[R],
+                                           int& stride, int& width)
 {
+    bool any = false;
     if constexpr (LAYOUT == kSliced) {
         const int* h = a.chunks + 4 * blockIdx.x;
         const int cnt = h[1];
-        i = h[0] + (int)threadIdx.x;
-        e0 = h[2] + (int)threadIdx.x;
         stride = cnt;
         width = h[3];
-        return (int)threadIdx.x < cnt;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int t = (int)threadIdx.x + r * kBlock;
+            i[r] = h[0] + t;
+            e0[r] = h[2] + t;
+            has[r] = t < cnt;
+            any |= has[r];
+        }
     } else {
+        static_assert(R == 1, "CSR layouts take one row per lane");
         const int c0 = a.chunks[2 * blockIdx.x], cnt = a.chunks[2 * blockIdx.x + 1];
-        i = LAYOUT == kRange ? c0 + (int)threadIdx.x : ((int)threadIdx.x < cnt ? a.rows[c0 + threadIdx.x] : 0);
-        e0 = stride = width = 0;
-        return (int)threadIdx.x < cnt;
+        has[0] = (int)threadIdx.x < cnt;
+        i[0] = LAYOUT == kRange ? c0 + (int)threadIdx.x : (has[0] ? a.rows[c0 + threadIdx.x] : 0);
+        e0[0] = stride = width = 0;
+        any = has[0];
+    }
+    return any;
+}
+
+// s[r] - sum_k T_ik buf[k] for each of the lane's rows, subtracted in ascending k (the sequential order).
+// Sliced rows: the column codes, factor values and first reads of the dependencies of all the lane's rows
+// (kDeps / R entries each) are issued together before any is consumed. WAIT < 0: buf holds the sentinel
+// until published; WAIT = 0 or 1: buf is a tagged buffer (FP64), a dependency is ready once it carries
+// tag WAIT.
+template <int LAYOUT, int WAIT, int R, typename TV, typename TS, typename TF>
+__device__ __forceinline__ void sweep_sums(const SweepArgs<TV, TS, TF>& a, TF (&s)[R], const int (&i)[R],
+                                          const int (&e0)[R], const bool (&has)[R], int stride, int width, TF* buf)
+{
+    if constexpr (LAYOUT == kSliced) {
+        using U = typename Sentinel<TF>::U;
+        constexpr U sent = Sentinel<TF>::bits;
+        constexpr int K = kDeps / R;
+        const ChunkBases cb = a.col16 ? chunk_bases(a.cbase + 4 * blockIdx.x) : ChunkBases{0, 0, 0, 0, true};
+        for (int w0 = 0; w0 < width; w0 += K) {
+            int cv[R][K];
+            TF fv[R][K];
+            U yv[R][K];
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    cv[r][k] = -1;
+                    if (has[r] && w0 + k < width) {
+                        const int e = e0[r] + (w0 + k) * stride;
+                        cv[r][k] = slot_col(e, a.col, a.col16, cb);
+                        fv[r][k] = (TF)a.f[e];
+                    }
+                }
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+                    if (cv[r][k] >= 0)
+                        yv[r][k] = __hip_atomic_load(reinterpret_cast<U*>(buf + cv[r][k]), __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+                    if (cv[r][k] >= 0) {
+                        if constexpr (WAIT < 0) {
+                            if (yv[r][k] == sent) yv[r][k] = __builtin_bit_cast(U, await_value(buf + cv[r][k], a.err, a.limit));
+                        } else {
+                            static_assert(sizeof(TF) == 8, "tagged buffers hold FP64 values");
+                            if ((yv[r][k] & 1ull) != (U)WAIT)
+                                yv[r][k] = __builtin_bit_cast(U, await_tagged<WAIT>(buf + cv[r][k], a.err, a.limit));
+                        }
+                        s[r] -= fv[r][k] * __builtin_bit_cast(TF, yv[r][k]);
+                    }
+        }
+    } else {
+        if (has[0]) s[0] = dep_sum(s[0], a.ptr[i[0]], a.ptr[i[0] + 1], a.col, a.f, buf, a.err, a.limit);
     }
 }
 
-template <int LAYOUT, int WAIT, typename TV, typename TS, typename TF>
-__device__ __forceinline__ TF sweep_sum(const SweepArgs<TV, TS, TF>& a, TF s, int i, int e0, int stride, int width, TF* buf)
+template <int LAYOUT>
+constexpr int lane_rows()
 {
-    if constexpr (LAYOUT == kSliced)
-        return dep_sum_sliced<WAIT>(s, e0, stride, width, a.col, a.col16,
-                              a.col16 ? chunk_bases(a.cbase + 4 * blockIdx.x) : ChunkBases{0, 0, 0, 0, true}, a.f,
-                              buf, a.err, a.limit);
-    else return dep_sum(s, a.ptr[i], a.ptr[i + 1], a.col, a.f, buf, a.err, a.limit);
+    return LAYOUT == kSliced ? kLaneRows : 1;
 }
 
 // TAGGED (sliced layout, FP64 published values): sy and sw are one buffer; the forward sweep publishes y
@@ -303,6 +329,7 @@ template <typename TV, typename TS, typename TF, bool UNIT, int LAYOUT, bool XR,
 __global__ __launch_bounds__(kBlock) void k_sweep_lower(SweepArgs<TV, TS, TF> a)
 {
     constexpr bool kYY = TAGGED && XR && !UNIT;
+    constexpr int R = lane_rows<LAYOUT>();
     __shared__ double wsum[kWaves];
     __shared__ unsigned arrived;
     if (a.done && *a.done) return;
@@ -311,36 +338,50 @@ __global__ __launch_bounds__(kBlock) void k_sweep_lower(SweepArgs<TV, TS, TF> a)
         if (threadIdx.x == 0) arrived = 0u;
         __syncthreads();
     }
-    int i, e0, stride, width;
-    const bool has = sweep_row<LAYOUT>(a, i, e0, stride, width);
-    if (!yy && !has) return;
+    int i[R], e0[R], stride, width;
+    bool has[R];
+    const bool any = sweep_rows<LAYOUT, R>(a, i, e0, has, stride, width);
+    if (!yy && !any) return;
     double v = 0.0;
-    if (has) {
-        TV b;
-        if constexpr (XR) {
-            // updateXR (linear_solver.cu:68-77) for this row, as the separate vector kernel computes it
-            // (the x half is left to the p-update kernel when a.x is null)
-            const TV al = (TV)(a.acg ? a.acg->alpha[a.acg->it & 1] : *a.alpha);
-            if (a.x) a.x[i] += al * a.p[i];
-            b = a.r[i] - al * a.q[i];
-            a.r[i] = b;
-        } else {
-            b = a.src[a.perm ? a.perm[i] : i];
+    if (any) {
+        TF s[R];
+        TV al = 0;
+        if constexpr (XR) al = (TV)(a.acg ? a.acg->alpha[a.acg->it & 1] : *a.alpha);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (!has[r]) continue;
+            TV b;
+            if constexpr (XR) {
+                // updateXR (linear_solver.cu:68-77) for this row, as the separate vector kernel computes it
+                // (the x half is left to the p-update kernel when a.x is null)
+                if (a.x) a.x[i[r]] += al * a.p[i[r]];
+                b = a.r[i[r]] - al * a.q[i[r]];
+                a.r[i[r]] = b;
+            } else {
+                b = a.src[a.perm ? a.perm[i[r]] : i[r]];
+            }
+            TF b0 = (TF)b;
+            if constexpr (sizeof(TS) == 2) b0 *= (TF)a.in_scale[1];
+            s[r] = b0;
         }
         if constexpr (TAGGED) {
             static_assert(LAYOUT == kSliced && sizeof(TF) == 8, "tagged sweeps: sliced layout, FP64 values");
-            TF b0 = (TF)b;
-            if constexpr (sizeof(TS) == 2) b0 *= (TF)a.in_scale[1];
-            const TF s = sweep_sum<LAYOUT, 1>(a, b0, i, e0, stride, width, a.sy);
-            const TF y = UNIT ? s : s / (TF)a.d[i];
-            publish_tagged<1>(a.sy + i, y);
-            v = (double)y * (double)y;
+            sweep_sums<LAYOUT, 1>(a, s, i, e0, has, stride, width, a.sy);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (!has[r]) continue;
+                const TF y = UNIT ? s[r] : s[r] / (TF)a.d[i[r]];
+                publish_tagged<1>(a.sy + i[r], y);
+                v += (double)y * (double)y;
+            }
         } else {
-            TF b0 = (TF)b;
-            if constexpr (sizeof(TS) == 2) b0 *= (TF)a.in_scale[1];
-            const TF s = sweep_sum<LAYOUT, -1>(a, b0, i, e0, stride, width, a.sy);
-            publish(a.sy + i, UNIT ? s : s / (TF)a.d[i]);
-            a.sw[i] = sentinel_value<TF>();
+            sweep_sums<LAYOUT, -1>(a, s, i, e0, has, stride, width, a.sy);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (!has[r]) continue;
+                publish(a.sy + i[r], UNIT ? s[r] : s[r] / (TF)a.d[i[r]]);
+                a.sw[i[r]] = sentinel_value<TF>();
+            }
         }
     }
     if constexpr (kYY) {
@@ -371,63 +412,47 @@ __global__ __launch_bounds__(kBlock) void k_sweep_lower(SweepArgs<TV, TS, TF> a)
 // backward solve, all levels: z_i = (y_i - sum_{k>i} U_ik z_k) / U_ii, out[perm[i]] = z_i; re-arms sy.
 // part != null: the chunk's partial of (r, z) (r = rdot, in the vectors' order) goes to part[chunk].
 //
-// TAGGED with xu set (the fused IC0-PCG, whose forward sweep has already recorded gamma and beta): each row
-// also gets the end-of-iteration vector updates as soon as its z is known, x += alpha_k p_k (the x half of
-// updateXR, linear_solver.cu:68-77) and p = z + beta p (updateP, :79-87), with the operations of
-// k_cg_xp_recorded, so no separate vector pass runs. Iteration k = it - 1; when it stopped the solve only
-// the x update runs, and a launch finding it equal to the xp_seen its SpMV left does nothing.
 template <typename TV, typename TS, typename TF, int LAYOUT, bool TAGGED = false>
 __global__ __launch_bounds__(kBlock) void k_sweep_upper(SweepArgs<TV, TS, TF> a)
 {
+    constexpr int R = lane_rows<LAYOUT>();
     // (r, z) partial without a workgroup barrier after the sweep: each wave leaves its sum in LDS and the
     // last wave to arrive adds the four in wave order (the order block_sum uses)
     __shared__ double wsum[kWaves];
     __shared__ unsigned arrived;
-    if constexpr (TAGGED) {
-        if (a.xu) {
-            const CgState* st = a.acg;
-            const int it = st->it;
-            if (it == st->xp_seen) return;
-            const int k = it - 1;
-            const bool last = st->done;
-            const TV al = (TV)st->alpha[k & 1];
-            int i, e0, stride, width;
-            if (!sweep_row<LAYOUT>(a, i, e0, stride, width)) return;
-            const TV pi = a.pu[i];
-            a.xu[i] += al * pi;
-            if (last) return;
-            const TV beta = (TV)(st->gam[(k + 1) & 1] / st->gam[k & 1]);
-            TF z = sweep_sum<LAYOUT, 0>(a, a.sy[i], i, e0, stride, width, a.sw);
-            z = tagged<0>(z / (TF)a.d[i]);
-            publish_tagged<0>(a.sw + i, z);
-            a.pu[i] = (TV)z + beta * pi;
-            return;
-        }
-    }
+    int i[R], e0[R], stride, width;
+    bool has[R];
     if (a.done && *a.done) return;
     if (a.part) {
         if (threadIdx.x == 0) arrived = 0u;
         __syncthreads();
     }
     double v = 0.0;
-    int i, e0, stride, width;
-    if (sweep_row<LAYOUT>(a, i, e0, stride, width)) {
-        const TF yi = a.sy[i];
-        TF s;
-        if constexpr (TAGGED) {
-            s = sweep_sum<LAYOUT, 0>(a, yi, i, e0, stride, width, a.sw);
-            s = tagged<0>(s / (TF)a.d[i]);
-            publish_tagged<0>(a.sw + i, s);
-        } else {
-            a.sy[i] = sentinel_value<TF>();
-            s = sweep_sum<LAYOUT, -1>(a, yi, i, e0, stride, width, a.sw);
-            s = s / (TF)a.d[i];
-            publish(a.sw + i, s);
+    if (sweep_rows<LAYOUT, R>(a, i, e0, has, stride, width)) {
+        TF s[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (!has[r]) continue;
+            s[r] = a.sy[i[r]];
+            if constexpr (!TAGGED) a.sy[i[r]] = sentinel_value<TF>();
         }
-        const int o = a.perm ? a.perm[i] : i;
-        const TV z = (TV)s;
-        if ((const void*)a.out != (const void*)a.sw) a.out[o] = z;
-        if (a.part) v = (double)(a.rdot[o] * z);
+        sweep_sums<LAYOUT, TAGGED ? 0 : -1>(a, s, i, e0, has, stride, width, a.sw);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (!has[r]) continue;
+            TF zr;
+            if constexpr (TAGGED) {
+                zr = tagged<0>(s[r] / (TF)a.d[i[r]]);
+                publish_tagged<0>(a.sw + i[r], zr);
+            } else {
+                zr = s[r] / (TF)a.d[i[r]];
+                publish(a.sw + i[r], zr);
+            }
+            const int o = a.perm ? a.perm[i[r]] : i[r];
+            const TV z = (TV)zr;
+            if ((const void*)a.out != (const void*)a.sw) a.out[o] = z;
+            if (a.part) v += (double)(a.rdot[o] * z);
+        }
     }
     if (!a.part) return;
     v = wave_sum(v);
@@ -448,6 +473,53 @@ __global__ __launch_bounds__(kBlock) void k_sweep_upper(SweepArgs<TV, TS, TF> a)
         const double g = collect_partials(a.part, gridDim.x, a.err, a.limit);
         // iteration k = it (it counts the iterations ended so far)
         if (threadIdx.x == 0) cg_record(a.cg, g, a.cg->it);
+    }
+}
+
+// The fused IC0-PCG's backward sweep (tagged, sliced): z = L^-T y published with tag 0, and for each row
+// as soon as its z is known the end-of-iteration vector updates of iteration k = it - 1 (the forward sweep
+// has recorded gamma and beta): x += alpha_k p_k (the x half of updateXR, linear_solver.cu:68-77) and
+// p = z + beta p (updateP, :79-87), with the operations of k_cg_xp_recorded, so no separate vector pass
+// runs. When iteration k stopped the solve only the x update runs, and a launch finding it equal to the
+// xp_seen its SpMV left does nothing.
+template <typename TV, typename TS, typename TF>
+__global__ __launch_bounds__(kBlock) void k_sweep_upper_xp(SweepArgs<TV, TS, TF> a)
+{
+    constexpr int R = kLaneRows;
+    int i[R], e0[R], stride, width;
+    bool has[R];
+    const CgState* st = a.acg;
+    const int it = st->it;
+    if (it == st->xp_seen) return;
+    const int k = it - 1;
+    const bool last = st->done;
+    const TV al = (TV)st->alpha[k & 1];
+    if (!sweep_rows<kSliced, R>(a, i, e0, has, stride, width)) return;
+    if (last) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (has[r]) a.xu[i[r]] += al * a.pu[i[r]];
+        return;
+    }
+    const TV beta = (TV)(st->gam[(k + 1) & 1] / st->gam[k & 1]);
+    TF z[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (has[r]) z[r] = a.sy[i[r]];
+    sweep_sums<kSliced, 0>(a, z, i, e0, has, stride, width, a.sw);
+    // z is published first (the rows of lower levels wait on it); p_k is read after the sweep
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (!has[r]) continue;
+        z[r] = tagged<0>(z[r] / (TF)a.d[i[r]]);
+        publish_tagged<0>(a.sw + i[r], z[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (!has[r]) continue;
+        const TV pi = a.pu[i[r]];
+        a.xu[i[r]] += al * pi;
+        a.pu[i[r]] = (TV)z[r] + beta * pi;
     }
 }
 
@@ -503,6 +575,14 @@ __global__ void k_f64_to_f16(long n, _Float16* __restrict__ d, const double* __r
 __global__ void k_zero_u64(unsigned long long* p)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0ull;
+}
+
+// Diagonal entries of the 16-bit copy that came out zero or subnormal (|c L_ii| < 2^-14): those rows would
+// divide by (nearly) nothing. Counted into *lost (zeroed before).
+__global__ void k_f16_lost_diag(int n, const _Float16* __restrict__ d, int* lost)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && !(fabs((double)d[i]) >= 6.103515625e-05)) atomicAdd(lost, 1);
 }
 
 // ------------------------------------------------------------------ host setup
@@ -776,13 +856,13 @@ static void build_factor_structure(Context* c, pfem_precond_t* pc)
     // RCM order, so band x of one colour and band x of the next cover the same region of the mesh, and a
     // row's dependencies were published on its own XCD (its L2) rather than on any of the eight. The
     // reordering stays inside the level, so every workgroup still waits only on lower-numbered ones.
-    auto chunk_table = [&](const std::vector<int>& ptr, const std::vector<int>& rows) {
+    auto chunk_table = [&](const std::vector<int>& ptr, const std::vector<int>& rows, int rc) {
         constexpr int kXcds = 8;
         std::vector<int> ch;
         for (size_t l = 0; l + 1 < ptr.size(); ++l) {
             std::vector<std::pair<int, int>> level;
-            for (int o = ptr[l]; o < ptr[l + 1]; o += kBlock)
-                level.emplace_back(colour_levels ? rows[o] : o, std::min(kBlock, ptr[l + 1] - o));
+            for (int o = ptr[l]; o < ptr[l + 1]; o += rc)
+                level.emplace_back(colour_levels ? rows[o] : o, std::min(rc, ptr[l + 1] - o));
             const int m = (int)level.size();
             const int base = (int)ch.size() / 2;   // workgroup index of the level's first chunk
             std::vector<int> next(kXcds), end(kXcds);
@@ -808,7 +888,7 @@ static void build_factor_structure(Context* c, pfem_precond_t* pc)
         }
         return ch;
     };
-    const std::vector<int> lch = chunk_table(lptr, lrows), uch = chunk_table(uptr, urows);
+    const std::vector<int> lch = chunk_table(lptr, lrows, kBlock), uch = chunk_table(uptr, urows, kBlock);
     pc->contiguous = colour_levels;
     pc->nlchunks = (int)lch.size() / 2;
     pc->nuchunks = (int)uch.size() / 2;
@@ -903,8 +983,12 @@ static void build_factor_structure(Context* c, pfem_precond_t* pc)
             upload(c, out.pos, pos);
             val_d.alloc(col.size());
         };
-        slice(lch, 0, pc->sl_lower, pc->sell_l);
-        slice(uch, 1, pc->sl_upper, pc->sell_u);
+        // the sliced sweeps take kSliceRows rows per chunk (kLaneRows per lane)
+        const std::vector<int> lch2 = chunk_table(lptr, lrows, kSliceRows), uch2 = chunk_table(uptr, urows, kSliceRows);
+        pc->nlslices = (int)lch2.size() / 2;
+        pc->nuslices = (int)uch2.size() / 2;
+        slice(lch2, 0, pc->sl_lower, pc->sell_l);
+        slice(uch2, 1, pc->sl_upper, pc->sell_u);
         pc->sell = true;
         // The whole matrix over the SpMV tiles, dealt by region across colours: XCD x (workgroups b = x mod
         // 8) takes the tiles of the x-th eighth of every colour's range. A row's neighbours lie in the other
@@ -949,8 +1033,8 @@ static void build_factor_structure(Context* c, pfem_precond_t* pc)
     pc->sw.alloc(n);
     hipLaunchKernelGGL((k_fill_sentinel<double>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, (long)n, pc->sy.p);
     hipLaunchKernelGGL((k_fill_sentinel<double>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, (long)n, pc->sw.p);
-    pc->status.alloc(1);
-    zero_async(c, pc->status.p, sizeof(int));
+    pc->status.alloc(2);   // first failing row (+1); diagonals lost to the 16-bit copy
+    zero_async(c, pc->status.p, 2 * sizeof(int));
     pc->sweep_err.alloc(1);
     zero_async(c, pc->sweep_err.p, sizeof(int));
 }
@@ -1032,6 +1116,15 @@ void precond_initialize(Context* c, pfem_precond_t* pc, pfem_csr_t* a)
                            pc->sell_l.p, pc->scale16.p);
         hipLaunchKernelGGL(k_f64_to_f16, dim3(blocks_for(nu)), dim3(kBlock), 0, c->stream, nu, pc->sell_u16.p,
                            pc->sell_u.p, pc->scale16.p);
+        // a diagonal lost to the 16-bit range (the scale follows the largest entry, so this happens when
+        // the factor's entries span more than about 2^27): this factor is applied from its 32-bit copy
+        int* lost = pc->status.p + 1;
+        zero_async(c, lost, sizeof(int));
+        hipLaunchKernelGGL(k_f16_lost_diag, dim3(blocks_for(F.n)), dim3(kBlock), 0, c->stream, F.n,
+                           (const _Float16*)(pc->split16.p + pc->split_diag), lost);
+        int h = 0;
+        d2h_sync(c, &h, lost, sizeof(int));
+        pc->half_ok = h == 0;
     }
     PFEM_HIP(hipGetLastError());
 }
@@ -1100,13 +1193,16 @@ static void sweeps(Context* c, pfem_precond_t* pc, const TV* src, TV* out, const
     const bool xr = o.alpha != nullptr;
     PFEM_REQUIRE(!xr || (o.factor_order && pc->contiguous && o.q && (o.p || !o.x)),
                  "the fused x/r update needs a multicolour factor and vectors in its ordering");
-    const dim3 gl(pc->nlchunks), gu(pc->nuchunks), bl(kBlock);
+    dim3 gl(pc->nlchunks), gu(pc->nuchunks);
+    const dim3 bl(kBlock);
     SweepArgs<TV, TS, TF> lo = a, up = a;
     lo.chunks = pc->lchunks.p;
     up.chunks = pc->uchunks.p;
     lo.ptr = pc->split_ptr.p;
     up.ptr = pc->split_ptr.p + (F.n + 1);
     if (pc->contiguous && pc->sell) {
+        gl = dim3(pc->nlslices);
+        gu = dim3(pc->nuslices);
         lo.chunks = pc->sl_lower.hdr.p;
         up.chunks = pc->sl_upper.hdr.p;
         lo.col = pc->sl_lower.col.p;
@@ -1153,7 +1249,8 @@ static void sweeps(Context* c, pfem_precond_t* pc, const TV* src, TV* out, const
                     if (ic) hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, false, kSliced, false, true>), gl, bl, 0, c->stream, lo);
                     else hipLaunchKernelGGL((k_sweep_lower<TV, TS, TF, true, kSliced, false, true>), gl, bl, 0, c->stream, lo);
                 }
-                hipLaunchKernelGGL((k_sweep_upper<TV, TS, TF, kSliced, true>), gu, bl, 0, c->stream, up);
+                if (up.xu) hipLaunchKernelGGL((k_sweep_upper_xp<TV, TS, TF>), gu, bl, 0, c->stream, up);
+                else hipLaunchKernelGGL((k_sweep_upper<TV, TS, TF, kSliced, true>), gu, bl, 0, c->stream, up);
                 return;
             }
         }
@@ -1214,7 +1311,7 @@ int precond_apply_t(Context* c, pfem_precond_t* pc, TV* z, const TV* r, bool fp3
             if (fp32) {
                 // factor stored in FP32, solves in FP64: an exact apply of the rounded factor
                 PFEM_REQUIRE(pc->fac32.p && pc->sweep, "FP32 factor not enabled (or per-level apply selected)");
-                if (pc->factor_bits == 16) {
+                if (pc->factor_bits == 16 && pc->half_ok) {
                     PFEM_REQUIRE(pc->split16.p && pc->sell && pc->contiguous,
                                  "a 16-bit factor needs a multicolour factor with the sliced sweep layout");
                     sweeps<double, _Float16, double>(c, pc, r, z, pc->split16.p, pc->sy.p, pc->sw.p, part, opt);
@@ -1233,7 +1330,9 @@ int precond_apply_t(Context* c, pfem_precond_t* pc, TV* z, const TV* r, bool fp3
         }
     }
     PFEM_HIP(hipGetLastError());
-    return part && (pc->kind == PFEM_PRECOND_IC0 || pc->kind == PFEM_PRECOND_ILU0) ? pc->nuchunks : 0;
+    // partials written: one per backward launch workgroup
+    const int nbw = pc->contiguous && pc->sell ? pc->nuslices : pc->nuchunks;
+    return part && (pc->kind == PFEM_PRECOND_IC0 || pc->kind == PFEM_PRECOND_ILU0) ? nbw : 0;
 }
 
 template int precond_apply_t<double>(Context*, pfem_precond_t*, double*, const double*, bool, const ApplyOpts<double>&);

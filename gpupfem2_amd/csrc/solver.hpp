@@ -196,6 +196,7 @@ struct pfem_precond_t {
     int spin_limit = 1 << 16;             // polls of one single-pass wait before it is declared failed
     pfem::DevArray<int> lchunks, uchunks;
     int nlchunks = 0, nuchunks = 0;
+    int nlslices = 0, nuslices = 0;       // chunks of the sliced layouts (up to 512 rows each)
     bool contiguous = false;              // levels are row ranges (multicolour): chunks hold row ranges
     pfem::DevArray<double> sy, sw;
     pfem::DevArray<float> sy32, sw32;
@@ -217,6 +218,7 @@ struct pfem_precond_t {
     // the reduced-precision copy used with FP64 vectors (PFEM_PRECISION_MIXED_PRECOND): 32 or 16 bits per
     // value (16: the split and sliced factor only, for the single-pass sweeps)
     int factor_bits = 32;
+    bool half_ok = true;                  // the last 16-bit copy kept every diagonal (else the 32-bit copy is used)
     pfem::DevArray<_Float16> split16, sell_l16, sell_u16;   // c L, c = scale16[0] (a power of two)
     pfem::DevArray<double> scale16;       // c, c^2, and the largest |L| found (as bits)
     bool sell = false;
@@ -251,6 +253,7 @@ struct pfem_solver_t {
     hipEvent_t ev[2] = {nullptr, nullptr};   // pipelined polls of the device state
     std::unique_ptr<pfem::MultiWork, pfem::MultiWorkDeleter> multi;   // pfem_solver_solve_multi
     std::unique_ptr<pfem::BatchWork, pfem::BatchWorkDeleter> batch;   // pfem_solver_solve_batched
+    std::vector<int> batch_tiling;   // {row bound, tile rows, tile non-zeros} per size bucket (empty: defaults)
     pfem::DevArray<pfem::CgState> cg;
     pfem::DevArray<pfem::GmState> gm;
     pfem::CgState* h_cg = nullptr;   // pinned, two slots

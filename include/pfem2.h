@@ -237,6 +237,15 @@ int pfem_solver_destroy(pfem_solver s);
 /* LinearSolver::solve(A, x, b) -> bool (linear_solver.cu:146-154, 301-376, 421-521): warm start from x,
  * the preconditioner is re-initialised from A first. stats may be NULL. */
 int pfem_solver_solve(pfem_solver s, pfem_csr a, double* d_x, const double* d_b, pfem_solve_stats* stats);
+/* Two systems on one pattern solved together (the two velocity-prediction systems, which the reference
+ * solves one after the other with one solver, cases/Cylinder2D/main.cu:820-824): s0 solves a0 x0 = b0 and
+ * s1 solves a1 x1 = b1, each with its own Krylov state, scalars and stopping test, so each x is that of a
+ * single pfem_solver_solve. For GMRES with a Jacobi or no preconditioner, in FP64 or PFEM_PRECISION_MIXED,
+ * and matrices sharing one pattern, the Arnoldi steps run in lockstep with one SpMV launch for both
+ * (the pattern streamed once) and one host poll for both; other combinations solve one after the other.
+ * stats may be NULL (else two entries). */
+int pfem_solver_solve_pair(pfem_solver s0, pfem_csr a0, double* d_x0, const double* d_b0, pfem_solver s1,
+                           pfem_csr a1, double* d_x1, const double* d_b1, pfem_solve_stats stats[2]);
 /* Multi-right-hand-side PCG (no counterpart in the reference, whose SolverCG::solve, linear_solver.cu:301-376,
  * takes one right-hand side per call): A X = B for nrhs (1..8) columns sharing the matrix, column j of X and
  * B at d_X + j*n and d_B + j*n (X is also the initial guess). Each column runs the solver's PCG recurrence
@@ -247,6 +256,12 @@ int pfem_solver_solve(pfem_solver s, pfem_csr a, double* d_x, const double* d_b,
  * counts the replacements). stats: nrhs entries (or NULL). */
 int pfem_solver_solve_multi(pfem_solver s, pfem_csr a, int nrhs, double* d_X, const double* d_B,
                             pfem_solve_stats* stats);
+/* Tile limits of pfem_solver_solve_batched per system size (no counterpart in the reference): a system of
+ * n rows takes the first bucket k with n <= max_rows[k] and is cut into SpMV tiles of at most tile_rows[k]
+ * rows (<= 256) and tile_nnz[k] non-zeros (<= 2048); larger systems, and every system when nbuckets = 0
+ * (the default), take 256 rows and 2048 non-zeros. Set by the offline sweep of gpupfem2_amd/tune.py. */
+int pfem_solver_set_batch_tiles(pfem_solver s, int nbuckets, const int* max_rows, const int* tile_rows,
+                                const int* tile_nnz);
 /* Batched independent PCG solves (no counterpart in the reference, which solves one system per
  * SolverCG::solve call, linear_solver.cu:301-376): a is block diagonal, system k owning rows and columns
  * [h_sys_rows[k], h_sys_rows[k+1]) (h_sys_rows: nsys + 1 host ints from 0 to n; checked). Each system runs
@@ -367,6 +382,9 @@ typedef struct {
     int pres_ordering;         /* PFEM_ORDER_* for an IC0/ILU0 pressure preconditioner */
     int pres_factor_bits;      /* MIXED with a multicolour IC0 pressure factor: bits of its reduced copy, 16 or
                                   32 (0 means 32; pfem_precond_set_factor_bits) */
+    int pred_gs_passes;        /* GMRES prediction solves: classical Gram-Schmidt passes per Arnoldi step, 1 or
+                                  2 (PFEM_SOLVER_OPT_GS_PASSES; 0 means 1, enough for these mass-dominated
+                                  systems; 2 orthogonalises like the reference's modified Gram-Schmidt) */
 } pfem_step_config;
 /* The time loop body of cases/Cylinder2D/main.cu:793-880 on a mesh with edge IDs set. Velocity BCs
  * share one node list with a value per component; pressure BCs have their own list. */

@@ -124,3 +124,40 @@ def test_batched_256_meshes(ctx):
         assert res < 1.05e-10, (k, res)
         assert stats[k].final_residual <= 1e-10 * math.sqrt(np.sum(b[lo:hi] ** 2 / d[lo:hi])) * 1.0001, k
     assert np.all(tms > 0.0)
+
+
+def test_batched_tiles_per_bucket(ctx, oracle):
+    """Tile limits chosen per system-size bucket (pfem_solver_set_batch_tiles): the systems are cut into
+    smaller SpMV tiles, which regroups only the tile partials of the dot products, so every system still
+    matches the oracle's PCG and the default tiling to 1e-10 (both stop at relative 1e-12); bad buckets are
+    rejected."""
+    from gpupfem2_amd import capi, cases
+    print("batched tiles per bucket", flush=True)
+    case = small_batch(nmesh=8, seed=6)
+    mesh, a, b = cases.build_batched_poisson(ctx, case)
+    rp, col, val = a.download()
+    rows = case["sys_rows"]
+    n = rows[-1]
+    sizes = np.diff(rows)
+    mid = int(np.median(sizes))
+    out = {}
+    for name, buckets in (("default", []), ("bucketed", [(mid, 64, 1024), (1 << 30, 100, 512)])):
+        s = capi.Solver(a, capi.SOLVER_CG, 1e-12, 6000, 0, capi.TOL_RELATIVE, capi.Preconditioner(a, capi.PRECOND_JACOBI))
+        s.set_batch_tiles(buckets)
+        x = ctx.zeros(n)
+        stats, _ = s.solve_batched(a, rows, x, b)
+        assert all(st.converged for st in stats), name
+        out[name] = (x.download(), [st.iterations for st in stats])
+    xh, its = out["bucketed"]
+    for k in range(rows.size - 1):
+        lo, hi = rows[k], rows[k + 1]
+        xr = np.zeros(hi - lo)
+        ost = oracle.pcg(rp[lo:hi + 1] - rp[lo], col[rp[lo]:rp[hi]] - lo, val[rp[lo]:rp[hi]], xr,
+                         case["rhs"][lo:hi], 1e-12, 6000, 1, "jacobi")
+        assert ost.converged and abs(its[k] - ost.iterations) <= 3, (k, its[k], ost.iterations)
+        assert rel_l2(xh[lo:hi], xr) < 1e-10, k
+        assert rel_l2(xh[lo:hi], out["default"][0][lo:hi]) < 1e-10, k
+    s = capi.Solver(a, capi.SOLVER_CG, 1e-10, 100, 0, 0, None)
+    for bad in ([(100, 64, 1024), (50, 64, 1024)], [(100, 300, 1024)], [(100, 64, 4096)], [(100, 0, 1024)]):
+        with pytest.raises(capi.PfemError):
+            s.set_batch_tiles(bad)

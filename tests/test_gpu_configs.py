@@ -147,15 +147,17 @@ def test_cylinder_mixed_vs_fp64_full_size(ctx):
     assert np.all(rel_f <= 1e-6), rel_f
 
 
-def test_cylinder_h28_fp64_vs_oracle(ctx, oracle):
-    """The FP64 GPU pipeline in its benchmark form (multicolour IC0 pressure factor) against the oracle's
-    whole step (natural-order IC0) on the cylinder channel at h = 1/28 (126k triangles, 2.5M particles),
+@pytest.mark.parametrize("passes", [1, 2])
+def test_cylinder_h28_fp64_vs_oracle(ctx, oracle, passes):
+    """The FP64 GPU pipeline in its benchmark form (multicolour IC0 pressure factor; prediction GMRES with
+    one classical Gram-Schmidt pass, the default, or two) against the oracle's whole step (natural-order
+    IC0, modified Gram-Schmidt GMRES) on the cylinder channel at h = 1/28 (126k triangles, 2.5M particles),
     two steps: fields within relative L2 1e-8 (both pressure solves stop at 1e-12 relative), C_D and C_L
     within 1e-6 relative, and the same particle -> cell indices."""
     from gpupfem2_amd import cases
     case = cases.cylinder_case(h=1.0 / 28)
-    print("cylinder h=1/28 vs oracle", case.name, case.tri.shape[0], flush=True)
-    mesh, ph, stepper = cases.build_gpu(ctx, case, pres_ordering=1)
+    print("cylinder h=1/28 vs oracle", case.name, case.tri.shape[0], "passes", passes, flush=True)
+    mesh, ph, stepper = cases.build_gpu(ctx, case, pres_ordering=1, pred_gs_passes=passes)
     oph, ost = oracle_stepper(oracle, case, gpu_mesh=mesh)
     for s in range(2):
         fg = stepper.step()
@@ -165,6 +167,9 @@ def test_cylinder_h28_fp64_vs_oracle(ctx, oracle):
             assert rel_l2(g[k], o[k]) < 1e-8, (s, k, rel_l2(g[k], o[k]))
         assert np.array_equal(ph.get()["cell"], oph.get()["cell"]), s
         assert np.all(np.abs(fg - fo) <= 1e-6 * np.abs(fo)), (fg, fo)
+        print("passes", passes, "step", s, "rel L2 u0 %.2e p %.2e, C_D %.2e C_L %.2e" % (
+            rel_l2(g["u0"], o["u0"]), rel_l2(g["p"], o["p"]), abs(fg[0] - fo[0]) / abs(fo[0]),
+            abs(fg[1] - fo[1]) / abs(fo[1])), flush=True)
 
 
 def test_cylinder_step_sweep_fallback(ctx):

@@ -387,6 +387,51 @@ def test_gmres(ctx, oracle, restart, passes):
     assert rel_l2(x.download(), xr) < 1e-10
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_gmres_pair(ctx, oracle, prec):
+    """Two nonsymmetric systems on one pattern solved together (pfem_solver_solve_pair: Arnoldi steps in
+    lockstep, one SpMV launch for both) give the bits and iteration counts of two single solves, in FP64
+    and in mixed precision (FP32 inner GMRES with FP64 refinement); the FP64 pair agrees with the oracle's
+    modified Gram-Schmidt GMRES. A second call reuses the solvers (restarted state)."""
+    from gpupfem2_amd import capi
+    xy, m, a, rhs, rp, col, val = bc_system(ctx, oracle, n=24)
+    n = rp.size - 1
+    rng = np.random.default_rng(8)
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    vals = [val + np.where(col == rows, 1.0, w * rng.uniform(-1, 1, col.size) * np.abs(val)) for w in (0.3, 0.1)]
+    mats = [capi.CSR(ctx, like=a) for _ in vals]
+    for mt, v in zip(mats, vals):
+        mt.upload_values(v)
+    bs = [rng.standard_normal(n), rng.standard_normal(n)]
+
+    def solver(mt):
+        s = capi.Solver(mt, capi.SOLVER_GMRES, 1e-12, 3000, 30, capi.TOL_RELATIVE, capi.Preconditioner(mt, capi.PRECOND_JACOBI))
+        if prec:
+            s.set_precision(capi.PRECISION_MIXED, 1e-5, 20)
+        return s
+
+    single = []
+    for k in range(2):
+        s = solver(mats[k])
+        x = ctx.zeros(n)
+        st = s.solve(mats[k], x, ctx.to_device(bs[k]))
+        assert st.converged
+        single.append((st.iterations, x.download()))
+    s0, s1 = solver(mats[0]), solver(mats[1])
+    for rep in range(2):
+        x0, x1 = ctx.zeros(n), ctx.zeros(n)
+        st = s0.solve_pair(mats[0], x0, ctx.to_device(bs[0]), s1, mats[1], x1, ctx.to_device(bs[1]))
+        for k, (x, t) in enumerate(((x0, st[0]), (x1, st[1]))):
+            assert t.converged and t.iterations == single[k][0], (rep, k, t.iterations, single[k][0])
+            assert np.array_equal(x.download(), single[k][1]), (rep, k)
+    if prec == 0:
+        for k in range(2):
+            xr = np.zeros(n)
+            ost = oracle.gmres(rp, col, vals[k], xr, bs[k], 1e-12, 3000, 30, 1, "jacobi")
+            assert ost.converged and abs(ost.iterations - single[k][0]) <= 3
+            assert rel_l2(single[k][1], xr) < 1e-10
+
+
 @pytest.mark.parametrize("kind,precond", [(0, 1), (0, 2), (1, 1)])
 def test_mixed_precision_refinement(ctx, oracle, kind, precond):
     """FP32 inner Krylov + FP64 refinement reaches the FP64 tolerance."""
@@ -618,6 +663,38 @@ def test_half_factor(ctx, oracle):
         pc.set_factor_bits(16)
     with pytest.raises(capi.PfemError):
         capi.Preconditioner(a, capi.PRECOND_IC0, capi.ORDER_MULTICOLOR).set_factor_bits(8)
+
+
+def test_half_factor_wide_range_falls_back(ctx, oracle):
+    """A factor whose entries span more than the 16-bit copy's range (the system D A D with D from 1 to
+    1e-10 along x: IC0(D A D) = D IC0(A), so the factor's diagonal runs from about 1 to 1e-10, past the
+    2^28 that half precision holds between its largest entry and its smallest normal) would lose diagonals
+    to zero or subnormal halves and divide by nothing; the preconditioner then applies its 32-bit copy
+    instead: the 16-bit request takes the 32-bit iterations exactly and reaches the direct solution."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    from gpupfem2_amd import capi
+    xy, m, a, rhs, rp, col, val = bc_system(ctx, oracle)
+    n = xy.shape[0]
+    d = 10.0 ** (-10.0 * xy[:, 0])
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    ms = capi.CSR(ctx, mesh=m)
+    vs = val * d[rows] * d[col]
+    ms.upload_values(np.ascontiguousarray(vs))
+    b = rhs.download() * d
+    exact = spla.spsolve(sp.csr_matrix((vs, col, rp)).tocsc(), b)
+    its = {}
+    for bits in (32, 16):
+        pc = capi.Preconditioner(ms, capi.PRECOND_IC0, capi.ORDER_MULTICOLOR)
+        pc.set_factor_bits(bits)
+        s = capi.Solver(ms, capi.SOLVER_CG, 1e-12, 5000, 0, capi.TOL_RELATIVE, pc)
+        s.set_precision(capi.PRECISION_MIXED_PRECOND, 1e-5, 20)
+        x = ctx.zeros(n)
+        st = s.solve(ms, x, ctx.to_device(b))
+        assert st.converged and not st.nan_detected, bits
+        assert rel_l2(x.download(), exact) < 1e-8, bits
+        its[bits] = st.iterations
+    assert its[16] == its[32], its
 
 
 def neighbors_by_incidence(nv, tri, self_included):
