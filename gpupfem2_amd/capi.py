@@ -146,6 +146,7 @@ SIGNATURES = {
     "pfem_pfem2_field": (_i, [_vp, _i, _pp]),
     "pfem_pfem2_last_iterations": (_i, [_vp, _vp, _i]),
     "pfem_pfem2_enable_timing": (_i, [_vp, _i]),
+    "pfem_pfem2_set_graphs": (_i, [_vp, _i]),
     "pfem_pfem2_phase_times": (_i, [_vp, _vp, _i]),
     "pfem_pfem2_matrix": (_i, [_vp, _i, _pp]),
     "pfem_pfem2_precond": (_i, [_vp, _i, _pp]),
@@ -689,6 +690,10 @@ class Pfem2(_Handle):
 
     def enable_timing(self, on=True):
         lib().pfem_pfem2_enable_timing(self.h, int(on))
+
+    def set_graphs(self, on=True):
+        """replay the work between the solves from HIP graphs from the next step on (pfem_pfem2_set_graphs)"""
+        self.ctx.check(lib().pfem_pfem2_set_graphs(self.h, int(on)))
 
     def phase_times(self):
         ms = np.zeros(8, np.float64)

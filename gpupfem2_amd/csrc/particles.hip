@@ -12,6 +12,15 @@
 // is what makes the projection a deterministic segmented reduction: per cell over its contiguous
 // particle range, then per node over its incident cells -- no atomics (the reference uses 9 FP64
 // atomicAdd per particle, :90-107).
+//
+// The particle count lives on the device ({count, next ID, deleted, added} in pst): every kernel of a step
+// reads it there and is launched over the buffers' capacity, which the host keeps at least the count plus
+// the largest possible refill (one particle per subcell of every cell), so a step never waits for the
+// device to report how many particles it added or deleted. The host learns the new count from an
+// asynchronous copy read at its next use (the next step's capacity check, pfem_particles_count); by then
+// the stream has long passed it. With fixed capacity and no host-side count, a step's particle kernels
+// are the same launch sequence every step (two sequences: the buffers alternate), which the stepper can
+// replay from a HIP graph.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -38,6 +47,18 @@ struct pfem_particles_t {
     pfem::DevArray<int> hist, scan_tmp;
     pfem::DevArray<double> part;
     long deleted_last = 0, added_last = 0;
+    // device state {count, next ID, deleted, added} of the last step, its pinned host copy and the event
+    // that completes the copy; generation counts reallocations of the particle buffers
+    pfem::DevArray<long> pst;
+    long* h_pst = nullptr;
+    hipEvent_t pst_ev = nullptr;
+    bool pst_pending = false;
+    long generation = 0;
+    ~pfem_particles_t()
+    {
+        if (pst_ev) (void)hipEventDestroy(pst_ev);
+        if (h_pst) (void)hipHostFree(h_pst);
+    }
 };
 
 namespace pfem {
@@ -114,13 +135,13 @@ __global__ void k_seed(int nt, int ppc, int mode, const double* __restrict__ xy,
 }
 
 // kCorrectParticleVelocity (particle_handler_2d.cu:72-88); uo == nullptr is initParticleVelocity
-__global__ void k_correct(long P, const int* __restrict__ cell, const double2* __restrict__ lam12,
+__global__ void k_correct(const long* __restrict__ pst, const int* __restrict__ cell, const double2* __restrict__ lam12,
                           const double* __restrict__ lam3, double2* __restrict__ vel, const int* __restrict__ tri,
                           const double* __restrict__ u0, const double* __restrict__ u1,
                           const double* __restrict__ uo0, const double* __restrict__ uo1)
 {
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
+    if (p >= pst[0]) return;
     const int c = cell[p];
     const double2 l12 = lam12[p];
     const double l[3] = {l12.x, l12.y, lam3[p]};
@@ -154,13 +175,13 @@ __device__ __forceinline__ bool inside(const double4 J, double v3x, double v3y, 
 // relocation of sortParticlesInCells (:363-395): own cell, then the vertex neighbours ascending, first
 // hit wins; a particle found nowhere is marked dead (cell = -1).
 __global__ __launch_bounds__(kBlock) void k_advect_relocate(
-    long P, int substeps, double dts, int* __restrict__ cell, double2* __restrict__ lam12, double* __restrict__ lam3,
-    double2* __restrict__ pos, const double* __restrict__ xy, const int* __restrict__ tri,
+    const long* __restrict__ pst, int substeps, double dts, int* __restrict__ cell, double2* __restrict__ lam12,
+    double* __restrict__ lam3, double2* __restrict__ pos, const double* __restrict__ xy, const int* __restrict__ tri,
     const double4* __restrict__ invJ, const int* __restrict__ nbr_off, const int* __restrict__ nbr_idx,
     const double* __restrict__ u0, const double* __restrict__ u1, int* __restrict__ dead)
 {
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
+    if (p >= pst[0]) return;
     int c = cell[p];
     if (c < 0) return;
     const double2 l12 = lam12[p];
@@ -204,7 +225,7 @@ __global__ __launch_bounds__(kBlock) void k_advect_relocate(
 
 // kCountParticlesInSubcells (particle_handler_2d.cu:173-181): only emptiness matters, so one bit per
 // subcell (atomicOr into a 16-bit mask per cell) replaces the reference's int counter per subcell
-__global__ void k_census(long P, const int* __restrict__ cell, const double2* __restrict__ lam12,
+__global__ void k_census(const long* __restrict__ pst, const int* __restrict__ cell, const double2* __restrict__ lam12,
                          const double* __restrict__ lam3, int L, double step, uint32_t* __restrict__ mask)
 {
     // Particles arrive grouped by cell (the array was sorted by cell at the end of the previous step and
@@ -212,6 +233,7 @@ __global__ void k_census(long P, const int* __restrict__ cell, const double2* __
     // bits are OR-ed across lanes with the same cell first (a suffix scan keyed on the cell), and only the
     // first lane of each run of equal cells issues the atomic: one atomic per run instead of per particle.
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long P = pst[0];
     const int lane = threadIdx.x & 63;
     int c = -1;
     uint32_t bits = 0u;
@@ -242,8 +264,9 @@ __global__ void k_refill_count(int nt, int ns, const uint32_t* __restrict__ mask
 }
 
 // kAddParticlesToCell (:197-236): new particles at the centroids of the empty subcells, velocity
-// interpolated from the grid; appended in (cell, subcell) order with consecutive IDs
-__global__ void k_refill_write(int nt, int ns, long base, uint32_t id0, const uint32_t* __restrict__ mask,
+// interpolated from the grid; appended after the current count in (cell, subcell) order with consecutive
+// IDs from the next free one
+__global__ void k_refill_write(int nt, int ns, const long* __restrict__ pst, const uint32_t* __restrict__ mask,
                                const int* __restrict__ add, const int* __restrict__ add_off,
                                const double* __restrict__ xy, const int* __restrict__ tri, Centers cen,
                                const double* __restrict__ u0, const double* __restrict__ u1, int* __restrict__ cell,
@@ -252,6 +275,8 @@ __global__ void k_refill_write(int nt, int ns, long base, uint32_t id0, const ui
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nt || add[c] == 0) return;
+    const long base = pst[0];
+    const uint32_t id0 = (uint32_t)pst[1];
     const uint32_t m = mask[c];
     const int a = tri[3 * c], b = tri[3 * c + 1], d = tri[3 * c + 2];
     int k = 0;
@@ -375,9 +400,17 @@ void exclusive_scan_int(Context* c, const int* in, int* out, long n, int* total_
 constexpr int kRadixItems = 16;
 constexpr int kRadixTile = kBlock * kRadixItems;
 
-__global__ __launch_bounds__(kBlock) void k_radix_hist(long n, const uint32_t* __restrict__ keys, int shift,
+// Particles to sort after a refill: the count before it plus the particles it appended
+__device__ __forceinline__ long sort_count(const long* __restrict__ pst, const int* __restrict__ counters)
+{
+    return pst[0] + counters[1];
+}
+
+__global__ __launch_bounds__(kBlock) void k_radix_hist(const long* __restrict__ pst, const int* __restrict__ counters,
+                                                       const uint32_t* __restrict__ keys, int shift,
                                                        int* __restrict__ hist, int nblocks)
 {
+    const long n = sort_count(pst, counters);
     __shared__ int h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
@@ -392,11 +425,13 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(long n, const uint32_t* _
 
 // Scatter with stable ranks: the tile is processed in 256-key rounds in index order; inside a round
 // each wave finds the lanes sharing its digit with 8 ballots, and waves are ordered through LDS counts.
-__global__ __launch_bounds__(kBlock) void k_radix_scatter(long n, const uint32_t* __restrict__ kin,
+__global__ __launch_bounds__(kBlock) void k_radix_scatter(const long* __restrict__ pst, const int* __restrict__ counters,
+                                                          const uint32_t* __restrict__ kin,
                                                           const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
                                                           uint32_t* __restrict__ vout, int shift,
                                                           const int* __restrict__ offs, int nblocks)
 {
+    const long n = sort_count(pst, counters);
     __shared__ int base[256];
     __shared__ int wcnt[kWaves][256];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -441,16 +476,18 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(long n, const uint32_t
     }
 }
 
-__global__ void k_make_keys(long n, const int* __restrict__ cell, int nt, uint32_t* __restrict__ keys)
+__global__ void k_make_keys(const long* __restrict__ pst, const int* __restrict__ counters, const int* __restrict__ cell,
+                            int nt, uint32_t* __restrict__ keys)
 {
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n) {
+    if (p < sort_count(pst, counters)) {
         const int c = cell[p];
         keys[p] = c < 0 ? (uint32_t)nt : (uint32_t)c;
     }
 }
 
-__global__ void k_gather(long n, const uint32_t* __restrict__ perm, const int* __restrict__ cell0,
+__global__ void k_gather(const long* __restrict__ pst, const int* __restrict__ counters,
+                         const uint32_t* __restrict__ perm, const int* __restrict__ cell0,
                          const double2* __restrict__ l120, const double* __restrict__ l30,
                          const double2* __restrict__ pos0, const double2* __restrict__ vel0,
                          const uint32_t* __restrict__ id0, int* __restrict__ cell1, double2* __restrict__ l121,
@@ -458,7 +495,7 @@ __global__ void k_gather(long n, const uint32_t* __restrict__ perm, const int* _
                          uint32_t* __restrict__ id1)
 {
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+    if (p >= sort_count(pst, counters)) return;
     const uint32_t s = perm[p];
     cell1[p] = cell0[s];
     l121[p] = l120[s];
@@ -468,10 +505,29 @@ __global__ void k_gather(long n, const uint32_t* __restrict__ perm, const int* _
     id1[p] = id0[s];
 }
 
+// The step's new device state: count = count + added - deleted (the deleted particles sorted last), the
+// next free ID, and the step's deleted and added counts
+__global__ void k_pstate_finish(long* __restrict__ pst, const int* __restrict__ counters)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const long dead = counters[0], added = counters[1];
+    pst[0] = pst[0] + added - dead;
+    pst[1] = pst[1] + added;
+    pst[2] = dead;
+    pst[3] = added;
+}
+
+__global__ void k_zero_u32(long n, uint32_t* __restrict__ d)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = 0u;
+}
+
 // first particle of every cell in the cell-sorted array (cell_start[nt] = number of particles)
-__global__ void k_cell_start(long n, const int* __restrict__ cell, int nt, int* __restrict__ start)
+__global__ void k_cell_start(const long* __restrict__ pst, const int* __restrict__ cell, int nt, int* __restrict__ start)
 {
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = pst[0];
     if (p > n) return;
     const int prev = p == 0 ? -1 : cell[p - 1];
     const int curc = p == n ? nt : cell[p];
@@ -552,8 +608,35 @@ static void alloc_state(pfem_particles_t* ph, long cap)
     ph->hist.alloc(256 * (size_t)nblk);
     ph->scan_tmp.alloc(scan_tmp_size(std::max<long>(256 * nblk, ph->mesh->nt + 1)) + 8);
     ph->capacity = cap;
+    ++ph->generation;
 }
 
+// The host copy of the device state {count, next ID, deleted, added}, once its pending copy is complete
+// (normally long before: the stream has passed it by the next call that needs it).
+static void sync_host_state(pfem_particles_t* ph)
+{
+    if (!ph->pst_pending) return;
+    PFEM_HIP(hipEventSynchronize(ph->pst_ev));
+    ph->count = ph->h_pst[0];
+    ph->next_id = (uint32_t)ph->h_pst[1];
+    ph->deleted_last = ph->h_pst[2];
+    ph->added_last = ph->h_pst[3];
+    ph->pst_pending = false;
+}
+
+// The device state from the host's (after a seed or an upload)
+static void put_device_state(Context* c, pfem_particles_t* ph)
+{
+    ph->h_pst[0] = ph->count;
+    ph->h_pst[1] = ph->next_id;
+    ph->h_pst[2] = ph->deleted_last;
+    ph->h_pst[3] = ph->added_last;
+    h2d(c, ph->pst.p, ph->h_pst, 4 * sizeof(long));
+    ph->pst_pending = false;
+}
+
+// capacity for `need` particles; the first ph->count particles of the current buffers are kept (the host
+// state must be current)
 static void grow(Context* c, pfem_particles_t* ph, long need)
 {
     if (need <= ph->capacity) return;
@@ -581,41 +664,43 @@ static void grow(Context* c, pfem_particles_t* ph, long need)
     ph->id[b] = std::move(id);
 }
 
-static void compute_cell_start(Context* c, pfem_particles_t* ph)
+// cell_start of buffer b for the device count (grid over the capacity)
+static void compute_cell_start(Context* c, pfem_particles_t* ph, int b)
 {
-    const long n = ph->count;
-    hipLaunchKernelGGL(k_cell_start, dim3(blocks_for(n + 1)), dim3(kBlock), 0, c->stream, n, ph->cell[ph->cur].p,
-                       ph->mesh->nt, ph->cell_start.p);
+    hipLaunchKernelGGL(k_cell_start, dim3(blocks_for(ph->capacity + 1)), dim3(kBlock), 0, c->stream,
+                       (const long*)ph->pst.p, ph->cell[b].p, ph->mesh->nt, ph->cell_start.p);
 }
 
-// stable reorder of the first n particles by cell (dead ones, cell = -1, last); returns the live count
-static void sort_by_cell(Context* c, pfem_particles_t* ph, long n)
+// stable reorder of buffer b's particles (count + added, from the device) by cell into buffer 1 - b, the
+// deleted ones (cell = -1) last
+static void sort_by_cell(Context* c, pfem_particles_t* ph, int b)
 {
     const int nt = ph->mesh->nt;
-    const int b = ph->cur, o = 1 - b;
-    if (n == 0) return;
+    const int o = 1 - b;
+    const long cap = ph->capacity;
+    const long* pst = ph->pst.p;
+    const int* cnt = ph->counters.p;
     int bits = 1;
     while ((1u << bits) <= (uint32_t)nt) ++bits;   // keys are in [0, nt]
     const int passes = (bits + 7) / 8;
-    hipLaunchKernelGGL(k_make_keys, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, ph->cell[b].p, nt,
+    hipLaunchKernelGGL(k_make_keys, dim3(blocks_for(cap)), dim3(kBlock), 0, c->stream, pst, cnt, ph->cell[b].p, nt,
                        ph->keys[0].p);
-    const int nblk = (int)((n + kRadixTile - 1) / kRadixTile);
+    const int nblk = (int)((cap + kRadixTile - 1) / kRadixTile);
     int kb = 0;
     for (int pass = 0; pass < passes; ++pass) {
         const int shift = 8 * pass;
-        hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(kBlock), 0, c->stream, n, ph->keys[kb].p, shift,
+        hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(kBlock), 0, c->stream, pst, cnt, ph->keys[kb].p, shift,
                            ph->hist.p, nblk);
         scan_with_tmp(c, ph->hist.p, ph->hist.p, 256L * nblk, ph->scan_tmp.p, nullptr);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(kBlock), 0, c->stream, n, ph->keys[kb].p,
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(kBlock), 0, c->stream, pst, cnt, ph->keys[kb].p,
                            pass == 0 ? (const uint32_t*)nullptr : ph->vals[kb].p, ph->keys[1 - kb].p,
                            ph->vals[1 - kb].p, shift, ph->hist.p, nblk);
         kb = 1 - kb;
     }
-    hipLaunchKernelGGL(k_gather, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, ph->vals[kb].p, ph->cell[b].p,
-                       ph->lam12[b].p, ph->lam3[b].p, ph->pos[b].p, ph->vel[b].p, ph->id[b].p, ph->cell[o].p,
-                       ph->lam12[o].p, ph->lam3[o].p, ph->pos[o].p, ph->vel[o].p, ph->id[o].p);
+    hipLaunchKernelGGL(k_gather, dim3(blocks_for(cap)), dim3(kBlock), 0, c->stream, pst, cnt, ph->vals[kb].p,
+                       ph->cell[b].p, ph->lam12[b].p, ph->lam3[b].p, ph->pos[b].p, ph->vel[b].p, ph->id[b].p,
+                       ph->cell[o].p, ph->lam12[o].p, ph->lam3[o].p, ph->pos[o].p, ph->vel[o].p, ph->id[o].p);
     PFEM_HIP(hipGetLastError());
-    ph->cur = o;
 }
 
 static Centers centers_of(const pfem_particles_t* ph)
@@ -625,46 +710,64 @@ static Centers centers_of(const pfem_particles_t* ph)
     return cen;
 }
 
-void particles_advect(pfem_particles_t* ph, const double* u0, const double* u1, double dt, int substeps)
+// Host part of an advection step, before any launch: the count of the previous step, and capacity for
+// the largest refill this step can make (every subcell of every cell). Never inside a captured region.
+void particles_advect_prepare(pfem_particles_t* ph)
 {
     Context* c = C(ph->ctx);
     pfem_mesh2d_t* m = ph->mesh;
-    PFEM_REQUIRE(substeps > 0, "substeps must be positive");
     PFEM_REQUIRE(m->nbr_off.p, "the mesh was created without neighbour lists");
-    const int b = ph->cur;
-    const long P = ph->count;
-    zero_async(c, ph->counters.p, 4 * sizeof(int));
+    sync_host_state(ph);
+    grow(c, ph, ph->count + (long)m->nt * ph->ns);
+}
+
+// The launches of an advection step on buffer b (the particles end in buffer 1 - b): advection and
+// relocation, census, refill, reorder by cell, the new device state and cell_start. Launches only (no
+// host waits, allocations or host-side state), so that it can be captured; the caller sets ph->cur.
+void particles_advect_launch(pfem_particles_t* ph, int b, const double* u0, const double* u1, double dt, int substeps)
+{
+    Context* c = C(ph->ctx);
+    pfem_mesh2d_t* m = ph->mesh;
+    const long cap = ph->capacity;
+    const long* pst = ph->pst.p;
     const double dts = dt / substeps;
-    if (P > 0)
-        hipLaunchKernelGGL(k_advect_relocate, dim3(blocks_for(P)), dim3(kBlock), 0, c->stream, P, substeps, dts,
-                           ph->cell[b].p, ph->lam12[b].p, ph->lam3[b].p, ph->pos[b].p, m->xy.p, m->tri.p,
-                           (const double4*)m->invJ.p, m->nbr_off.p, m->nbr_idx.p, u0, u1, ph->counters.p);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, 4L, (uint32_t*)ph->counters.p);
+    hipLaunchKernelGGL(k_advect_relocate, dim3(blocks_for(cap)), dim3(kBlock), 0, c->stream, pst, substeps, dts,
+                       ph->cell[b].p, ph->lam12[b].p, ph->lam3[b].p, ph->pos[b].p, m->xy.p, m->tri.p,
+                       (const double4*)m->invJ.p, m->nbr_off.p, m->nbr_idx.p, u0, u1, ph->counters.p);
     // census
-    zero_async(c, ph->mask.p, ph->mask.bytes());
-    if (P > 0)
-        hipLaunchKernelGGL(k_census, dim3(blocks_for(P)), dim3(kBlock), 0, c->stream, P, ph->cell[b].p,
-                           ph->lam12[b].p, ph->lam3[b].p, ph->level, 1.0 / ph->level, ph->mask.p);
+    hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(m->nt)), dim3(kBlock), 0, c->stream, (long)m->nt, ph->mask.p);
+    hipLaunchKernelGGL(k_census, dim3(blocks_for(cap)), dim3(kBlock), 0, c->stream, pst, ph->cell[b].p,
+                       ph->lam12[b].p, ph->lam3[b].p, ph->level, 1.0 / ph->level, ph->mask.p);
     hipLaunchKernelGGL(k_refill_count, dim3(blocks_for(m->nt)), dim3(kBlock), 0, c->stream, m->nt, ph->ns,
                        ph->mask.p, ph->add.p);
     scan_with_tmp(c, ph->add.p, ph->add_off.p, m->nt, ph->scan_tmp.p, ph->counters.p + 1);
+    hipLaunchKernelGGL(k_refill_write, dim3(blocks_for(m->nt)), dim3(kBlock), 0, c->stream, m->nt, ph->ns, pst,
+                       ph->mask.p, ph->add.p, ph->add_off.p, m->xy.p, m->tri.p, centers_of(ph), u0, u1,
+                       ph->cell[b].p, ph->lam12[b].p, ph->lam3[b].p, ph->pos[b].p, ph->vel[b].p, ph->id[b].p);
+    sort_by_cell(c, ph, b);
+    hipLaunchKernelGGL(k_pstate_finish, dim3(1), dim3(64), 0, c->stream, ph->pst.p, (const int*)ph->counters.p);
+    compute_cell_start(c, ph, 1 - b);
     PFEM_HIP(hipGetLastError());
-    int h[2];
-    d2h_sync(c, h, ph->counters.p, 2 * sizeof(int));
-    const long dead = h[0], added = h[1];
-    grow(c, ph, P + added);
-    const int b2 = ph->cur;
-    if (added > 0)
-        hipLaunchKernelGGL(k_refill_write, dim3(blocks_for(m->nt)), dim3(kBlock), 0, c->stream, m->nt, ph->ns, P,
-                           ph->next_id, ph->mask.p, ph->add.p, ph->add_off.p, m->xy.p, m->tri.p, centers_of(ph), u0,
-                           u1, ph->cell[b2].p, ph->lam12[b2].p, ph->lam3[b2].p, ph->pos[b2].p, ph->vel[b2].p,
-                           ph->id[b2].p);
-    ph->next_id += (uint32_t)added;
-    sort_by_cell(c, ph, P + added);
-    ph->count = P + added - dead;
-    ph->deleted_last = dead;
-    ph->added_last = added;
-    compute_cell_start(c, ph);
-    PFEM_HIP(hipGetLastError());
+}
+
+// After the launches of a step: the device state's copy to the host, read at its next use
+void particles_advect_post(pfem_particles_t* ph)
+{
+    Context* c = C(ph->ctx);
+    PFEM_HIP(hipMemcpyAsync(ph->h_pst, ph->pst.p, 4 * sizeof(long), hipMemcpyDeviceToHost, c->stream));
+    PFEM_HIP(hipEventRecord(ph->pst_ev, c->stream));
+    ph->pst_pending = true;
+}
+
+void particles_advect(pfem_particles_t* ph, const double* u0, const double* u1, double dt, int substeps)
+{
+    PFEM_REQUIRE(substeps > 0, "substeps must be positive");
+    particles_advect_prepare(ph);
+    const int b = ph->cur;
+    particles_advect_launch(ph, b, u0, u1, dt, substeps);
+    ph->cur = 1 - b;
+    particles_advect_post(ph);
 }
 
 void particles_project(pfem_particles_t* ph, double* u0, double* u1)
@@ -683,12 +786,16 @@ void particles_correct(pfem_particles_t* ph, const double* u0, const double* u1,
 {
     Context* c = C(ph->ctx);
     const int b = ph->cur;
-    if (ph->count > 0)
-        hipLaunchKernelGGL(k_correct, dim3(blocks_for(ph->count)), dim3(kBlock), 0, c->stream, ph->count,
-                           ph->cell[b].p, ph->lam12[b].p, ph->lam3[b].p, ph->vel[b].p, ph->mesh->tri.p, u0, u1, uo0,
-                           uo1);
+    hipLaunchKernelGGL(k_correct, dim3(blocks_for(ph->capacity)), dim3(kBlock), 0, c->stream, (const long*)ph->pst.p,
+                       ph->cell[b].p, ph->lam12[b].p, ph->lam3[b].p, ph->vel[b].p, ph->mesh->tri.p, u0, u1, uo0,
+                       uo1);
     PFEM_HIP(hipGetLastError());
 }
+
+// The particle set's buffer generation (changes when the buffers are reallocated) and current buffer
+long particles_generation(const pfem_particles_t* ph) { return ph->generation; }
+int particles_current(const pfem_particles_t* ph) { return ph->cur; }
+void particles_set_current(pfem_particles_t* ph, int b) { ph->cur = b; }
 
 }  // namespace pfem
 
@@ -730,8 +837,13 @@ int pfem_particles_create(pfem_mesh2d m, int level, pfem_particles* out)
         ph->counters.alloc(4);
         ph->cell_start.alloc(m->nt + 1);
         ph->part.alloc(9 * (size_t)m->nt);
+        ph->pst.alloc(4);
+        PFEM_HIP(hipHostMalloc((void**)&ph->h_pst, 4 * sizeof(long), hipHostMallocDefault));
+        PFEM_HIP(hipEventCreateWithFlags(&ph->pst_ev, hipEventDisableTiming));
         zero_async(c, ph->counters.p, 4 * sizeof(int));
         alloc_state(ph.get(), 1024);
+        put_device_state(c, ph.get());
+        compute_cell_start(c, ph.get(), ph->cur);
         PFEM_HIP(hipStreamSynchronize(c->stream));
         *out = ph.release();
     });
@@ -754,6 +866,7 @@ int pfem_particles_seed(pfem_particles ph, int mode, int ppc, uint64_t seed)
         PFEM_REQUIRE(ppc > 0, "particles per cell must be positive");
         Context* c = C(ph->ctx);
         const long P = (long)ph->mesh->nt * ppc;
+        sync_host_state(ph);
         ph->count = 0;
         grow(c, ph, P + P / 5);
         const int b = ph->cur;
@@ -764,7 +877,9 @@ int pfem_particles_seed(pfem_particles ph, int mode, int ppc, uint64_t seed)
         PFEM_HIP(hipGetLastError());
         ph->count = P;
         ph->next_id = (uint32_t)P;
-        compute_cell_start(c, ph);
+        ph->deleted_last = ph->added_last = 0;
+        put_device_state(c, ph);
+        compute_cell_start(c, ph, b);
         PFEM_HIP(hipStreamSynchronize(c->stream));
     });
 }
@@ -804,9 +919,11 @@ int pfem_particles_correct(pfem_particles ph, const double* d_u0, const double* 
 
 int pfem_particles_count(pfem_particles ph, long* count)
 {
-    if (!ph || !count) return PFEM_EINVAL;
-    *count = ph->count;
-    return PFEM_OK;
+    PFEM_TRY(ph ? ph->ctx : nullptr, {
+        PFEM_REQUIRE(ph && count, "null argument");
+        sync_host_state(ph);
+        *count = ph->count;
+    });
 }
 
 int pfem_particles_download(pfem_particles ph, int* cell, double* lam, double* pos, double* vel, uint32_t* id)
@@ -814,6 +931,7 @@ int pfem_particles_download(pfem_particles ph, int* cell, double* lam, double* p
     PFEM_TRY(ph ? ph->ctx : nullptr, {
         PFEM_REQUIRE(ph, "null argument");
         Context* c = C(ph->ctx);
+        sync_host_state(ph);
         const size_t n = ph->count;
         const int b = ph->cur;
         if (cell) d2h_sync(c, cell, ph->cell[b].p, n * sizeof(int));
@@ -844,6 +962,7 @@ int pfem_particles_upload(pfem_particles ph, long count, const int* cell, const 
             PFEM_REQUIRE(p == 0 || cell[p] >= cell[p - 1], "uploaded particles must be ordered by cell");
         }
         Context* c = C(ph->ctx);
+        sync_host_state(ph);
         ph->count = 0;
         grow(c, ph, count + count / 5 + 16);
         const int b = ph->cur;
@@ -861,17 +980,21 @@ int pfem_particles_upload(pfem_particles ph, long count, const int* cell, const 
         h2d(c, ph->id[b].p, id, count * sizeof(uint32_t));
         ph->count = count;
         ph->next_id = next_id;
-        compute_cell_start(c, ph);
+        ph->deleted_last = ph->added_last = 0;
+        put_device_state(c, ph);
+        compute_cell_start(c, ph, b);
         PFEM_HIP(hipStreamSynchronize(c->stream));
     });
 }
 
 int pfem_particles_stats(pfem_particles ph, long* deleted_last, long* added_last)
 {
-    if (!ph) return PFEM_EINVAL;
-    if (deleted_last) *deleted_last = ph->deleted_last;
-    if (added_last) *added_last = ph->added_last;
-    return PFEM_OK;
+    PFEM_TRY(ph ? ph->ctx : nullptr, {
+        PFEM_REQUIRE(ph, "null argument");
+        sync_host_state(ph);
+        if (deleted_last) *deleted_last = ph->deleted_last;
+        if (added_last) *added_last = ph->added_last;
+    });
 }
 
 }  // extern "C"

@@ -3,6 +3,13 @@
 // All systems share the mesh's CSR pattern; the two correction matrices of the reference are one matrix
 // with two right-hand sides (they are identical, :248-251), solved together as a multi-RHS PCG in mixed
 // precision when the correction solver is Jacobi PCG.
+//
+// With graphs on (pfem_pfem2_set_graphs), the work between the solves -- the particle step, the
+// projection, the three assemblies with their Dirichlet passes, the particle correction and the loads --
+// is captured as six HIP graphs on first use and replayed afterwards; only the Krylov solves, whose
+// iteration counts are data-dependent, are launched kernel by kernel. The particle step has no host
+// round trip (its counts stay on the device), so the graphs hold the same launches as the eager step
+// and give the same bits.
 #include <cstdio>
 #include <cstring>
 
@@ -12,6 +19,12 @@
 
 namespace pfem {
 void particles_advect(pfem_particles_t* ph, const double* u0, const double* u1, double dt, int substeps);
+void particles_advect_prepare(pfem_particles_t* ph);
+void particles_advect_launch(pfem_particles_t* ph, int b, const double* u0, const double* u1, double dt, int substeps);
+void particles_advect_post(pfem_particles_t* ph);
+long particles_generation(const pfem_particles_t* ph);
+int particles_current(const pfem_particles_t* ph);
+void particles_set_current(pfem_particles_t* ph, int b);
 void particles_project(pfem_particles_t* ph, double* u0, double* u1);
 void particles_correct(pfem_particles_t* ph, const double* u0, const double* u1, const double* uo0, const double* uo1);
 void assemble_prediction(pfem_mesh2d m, pfem_csr a0, pfem_csr a1, double* rhs0, double* rhs1, const double* u0,
@@ -44,9 +57,28 @@ struct pfem_pfem2_t {
     bool timing = false;
     double phase_ms[8] = {0};
     hipEvent_t ev[2] = {nullptr, nullptr};
+    // graph replay of the work between the solves: segment k, buffer variant v (the particle buffers
+    // alternate), captured for the particle buffers' generation `key`
+    struct Segment {
+        hipGraphExec_t exec = nullptr;
+        long key = -1;
+    };
+    static constexpr int kSegments = 6;
+    bool graphs = false;
+    long steps = 0;   // steps taken (graphs are captured from the second one, after the lazy set-ups)
+    Segment seg[kSegments][2];
     double* field(int k) { return f.p + (size_t)k * n; }
+    void drop_graphs()
+    {
+        for (auto& row : seg)
+            for (auto& g : row) {
+                if (g.exec) (void)hipGraphExecDestroy(g.exec);
+                g = Segment{};
+            }
+    }
     ~pfem_pfem2_t()
     {
+        drop_graphs();
         pfem_solver_destroy(s_pred); pfem_solver_destroy(s_pres); pfem_solver_destroy(s_corr);
         pfem_solver_destroy(s_pred1);
         pfem_precond_destroy(pc_pred); pfem_precond_destroy(pc_pres); pfem_precond_destroy(pc_corr);
@@ -86,6 +118,38 @@ struct Phase {
         s->phase_ms[k] += ms;
     }
 };
+
+// Runs body (launches only) directly, or from segment k's graph (variant v, captured for key on first use
+// and whenever the key changes) when graphs are on and a first step has made the lazy set-ups.
+template <typename F>
+void run_segment(pfem_pfem2_t* s, int k, int v, long key, F&& body)
+{
+    if (!s->graphs || s->steps == 0) {
+        body();
+        return;
+    }
+    hipStream_t st = s->ctx->stream;
+    pfem_pfem2_t::Segment& g = s->seg[k][v];
+    if (!g.exec || g.key != key) {
+        if (g.exec) PFEM_HIP(hipGraphExecDestroy(g.exec));
+        g = pfem_pfem2_t::Segment{};
+        hipGraph_t graph = nullptr;
+        PFEM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            body();
+        } catch (...) {
+            (void)hipStreamEndCapture(st, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        PFEM_HIP(hipStreamEndCapture(st, &graph));
+        const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        PFEM_HIP(e);
+        g.key = key;
+    }
+    PFEM_HIP(hipGraphLaunch(g.exec, st));
+}
 
 void make_solver(pfem_pfem2_t* s, pfem_csr a, int kind, int pc_kind, int ordering, pfem_precond* pc, pfem_solver* sv,
                  int factor_bits = 32)
@@ -168,11 +232,14 @@ int pfem_pfem2_create(pfem_mesh2d m, pfem_particles ph, const pfem_step_config* 
         // Arnoldi step keeps their iteration counts (cylinder, 2M triangles: prediction 9.0 -> 5.9 ms per
         // step in FP64); pred_gs_passes = 2 orthogonalises twice
         PFEM_REQUIRE(cfg->pred_gs_passes >= 0 && cfg->pred_gs_passes <= 2, "pred_gs_passes must be 0, 1 or 2");
-        if (cfg->pred_solver == PFEM_SOLVER_GMRES) {
+        if (cfg->pred_solver == PFEM_SOLVER_GMRES)
             check(pfem_solver_set_option(s->s_pred, PFEM_SOLVER_OPT_GS_PASSES, cfg->pred_gs_passes == 2 ? 2 : 1),
                   s->ctx);
-            // the second component's own solver: the two prediction systems are solved as a pair
-            // (pfem_solver_solve_pair: lockstep Arnoldi steps, one SpMV launch streaming the pattern once)
+        if (cfg->pred_solver == PFEM_SOLVER_GMRES && cfg->precision == PFEM_PRECISION_MIXED) {
+            // the second component's own solver: in mixed precision the two prediction systems are solved
+            // as a pair (pfem_solver_solve_pair: lockstep Arnoldi steps, one SpMV launch streaming the
+            // pattern once). In FP64 they run one after the other: there the pair measured slower
+            // (cylinder, 2M triangles: 6.75 against 6.20 ms per step; mixed: 5.44 against 5.77 ms).
             make_solver(s.get(), s->A1, cfg->pred_solver, cfg->pred_precond, PFEM_ORDER_NATURAL, &s->pc_pred1,
                         &s->s_pred1);
             check(pfem_solver_set_option(s->s_pred1, PFEM_SOLVER_OPT_GS_PASSES, cfg->pred_gs_passes == 2 ? 2 : 1),
@@ -209,27 +276,39 @@ int pfem_pfem2_step(pfem_pfem2 s, double forces[2])
         const bool B = prm->scheme == 1;
         double* pold = B ? s->field(POLD) : nullptr;
         s->its.clear();
+        PFEM_REQUIRE(cfg.substeps > 0, "substeps must be positive");
         {
             Phase ph(s, 0);
-            particles_advect(s->ph, s->field(U0), s->field(U1), prm->dt, cfg.substeps);
+            particles_advect_prepare(s->ph);
+            const int b = particles_current(s->ph);
+            const long key = particles_generation(s->ph);
+            run_segment(s, 0, b, key, [&] {
+                particles_advect_launch(s->ph, b, s->field(U0), s->field(U1), prm->dt, cfg.substeps);
+            });
+            particles_set_current(s->ph, 1 - b);
+            particles_advect_post(s->ph);
         }
         {
             Phase ph(s, 1);
-            particles_project(s->ph, s->field(U0), s->field(U1));
-            launch_copy(c, s->field(UO0), s->field(U0), n);
-            launch_copy(c, s->field(UO1), s->field(U1), n);
-            if (B) launch_copy(c, s->field(POLD), s->field(P), n);
+            run_segment(s, 1, particles_current(s->ph), particles_generation(s->ph), [&] {
+                particles_project(s->ph, s->field(U0), s->field(U1));
+                launch_copy(c, s->field(UO0), s->field(U0), n);
+                launch_copy(c, s->field(UO1), s->field(U1), n);
+                if (B) launch_copy(c, s->field(POLD), s->field(P), n);
+            });
         }
         for (int outer = 0; outer < cfg.outer_iterations; ++outer) {
             {
                 Phase ph(s, 2);
-                assemble_prediction(s->m, s->A0, s->A1, s->field(RHS0), s->field(RHS1), s->field(U0), s->field(U1),
-                                    s->field(UO0), s->field(UO1), pold, prm);
-                for (int k = 0; k < 2; ++k) {
-                    prediction_bc_values(s->m, s->bcpred[k], s->bcu[k], s->field(P), pold, k, prm);
-                    dirichlet_apply(c, s->bcpred[k], nullptr, k == 0 ? s->A0 : s->A1, s->field(RHS0 + k), nullptr,
-                                    cfg.dirichlet_mode);
-                }
+                run_segment(s, 2, 0, 0, [&] {
+                    assemble_prediction(s->m, s->A0, s->A1, s->field(RHS0), s->field(RHS1), s->field(U0),
+                                        s->field(U1), s->field(UO0), s->field(UO1), pold, prm);
+                    for (int k = 0; k < 2; ++k) {
+                        prediction_bc_values(s->m, s->bcpred[k], s->bcu[k], s->field(P), pold, k, prm);
+                        dirichlet_apply(c, s->bcpred[k], nullptr, k == 0 ? s->A0 : s->A1, s->field(RHS0 + k),
+                                        nullptr, cfg.dirichlet_mode);
+                    }
+                });
             }
             {
                 Phase ph(s, 3);
@@ -247,8 +326,10 @@ int pfem_pfem2_step(pfem_pfem2 s, double forces[2])
             }
             {
                 Phase ph(s, 2);
-                assemble_pressure(s->m, s->Ap, s->field(RHS0), s->field(VP0), s->field(VP1), pold, prm);
-                dirichlet_apply(c, s->bcp, nullptr, s->Ap, s->field(RHS0), nullptr, cfg.dirichlet_mode);
+                run_segment(s, 3, 0, 0, [&] {
+                    assemble_pressure(s->m, s->Ap, s->field(RHS0), s->field(VP0), s->field(VP1), pold, prm);
+                    dirichlet_apply(c, s->bcp, nullptr, s->Ap, s->field(RHS0), nullptr, cfg.dirichlet_mode);
+                });
             }
             {
                 Phase ph(s, 4);
@@ -256,9 +337,12 @@ int pfem_pfem2_step(pfem_pfem2 s, double forces[2])
             }
             {
                 Phase ph(s, 2);
-                assemble_correction(s->m, s->Ac, s->field(RHS0), s->field(RHS1), s->field(VP0), s->field(VP1),
-                                    s->field(P), pold, prm);
-                dirichlet_apply(c, s->bcu[0], s->bcu[1], s->Ac, s->field(RHS0), s->field(RHS1), cfg.dirichlet_mode);
+                run_segment(s, 4, 0, 0, [&] {
+                    assemble_correction(s->m, s->Ac, s->field(RHS0), s->field(RHS1), s->field(VP0), s->field(VP1),
+                                        s->field(P), pold, prm);
+                    dirichlet_apply(c, s->bcu[0], s->bcu[1], s->Ac, s->field(RHS0), s->field(RHS1),
+                                    cfg.dirichlet_mode);
+                });
             }
             {
                 Phase ph(s, 5);
@@ -281,11 +365,14 @@ int pfem_pfem2_step(pfem_pfem2 s, double forces[2])
         }
         {
             Phase ph(s, 6);
-            particles_correct(s->ph, s->field(U0), s->field(U1), s->field(UO0), s->field(UO1));
             double* fd = s->f.p + (size_t)NF * n;
+            run_segment(s, 5, particles_current(s->ph), particles_generation(s->ph), [&] {
+                particles_correct(s->ph, s->field(U0), s->field(U1), s->field(UO0), s->field(UO1));
+                if (cfg.body_id >= 0)
+                    body_forces(s->m, cfg.body_id, cfg.body_cx, cfg.body_cy, s->field(U0), s->field(U1), s->field(P),
+                                prm, fd);
+            });
             if (cfg.body_id >= 0) {
-                body_forces(s->m, cfg.body_id, cfg.body_cx, cfg.body_cy, s->field(U0), s->field(U1), s->field(P), prm,
-                            fd);
                 if (forces) {
                     double h[4];
                     d2h_sync(c, h, fd, sizeof(h));
@@ -297,6 +384,16 @@ int pfem_pfem2_step(pfem_pfem2 s, double forces[2])
             }
         }
         PFEM_HIP(hipGetLastError());
+        ++s->steps;
+    });
+}
+
+int pfem_pfem2_set_graphs(pfem_pfem2 s, int on)
+{
+    PFEM_TRY(s ? s->ctx : nullptr, {
+        PFEM_REQUIRE(s, "null argument");
+        s->graphs = on != 0;
+        if (!s->graphs) s->drop_graphs();
     });
 }
 

@@ -403,6 +403,13 @@ int pfem_pfem2_last_iterations(pfem_pfem2 s, int* its, int cap);
  * 6 particle correction + loads. Timing is off unless enabled (it synchronises at phase ends). */
 int pfem_pfem2_enable_timing(pfem_pfem2 s, int on);
 int pfem_pfem2_phase_times(pfem_pfem2 s, double* ms, int cap);
+/* Whole-step orchestration (replaces the kernel-by-kernel launches of the time loop,
+ * cases/Cylinder2D/main.cu:793-880): with on != 0, the work between the solves (particle step,
+ * projection, the three assemblies with their Dirichlet passes, particle correction and loads) is
+ * captured as HIP graphs at the step after the call (a first step, eager, makes the one-time set-ups)
+ * and replayed at every later step; the Krylov solves stay kernel by kernel. The results are the same
+ * bits as the eager step. Graphs are re-captured after the particle buffers grow; 0 destroys them. */
+int pfem_pfem2_set_graphs(pfem_pfem2 s, int on);
 /* Solves re-run since creation because a single-pass triangular sweep's wait exceeded its spin limit (no
  * counterpart in the reference, whose SpSV has no such wait): such a solve is repeated from the same start
  * with per-level sweeps and the FP64 factor instead of failing the step, and a line goes to stderr. */

@@ -193,3 +193,33 @@ def test_particles_full_size(ctx, oracle):
     ph.correct(p0, p1, du0, du1)
     oph.correct(q0, q1, u0, u1)
     same()
+
+
+@pytest.mark.parametrize("ci,precision", [(0, 0), (1, 0), (1, 1)])
+def test_step_graphs_bitexact(ctx, ci, precision):
+    """Whole-step orchestration (pfem_pfem2_set_graphs): the work between the solves captured as HIP graphs
+    at the second step and replayed at the later ones gives the same bits as the eager steps -- fields,
+    forces, Krylov iteration counts and every particle array -- with eager work (a field download)
+    between the replays. The particle step keeps its counts on the device (no host round trip)."""
+    from gpupfem2_amd import cases
+    case = small_cases()[ci]
+    print("step graphs", case.name, "precision", precision, flush=True)
+    runs = []
+    for graphs in (False, True):
+        mesh, ph, stepper = cases.build_gpu(ctx, case, precision=precision, inner_tol=1e-5)
+        if graphs:
+            stepper.set_graphs(True)
+        out = []
+        for s in range(4):
+            f = stepper.step()
+            out.append((f.copy(), stepper.fields(), stepper.iterations().copy(), ph.get(), ph.stats(), ph.count()))
+        runs.append(out)
+        del stepper, ph, mesh
+    for s, (a, b) in enumerate(zip(*runs)):
+        assert np.array_equal(a[0], b[0]), (s, a[0], b[0])
+        for k in a[1]:
+            assert np.array_equal(a[1][k], b[1][k]), (s, k)
+        assert np.array_equal(a[2], b[2]), s
+        for k in a[3]:
+            assert np.array_equal(a[3][k], b[3][k]), (s, k)
+        assert a[4] == b[4] and a[5] == b[5], s
