@@ -12,7 +12,9 @@ PCG + Jacobi, relative tolerance 1e-12), in the mixed-precision pipeline and in 
 One step = advection + relocation + refill + reorder of all particles, projection, the two outer
 iterations of prediction / pressure / correction (5 assemblies + 5 solves each), particle velocity
 correction and the drag/lift integral. The flow starts from rest; W warm-up steps, then K timed
-steps bracketed by device synchronisation. Prints one JSON line.
+steps bracketed by device synchronisation. The work between the solves is replayed from HIP graphs
+(pfem_pfem2_set_graphs; --no-graphs launches it kernel by kernel). Prints one JSON line, with the
+mixed pipeline's error against the FP64 one after the same steps (fields, C_D, C_L).
 
 Mixed precision: Jacobi-preconditioned solves run FP32 inner Krylov + FP64 refinement; the IC0
 pressure solve runs FP64 PCG with the IC0 factor stored in 16 bits (IEEE half scaled by a power of two;
@@ -33,10 +35,12 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def gpu_run(ctx, case, precision, steps, warmup, inner_tol):
+def gpu_run(ctx, case, precision, steps, warmup, inner_tol, graphs=True):
     from gpupfem2_amd import cases
     t0 = time.perf_counter()
     mesh, ph, st = cases.build_gpu(ctx, case, precision=precision, inner_tol=inner_tol)
+    if graphs:
+        st.set_graphs(True)
     ctx.sync()
     setup = time.perf_counter() - t0
     for _ in range(warmup):
@@ -50,7 +54,7 @@ def gpu_run(ctx, case, precision, steps, warmup, inner_tol):
     ctx.sync()
     dt = time.perf_counter() - t0
     out = dict(steps_per_s=steps / dt, ms_per_step=1e3 * dt / steps, setup_s=setup, iterations=its,
-               particles=ph.count(), cd_cl=forces.tolist())
+               particles=ph.count(), cd_cl=forces.tolist(), fields={k: st.field(k) for k in ("u0", "u1", "p")})
     return out, (mesh, ph, st)
 
 
@@ -452,6 +456,8 @@ def main():
     ap.add_argument("--mode", choices=["both", "fp64", "mixed"], default="both")
     ap.add_argument("--inner-tol", type=float, default=1e-5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch the work between the solves kernel by kernel instead of from HIP graphs")
     ap.add_argument("--kernels", action="store_true", help="pressure-solve kernels only (counter runs)")
     ap.add_argument("--launches", type=int, default=20)
     ap.add_argument("--workload", choices=["cylinder", "poisson3d", "batched", "history"], default="cylinder",
@@ -485,7 +491,7 @@ def main():
     for prec in ([0, 1] if args.mode == "both" else [0] if args.mode == "fp64" else [1]):
         print(f"[bench] precision {'mixed' if prec else 'fp64'}: {case.tri.shape[0]} triangles, "
               f"{args.warmup} warm-up + {args.steps} timed steps", file=sys.stderr, flush=True)
-        r, (mesh, ph, st) = gpu_run(ctx, case, prec, args.steps, args.warmup, args.inner_tol)
+        r, (mesh, ph, st) = gpu_run(ctx, case, prec, args.steps, args.warmup, args.inner_tol, not args.no_graphs)
         res[prec] = r
         print(f"[bench] {r['steps_per_s']:.3f} steps/s, iterations {r['iterations'][-1]}", file=sys.stderr,
               flush=True)
@@ -504,6 +510,18 @@ def main():
             cpu["gpu_cd_cl_same_step"] = [float(v) for v in st.step()]
         del st, ph, mesh
     head = res.get(1, res.get(0))
+    err = None
+    if len(res) == 2:
+        # the two pipelines after the same steps from the same start (rest, the same particle seeds)
+        import numpy as np
+        f64, fmx = res[0]["fields"], res[1]["fields"]
+        cd = [abs(a - b) / abs(a) for a, b in zip(res[0]["cd_cl"], res[1]["cd_cl"])]
+        err = {"after_steps": args.warmup + args.steps,
+               "rel_l2": {k: float(np.linalg.norm(fmx[k] - f64[k]) / np.linalg.norm(f64[k])) for k in f64},
+               "cd_rel": cd[0], "cl_rel": cd[1],
+               "bound": "fields rel L2 <= 1e-8, C_D and C_L <= 1e-6 relative against FP64"}
+    for r in res.values():
+        r.pop("fields", None)
     line = {
         "metric": "PFEM steps/s mixed vs FP64 on 2M-tri cylinder mesh; solver HBM TB/s",
         "value": round(head["steps_per_s"], 4),
@@ -530,6 +548,8 @@ def main():
         "mixed_over_fp64": round(res[1]["steps_per_s"] / res[0]["steps_per_s"], 3) if len(res) == 2 else None,
         "krylov_iterations_last_step": {("mixed" if k else "fp64"): v["iterations"][-1] for k, v in res.items()},
         "cd_cl_last_step": {("mixed" if k else "fp64"): v["cd_cl"] for k, v in res.items()},
+        "mixed_vs_fp64": err,
+        "step_graphs": not args.no_graphs,
         "roofline": roof_pcg.get("mixed", roof_pcg.get("fp64")),
         "roofline_pcg_iteration_fp64": roof_pcg.get("fp64"),
         "roofline_ic0_apply": roof_ic0,
