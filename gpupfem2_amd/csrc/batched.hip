@@ -6,8 +6,11 @@
 //
 // Work is cut into SpMV row tiles (<= 256 rows, <= 2048 non-zeros) that never straddle two systems, and
 // every vector kernel works tile by tile, so a workgroup knows its system from the tile and a finished
-// system's tiles return at once. Per-system dot products: one partial per tile, added by one workgroup
-// per system over the system's contiguous tile range, in tile order (deterministic). Per-system time to
+// system's tiles return at once. Workgroups take the tiles in XCD bands (xcd_band_tile): the tiles of one
+// system's neighbouring rows, whose SpMV gathers overlap, run on one XCD and share its L2 (the FP64 SpMV
+// of the 256-mesh batch fetches 1.44 instead of 1.76 GB, in the same time). Per-system dot products: one
+// partial per tile, added by one workgroup per system over the system's contiguous tile range, in tile
+// order (deterministic). Per-system time to
 // solution is taken from the device wall clock when the system stops.
 //
 // PFEM_PRECISION_MIXED: PCG with reliable updates (residual replacement) per system: FP32 vectors and the
@@ -71,12 +74,13 @@ __global__ __launch_bounds__(kBlock) void k_b_init(const int* __restrict__ tiles
                                                    double* __restrict__ part, int x_zero)
 {
     __shared__ T prod[kTileNnz];
+    const int t = xcd_band_tile();
     int row, r1;
     T ax = 0;
-    if (!x_zero) ax = tile_row_product<VT, T>(tiles, rowptr, col, val, x, prod, row, r1);
+    if (!x_zero) ax = tile_row_product_at<VT, T>(t, tiles, rowptr, col, val, x, prod, row, r1);
     else {
-        row = tiles[2 * blockIdx.x] + (int)threadIdx.x;
-        r1 = tiles[2 * blockIdx.x + 1];
+        row = tiles[2 * t] + (int)threadIdx.x;
+        r1 = tiles[2 * t + 1];
     }
     double v[2] = {0.0, 0.0};
     if (row < r1) {
@@ -89,7 +93,7 @@ __global__ __launch_bounds__(kBlock) void k_b_init(const int* __restrict__ tiles
         v[0] = (double)(rr * zz);
         v[1] = (double)(bb * mb);
     }
-    block_partials<2>(v, part);
+    block_partials<2>(v, part, t);
 }
 
 // Sum of the tile partials of the system at position blockIdx.x (series i at part + i * ntiles), in tile
@@ -143,15 +147,16 @@ __global__ __launch_bounds__(kBlock) void k_b_spmv(const int* __restrict__ tiles
                                                    const BSys* __restrict__ sys, double* __restrict__ part)
 {
     __shared__ T prod[kTileNnz];
-    if (sys[tsys[blockIdx.x]].done) return;
+    const int t = xcd_band_tile();
+    if (sys[tsys[t]].done) return;
     int row, r1;
-    const T ap = tile_row_product<VT, T>(tiles, rowptr, col, val, p, prod, row, r1);
+    const T ap = tile_row_product_at<VT, T>(t, tiles, rowptr, col, val, p, prod, row, r1);
     double v[1] = {0.0};
     if (row < r1) {
         q[row] = ap;
         v[0] = (double)(p[row] * ap);
     }
-    block_partials<1>(v, part);
+    block_partials<1>(v, part, t);
 }
 
 // alpha per system; one workgroup per system
@@ -172,10 +177,11 @@ __global__ __launch_bounds__(kBlock) void k_b_update(const int* __restrict__ til
                                                      const T* __restrict__ q, const T* __restrict__ invd,
                                                      const BSys* __restrict__ sys, double* __restrict__ part)
 {
-    const BSys& S = sys[tsys[blockIdx.x]];
+    const int t = xcd_band_tile();
+    const BSys& S = sys[tsys[t]];
     if (S.done) return;
     const T alpha = (T)S.alpha;
-    const int row = tiles[2 * blockIdx.x] + (int)threadIdx.x, r1 = tiles[2 * blockIdx.x + 1];
+    const int row = tiles[2 * t] + (int)threadIdx.x, r1 = tiles[2 * t + 1];
     double v[1] = {0.0};
     if (row < r1) {
         x[row] += alpha * p[row];
@@ -184,7 +190,7 @@ __global__ __launch_bounds__(kBlock) void k_b_update(const int* __restrict__ til
         const T zz = JAC ? rr * invd[row] : rr;
         v[0] = (double)(rr * zz);
     }
-    block_partials<1>(v, part);
+    block_partials<1>(v, part, t);
 }
 
 // gamma_{k+1}, beta, the stopping test and the time of convergence per system; one workgroup per system
@@ -219,10 +225,11 @@ __global__ __launch_bounds__(kBlock) void k_b_p(const int* __restrict__ tiles, c
                                                 T* __restrict__ p, const T* __restrict__ r, const T* __restrict__ invd,
                                                 const BSys* __restrict__ sys)
 {
-    const BSys& S = sys[tsys[blockIdx.x]];
+    const int t = xcd_band_tile();
+    const BSys& S = sys[tsys[t]];
     if (S.done) return;
     const T beta = (T)S.beta;
-    const int row = tiles[2 * blockIdx.x] + (int)threadIdx.x, r1 = tiles[2 * blockIdx.x + 1];
+    const int row = tiles[2 * t] + (int)threadIdx.x, r1 = tiles[2 * t + 1];
     if (row < r1) {
         const T zz = JAC ? r[row] * invd[row] : r[row];
         p[row] = zz + beta * p[row];
@@ -240,8 +247,9 @@ __global__ __launch_bounds__(kBlock) void k_br_init(const int* __restrict__ tile
                                                     double* __restrict__ part)
 {
     __shared__ double prod[kTileNnz];
+    const int t = xcd_band_tile();
     int row, r1;
-    const double ax = tile_row_product<double, double>(tiles, rowptr, col, val, x, prod, row, r1);
+    const double ax = tile_row_product_at<double, double>(t, tiles, rowptr, col, val, x, prod, row, r1);
     double v[2] = {0.0, 0.0};
     if (row < r1) {
         const double bb = b[row];
@@ -254,7 +262,7 @@ __global__ __launch_bounds__(kBlock) void k_br_init(const int* __restrict__ tile
         v[0] = rr * (rr * d);
         v[1] = bb * (bb * d);
     }
-    block_partials<2>(v, part);
+    block_partials<2>(v, part, t);
 }
 
 __global__ void k_br_begin(BSys* __restrict__ sys, const double* __restrict__ part, int ntiles,
@@ -290,26 +298,27 @@ __global__ __launch_bounds__(kBlock) void k_br_spmv(const int* __restrict__ tile
                                                     const BSys* __restrict__ sys, double* __restrict__ part)
 {
     __shared__ double prod[kTileNnz];
-    const BSys& S = sys[tsys[blockIdx.x]];
+    const int t = xcd_band_tile();
+    const BSys& S = sys[tsys[t]];
     if (S.done) return;
     int row, r1;
     double v[1] = {0.0};
     if (S.mode == 1) {
-        const double ax = tile_row_product<double, double>(tiles, rowptr, col, val, x, prod, row, r1);
+        const double ax = tile_row_product_at<double, double>(t, tiles, rowptr, col, val, x, prod, row, r1);
         if (row < r1) {
             const double rr = b[row] - ax;
             rs[row] = (float)rr;
             v[0] = rr * (rr * (JAC ? invd[row] : 1.0));
         }
     } else {
-        const float ap = tile_row_product<float, float>(tiles, rowptr, col, val32, ps, reinterpret_cast<float*>(prod),
-                                                        row, r1);
+        const float ap = tile_row_product_at<float, float>(t, tiles, rowptr, col, val32, ps,
+                                                           reinterpret_cast<float*>(prod), row, r1);
         if (row < r1) {
             qs[row] = ap;
             v[0] = (double)(ps[row] * ap);
         }
     }
-    block_partials<1>(v, part);
+    block_partials<1>(v, part, t);
 }
 
 // alpha of an iterating system; for a replaced one the true gamma: the stopping test, then beta
@@ -351,10 +360,11 @@ __global__ __launch_bounds__(kBlock) void k_br_update(const int* __restrict__ ti
                                                       const float* __restrict__ invd32, const BSys* __restrict__ sys,
                                                       double* __restrict__ part)
 {
-    const BSys& S = sys[tsys[blockIdx.x]];
+    const int t = xcd_band_tile();
+    const BSys& S = sys[tsys[t]];
     if (S.done || S.mode != 0) return;
     const float alpha = (float)S.alpha;
-    const int row = tiles[2 * blockIdx.x] + (int)threadIdx.x, r1 = tiles[2 * blockIdx.x + 1];
+    const int row = tiles[2 * t] + (int)threadIdx.x, r1 = tiles[2 * t + 1];
     double v[1] = {0.0};
     if (row < r1) {
         xs[row] += alpha * ps[row];
@@ -362,7 +372,7 @@ __global__ __launch_bounds__(kBlock) void k_br_update(const int* __restrict__ ti
         rs[row] = rr;
         v[0] = (double)(rr * (JAC ? rr * invd32[row] : rr));
     }
-    block_partials<1>(v, part);
+    block_partials<1>(v, part, t);
 }
 
 // gamma of the recursive residual: the iteration count, then beta or a replacement
@@ -397,9 +407,10 @@ __global__ __launch_bounds__(kBlock) void k_br_p(const int* __restrict__ tiles, 
                                                  const float* __restrict__ invd32, double* __restrict__ x,
                                                  float* __restrict__ xs, const BSys* __restrict__ sys)
 {
-    const BSys& S = sys[tsys[blockIdx.x]];
+    const int t = xcd_band_tile();
+    const BSys& S = sys[tsys[t]];
     if (S.done) return;
-    const int row = tiles[2 * blockIdx.x] + (int)threadIdx.x, r1 = tiles[2 * blockIdx.x + 1];
+    const int row = tiles[2 * t] + (int)threadIdx.x, r1 = tiles[2 * t + 1];
     if (row >= r1) return;
     if (S.mode == 1) {
         x[row] += (double)xs[row];

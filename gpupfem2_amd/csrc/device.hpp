@@ -62,13 +62,19 @@ __device__ __forceinline__ void block_sum(double (&v)[NV])
 //
 // part[i * nb + b] holds value i of block b (nb = producer's grid size).
 template <int NV>
-__device__ __forceinline__ void block_partials(double (&v)[NV], double* __restrict__ part)
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* __restrict__ part, int slot)
 {
     block_sum<NV>(v);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int i = 0; i < NV; ++i) part[(size_t)i * gridDim.x + blockIdx.x] = v[i];
+        for (int i = 0; i < NV; ++i) part[(size_t)i * gridDim.x + slot] = v[i];
     }
+}
+
+template <int NV>
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* __restrict__ part)
+{
+    block_partials<NV>(v, part, (int)blockIdx.x);
 }
 
 // Totals of the nb partials of each of the NV values; valid in every thread. Lane t adds partials

@@ -234,6 +234,7 @@ __global__ void k_census(const long* __restrict__ pst, const int* __restrict__ c
     // first lane of each run of equal cells issues the atomic: one atomic per run instead of per particle.
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long P = pst[0];
+    if ((long)blockIdx.x * blockDim.x >= P) return;   // a whole workgroup past the count (the grid covers the capacity)
     const int lane = threadIdx.x & 63;
     int c = -1;
     uint32_t bits = 0u;
@@ -432,6 +433,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const long* __restrict
                                                           const int* __restrict__ offs, int nblocks)
 {
     const long n = sort_count(pst, counters);
+    if ((long)blockIdx.x * kRadixTile >= n) return;   // a tile past the count: no keys, nothing to place
     __shared__ int base[256];
     __shared__ int wcnt[kWaves][256];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -843,7 +845,7 @@ int pfem_particles_create(pfem_mesh2d m, int level, pfem_particles* out)
         zero_async(c, ph->counters.p, 4 * sizeof(int));
         alloc_state(ph.get(), 1024);
         put_device_state(c, ph.get());
-        compute_cell_start(c, ph.get(), ph->cur);
+        zero_async(c, ph->cell_start.p, ph->cell_start.bytes());   // no particles: every cell's range is empty
         PFEM_HIP(hipStreamSynchronize(c->stream));
         *out = ph.release();
     });

@@ -12,14 +12,13 @@ namespace pfem {
 
 constexpr int kTileNnz = 2048;
 
-// Product of the calling lane's row with x (tile t = blockIdx.x: rows [tiles[2t], tiles[2t+1])). Returns
-// the row index in r (r >= r1 means the lane owns no row). prod is an LDS buffer of kTileNnz entries.
+// Product of the calling lane's row with x (tile t: rows [tiles[2t], tiles[2t+1])). Returns the row index
+// in r (r >= r1 means the lane owns no row). prod is an LDS buffer of kTileNnz entries.
 template <typename VT, typename XT>
-__device__ __forceinline__ XT tile_row_product(const int* __restrict__ tiles, const int* __restrict__ rowptr,
-                                               const int* __restrict__ col, const VT* __restrict__ val,
-                                               const XT* __restrict__ x, XT* prod, int& r, int& r1)
+__device__ __forceinline__ XT tile_row_product_at(int t, const int* __restrict__ tiles, const int* __restrict__ rowptr,
+                                                  const int* __restrict__ col, const VT* __restrict__ val,
+                                                  const XT* __restrict__ x, XT* prod, int& r, int& r1)
 {
-    const int t = blockIdx.x;
     const int r0 = tiles[2 * t];
     r1 = tiles[2 * t + 1];
     const int k0 = rowptr[r0];
@@ -45,6 +44,25 @@ __device__ __forceinline__ XT tile_row_product(const int* __restrict__ tiles, co
         }
     }
     return sum;
+}
+
+// The same for the tile of this workgroup's own index (tile t = blockIdx.x).
+template <typename VT, typename XT>
+__device__ __forceinline__ XT tile_row_product(const int* __restrict__ tiles, const int* __restrict__ rowptr,
+                                               const int* __restrict__ col, const VT* __restrict__ val,
+                                               const XT* __restrict__ x, XT* prod, int& r, int& r1)
+{
+    return tile_row_product_at<VT, XT>((int)blockIdx.x, tiles, rowptr, col, val, x, prod, r, r1);
+}
+
+// Tile of this workgroup when a grid of gridDim.x workgroups covers tiles [0, gridDim.x): workgroup b runs
+// on XCD b mod 8, and XCD x takes the x-th contiguous band of tiles, so neighbouring row tiles (whose
+// gathers overlap) meet in one L2. A bijection for every grid size.
+__device__ __forceinline__ int xcd_band_tile()
+{
+    const int b = (int)blockIdx.x, g = (int)gridDim.x;
+    const int x = b & 7, q = g >> 3, rem = g & 7;
+    return x * q + min(x, rem) + (b >> 3);
 }
 
 }  // namespace pfem
