@@ -160,6 +160,36 @@ def roofline_spmv(ctx, st):
                 frac=round(gbs / HBM_PEAK_GBS, 4), avg_us=round(ms * 1e3, 2), bytes_per_launch=b)
 
 
+def copy_ceiling(ctx, gib=1, copies=10):
+    """The device's streaming ceiling: copies of `gib` GiB (4x the 256 MB last-level cache), each reading
+    and writing the buffer once -- the library's copy kernel (pfem_stream_copy: 16-byte loads, 4 or 8 in
+    flight per lane, the best of 14 launch forms, HIP events) and the runtime's device-to-device copy (wall
+    clock)."""
+    from gpupfem2_amd import capi
+    nbytes = gib << 30
+    ms, grid = capi.stream_copy(ctx, nbytes, copies)
+    kernel_gbs = 2 * nbytes / (ms * 1e-3) / 1e9
+    n = nbytes // 8
+    src, dst = ctx.zeros(n), ctx.empty(n)
+    dst.copy_from(src)
+    ctx.sync()
+    t = time.perf_counter()
+    for _ in range(copies):
+        dst.copy_from(src)
+    ctx.sync()
+    dt = (time.perf_counter() - t) / copies
+    memcpy_gbs = 2 * nbytes / dt / 1e9
+    src.free()
+    dst.free()
+    best = max(kernel_gbs, memcpy_gbs)
+    return dict(achieved=round(best, 1), frac_of_peak=round(best / HBM_PEAK_GBS, 4), unit="GB/s",
+                copy_kernel_gbs=round(kernel_gbs, 1), copy_kernel_loads_in_flight=grid // 100000,
+                copy_kernel_workgroups=grid % 100000, memcpy_d2d_gbs=round(memcpy_gbs, 1),
+                method=f"copies of {gib} GiB (read + write): pfem_stream_copy (HIP events, best of 14 launch forms, "
+                       f"{copies} launches each) and hipMemcpyAsync device-to-device (wall clock, {copies} copies); "
+                       f"the better of the two")
+
+
 def cpu_info():
     model = "unknown"
     try:
@@ -509,6 +539,10 @@ def main():
             # the GPU's own step from the same state, for the CPU step's C_D and C_L to be compared with
             cpu["gpu_cd_cl_same_step"] = [float(v) for v in st.step()]
         del st, ph, mesh
+    ceiling = copy_ceiling(ctx)
+    print(f"[bench] copy ceiling {ceiling['achieved']} GB/s", file=sys.stderr, flush=True)
+    for r in roof_pcg.values():
+        r["frac_of_copy_ceiling"] = round(r["achieved"] / ceiling["achieved"], 4)
     head = res.get(1, res.get(0))
     err = None
     if len(res) == 2:
@@ -554,6 +588,7 @@ def main():
         "roofline_pcg_iteration_fp64": roof_pcg.get("fp64"),
         "roofline_ic0_apply": roof_ic0,
         "roofline_spmv": roof_spmv,
+        "copy_ceiling": ceiling,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -151,6 +151,7 @@ SIGNATURES = {
     "pfem_pfem2_matrix": (_i, [_vp, _i, _pp]),
     "pfem_pfem2_precond": (_i, [_vp, _i, _pp]),
     "pfem_time_kernel": (_i, [_i, _vp, _vp, _i, C.POINTER(_d)]),
+    "pfem_stream_copy": (_i, [_vp, C.c_size_t, _i, C.POINTER(_d), C.POINTER(_i)]),
     "pfem_solver_solve_pair": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "pfem_solver_set_batch_tiles": (_i, [_vp, _i, _vp, _vp, _vp]),
     "pfem_pfem2_sweep_fallbacks": (_i, [_vp]),
@@ -725,6 +726,13 @@ def time_kernel(ctx, op, csr_handle, precond_handle=None, launches=50):
     ms = C.c_double()
     ctx.check(lib().pfem_time_kernel(op, csr_handle, precond_handle, launches, C.byref(ms)))
     return ms.value
+
+
+def stream_copy(ctx, nbytes, launches=10):
+    """(average ms of one copy of nbytes, best grid): the streaming ceiling of pfem_stream_copy"""
+    ms, g = C.c_double(), C.c_int()
+    ctx.check(lib().pfem_stream_copy(ctx.h, int(nbytes), launches, C.byref(ms), C.byref(g)))
+    return ms.value, g.value
 
 
 def export_vtk(mesh, path, fields):

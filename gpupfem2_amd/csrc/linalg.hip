@@ -189,5 +189,71 @@ void dirichlet_apply(Context* c, pfem_dirichlet_t* bc0, pfem_dirichlet_t* bc1, p
     PFEM_HIP(hipGetLastError());
 }
 
+// ------------------------------------------------------------------ streaming ceiling
+// d = s over n2 16-byte elements, grid-stride with U independent 16-byte loads in flight per lane
+template <int U>
+__global__ __launch_bounds__(kBlock) void k_stream_copy(long n2, const double2* __restrict__ s, double2* __restrict__ d)
+{
+    const long stride = (long)gridDim.x * blockDim.x;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + (U - 1) * stride < n2; i += U * stride) {
+        double2 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = s[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < U; ++u) d[i + u * stride] = v[u];
+    }
+    for (; i < n2; i += stride) d[i] = s[i];
+}
 
 }  // namespace pfem
+
+using namespace pfem;
+
+extern "C" {
+
+int pfem_stream_copy(pfem_ctx ctx, size_t bytes, int launches, double* avg_ms, int* best_grid)
+{
+    PFEM_TRY(ctx, {
+        PFEM_REQUIRE(avg_ms && launches > 0 && bytes >= 16 && bytes % 16 == 0, "bad arguments");
+        Context* c = C(ctx);
+        const long n2 = (long)(bytes / 16);
+        DevArray<double2> s(n2), d(n2);
+        zero_async(c, s.p, s.bytes());
+        hipEvent_t e0, e1;
+        PFEM_HIP(hipEventCreate(&e0));
+        PFEM_HIP(hipEventCreate(&e1));
+        double best = 1e30;
+        int bg = 0;
+        // workgroups per launch: 1 to 64 per CU of the 256; 4 or 8 loads in flight per lane
+        for (int u : {4, 8})
+            for (int g : {256, 512, 1024, 2048, 4096, 8192, 16384}) {
+                auto launch = [&]() {
+                    if (u == 4)
+                        hipLaunchKernelGGL(k_stream_copy<4>, dim3(g), dim3(kBlock), 0, c->stream, n2,
+                                           (const double2*)s.p, d.p);
+                    else
+                        hipLaunchKernelGGL(k_stream_copy<8>, dim3(g), dim3(kBlock), 0, c->stream, n2,
+                                           (const double2*)s.p, d.p);
+                };
+                launch();
+                PFEM_HIP(hipEventRecord(e0, c->stream));
+                for (int k = 0; k < launches; ++k) launch();
+                PFEM_HIP(hipEventRecord(e1, c->stream));
+                PFEM_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                PFEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (ms / launches < best) {
+                    best = ms / launches;
+                    bg = u * 100000 + g;
+                }
+            }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        PFEM_HIP(hipGetLastError());
+        *avg_ms = best;
+        if (best_grid) *best_grid = bg;
+    });
+}
+
+}  // extern "C"

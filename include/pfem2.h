@@ -431,6 +431,13 @@ int pfem_pfem2_precond(pfem_pfem2 s, int which, pfem_precond* out);
  * as pfem_solver_solve runs it with pc (op 10 FP64 factor, op 11 its PFEM_PRECISION_MIXED_PRECOND copy),
  * the device time of a solve of 2 launches iterations minus one of `launches` iterations, over launches. */
 int pfem_time_kernel(int op, pfem_csr a, pfem_precond pc, int launches, double* avg_ms);
+/* The device's streaming ceiling (no counterpart in the reference; the roofline's measured reference
+ * point): a copy kernel (16-byte loads, 4 or 8 in flight per lane, grid-stride) over two buffers of `bytes`
+ * bytes, timed with HIP events over `launches` launches for each of the 14 forms (seven grid sizes, 256 to
+ * 16,384 workgroups of 256 lanes, times the two load depths); returns the best average time of one copy
+ * (it reads and writes `bytes` each) and, if best_grid is not NULL, the form that gave it as
+ * depth * 100000 + workgroups. bytes: a positive multiple of 16. */
+int pfem_stream_copy(pfem_ctx ctx, size_t bytes, int launches, double* avg_ms, int* best_grid);
 
 /* ------------------------------------------------------------------ export (src/data_export.cuh) */
 /* DataExport::exportToVTK (src/data_export.cu:17-95): ASCII VTU with nodal scalars (device arrays) */

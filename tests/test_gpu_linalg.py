@@ -770,3 +770,17 @@ def test_pcg_full_size_repeatable(ctx, bits):
             assert np.linalg.norm(b - y.download()) / np.linalg.norm(b) < 1e-7
         else:
             assert got[0] == ref[0] and np.array_equal(got[1], ref[1]), rep
+
+
+def test_stream_copy_ceiling(ctx):
+    """pfem_stream_copy (the roofline's measured streaming ceiling) times its copy kernel over seven grid
+    sizes and two load depths and reports the best; bad sizes are rejected."""
+    from gpupfem2_amd import capi
+    ms, grid = capi.stream_copy(ctx, 64 << 20, 2)
+    assert ms > 0.0 and grid // 100000 in (4, 8) and grid % 100000 in (256, 512, 1024, 2048, 4096, 8192, 16384)
+    gbs = 2 * (64 << 20) / (ms * 1e-3) / 1e9
+    print("stream copy 64 MiB: %.1f GB/s, grid %d" % (gbs, grid), flush=True)
+    assert 100.0 < gbs < 20000.0
+    for bad in (0, 24):
+        with pytest.raises(capi.PfemError):
+            capi.stream_copy(ctx, bad, 1)

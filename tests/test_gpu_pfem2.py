@@ -223,3 +223,36 @@ def test_step_graphs_bitexact(ctx, ci, precision):
         for k in a[3]:
             assert np.array_equal(a[3][k], b[3][k]), (s, k)
         assert a[4] == b[4] and a[5] == b[5], s
+
+
+def test_step_graphs_after_growth(ctx):
+    """The step graphs are captured again when the particle buffers are reallocated: after two steps (the
+    graphs captured), every particle is uploaded three times over (each copy right after the particle, so
+    the set stays ordered by cell), which grows the buffers (their capacity is kept at the count plus one
+    particle per subcell of every cell); the graph steps that follow still give the eager steps' bits."""
+    from gpupfem2_amd import cases
+    case = small_cases()[1]
+    print("step graphs after growth", case.name, flush=True)
+    runs = []
+    for graphs in (False, True):
+        mesh, ph, stepper = cases.build_gpu(ctx, case)
+        if graphs:
+            stepper.set_graphs(True)
+        out = []
+        for s in range(5):
+            if s == 3:
+                st = ph.get()
+                dup = {k: np.repeat(v, 3, axis=0) for k, v in st.items()}
+                ph.set(dup, int(st["id"].max()) + 1)
+                assert ph.count() == 3 * st["cell"].size
+            f = stepper.step()
+            out.append((f.copy(), stepper.fields(), ph.get(), ph.count()))
+        runs.append(out)
+        del stepper, ph, mesh
+    for s, (a, b) in enumerate(zip(*runs)):
+        assert np.array_equal(a[0], b[0]), (s, a[0], b[0])
+        for k in a[1]:
+            assert np.array_equal(a[1][k], b[1][k]), (s, k)
+        for k in a[2]:
+            assert np.array_equal(a[2][k], b[2][k]), (s, k)
+        assert a[3] == b[3], s
