@@ -176,8 +176,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_alpha_sliced(const int* __re
         if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) st->xp_seen = st->it;
         return;
     }
-    const int* h = hdr + 4 * blockIdx.x;
-    const ChunkBases cb = col16 ? chunk_bases(cbase + 4 * blockIdx.x) : ChunkBases{0, 0, 0, 0, true};
+    const int* h = hdr + 4 * hdr_slot();
+    const ChunkBases cb = col16 ? chunk_bases(cbase + 4 * hdr_slot()) : ChunkBases{0, 0, 0, 0, true};
     const int cnt = h[1], width = h[3];
     double v[1] = {0.0};
     if ((int)threadIdx.x < cnt) {

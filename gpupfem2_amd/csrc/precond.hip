@@ -231,7 +231,7 @@ __device__ __forceinline__ bool sweep_rows(const SweepArgs<TV, TS, TF>& a, int (
 {
     bool any = false;
     if constexpr (LAYOUT == kSliced) {
-        const int* h = a.chunks + 4 * blockIdx.x;
+        const int* h = a.chunks + 4 * hdr_slot();
         const int cnt = h[1];
         stride = cnt;
         width = h[3];
@@ -267,7 +267,7 @@ __device__ __forceinline__ void sweep_sums(const SweepArgs<TV, TS, TF>& a, TF (&
         using U = typename Sentinel<TF>::U;
         constexpr U sent = Sentinel<TF>::bits;
         constexpr int K = kDeps / R;
-        const ChunkBases cb = a.col16 ? chunk_bases(a.cbase + 4 * blockIdx.x) : ChunkBases{0, 0, 0, 0, true};
+        const ChunkBases cb = a.col16 ? chunk_bases(a.cbase + 4 * hdr_slot()) : ChunkBases{0, 0, 0, 0, true};
         for (int w0 = 0; w0 < width; w0 += K) {
             int cv[R][K];
             TF fv[R][K];
@@ -975,6 +975,20 @@ static void build_factor_structure(Context* c, pfem_precond_t* pc)
                 col.push_back(-1);
                 pos.push_back(-1);
                 c16.push_back(0xFFFF);
+            }
+            {
+                // chunk q's entries at hdr_slot's position for workgroup q (grid = number of chunks)
+                const int G = (int)(hdr.size() / 4), q8 = G >> 3, rem = G & 7;
+                std::vector<int> h2(hdr.size()), c2(cbase.size());
+                for (int b = 0; b < G; ++b) {
+                    const int x = b & 7, sl = x * q8 + std::min(x, rem) + (b >> 3);
+                    for (int k = 0; k < 4; ++k) {
+                        h2[4 * sl + k] = hdr[4 * b + k];
+                        c2[4 * sl + k] = cbase[4 * b + k];
+                    }
+                }
+                hdr.swap(h2);
+                cbase.swap(c2);
             }
             upload(c, out.hdr, hdr);
             upload(c, out.cbase, cbase);

@@ -24,6 +24,18 @@ struct ChunkBases {
     bool wide;
 };
 
+// Position of this workgroup's entry in a sliced layout's header and base tables. Workgroup b runs on XCD
+// b mod 8; the tables hold the entries of XCD x's workgroups as the x-th contiguous band (the order of
+// xcd_band_tile), so the workgroups that one XCD runs one after the other read neighbouring entries, four to
+// a cache line of its L2, rather than one line each. A bijection for every grid size; the tables are
+// stored in this order by the layout builder.
+__device__ __forceinline__ int hdr_slot()
+{
+    const int b = (int)blockIdx.x, g = (int)gridDim.x;
+    const int x = b & 7, q = g >> 3, rem = g & 7;
+    return x * q + min(x, rem) + (b >> 3);
+}
+
 __device__ __forceinline__ ChunkBases chunk_bases(const int* __restrict__ cb)
 {
     return ChunkBases{cb[0], cb[1], cb[2], cb[3], cb[0] < 0};
