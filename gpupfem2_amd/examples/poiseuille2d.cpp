@@ -89,6 +89,8 @@ int main(int argc, char* argv[])
         gpupfem2::check(pfem_pfem2_create(mesh.handle(), particleHandler.handle(), &cfg, (int)bcu.size(), bcu.data(),
                                           bcu0.data(), bcu1.data(), (int)bcp.size(), bcp.data(), bcpv.data(),
                                           &stepper));
+        // the work between the solves replays from HIP graphs from the second step on (the same bits)
+        gpupfem2::check(pfem_pfem2_set_graphs(stepper, 1));
         double* field[5];
         for (int k = 0; k < 5; ++k) gpupfem2::check(pfem_pfem2_field(stepper, k, &field[k]));
         gpupfem2::check(pfem_particles_init_velocity(particleHandler.handle(), field[0], field[1]));

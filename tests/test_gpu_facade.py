@@ -67,7 +67,8 @@ def test_poisson2d_driver_matches_oracle(tmp_path, oracle):
 def test_cylinder2d_driver(ctx, tmp_path, oracle):
     """examples/cylinder2d.cpp, the Cylinder2D case driver (CylinderMesh3.dat, Scheme A, dt 0.001, level-2
     particles, Jacobi CG and non-restarted GMRES at the reference's absolute tolerance 1e-8), run for 4
-    steps: Forces.csv in the reference's "Time;Cx;Cy" format, VTU snapshots, and the loads of every step
+    steps with the step graphs on (captured at steps 2 and 3, one per particle buffer, replayed at step 4):
+    Forces.csv in the reference's "Time;Cx;Cy" format, VTU snapshots, and the loads of every step
     equal to those of the same case built in Python (edge IDs and Dirichlet lists from the reference's
     rules in gpupfem2_amd.cases) and stepped through the Python binding, and to the oracle's loads."""
     from gpupfem2_amd import cases
