@@ -152,6 +152,7 @@ SIGNATURES = {
     "pfem_pfem2_precond": (_i, [_vp, _i, _pp]),
     "pfem_time_kernel": (_i, [_i, _vp, _vp, _i, C.POINTER(_d)]),
     "pfem_stream_copy": (_i, [_vp, C.c_size_t, _i, C.POINTER(_d), C.POINTER(_i)]),
+    "pfem_mesh_locality_order": (_i, [_i, _i, _i, _vp, _vp, _vp]),
     "pfem_solver_solve_pair": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "pfem_solver_set_batch_tiles": (_i, [_vp, _i, _vp, _vp, _vp]),
     "pfem_pfem2_sweep_fallbacks": (_i, [_vp]),
@@ -726,6 +727,27 @@ def time_kernel(ctx, op, csr_handle, precond_handle=None, launches=50):
     ms = C.c_double()
     ctx.check(lib().pfem_time_kernel(op, csr_handle, precond_handle, launches, C.byref(ms)))
     return ms.value
+
+
+def locality_order(nv, cells):
+    """(node_order, cell_order) for memory locality (pfem_mesh_locality_order, host only): node_order[new] =
+    old in reverse Cuthill-McKee order, cell_order[new] = old by the cells' smallest new vertex index."""
+    c = np.ascontiguousarray(cells, np.int32)
+    nodes = np.empty(nv, np.int32)
+    order = np.empty(max(c.shape[0], 1), np.int32)
+    rc = lib().pfem_mesh_locality_order(int(nv), int(c.shape[0]), int(c.shape[1]), _ptr(c), _ptr(nodes), _ptr(order))
+    if rc != 0:
+        raise PfemError(rc, "pfem_mesh_locality_order failed")
+    return nodes, order[:c.shape[0]]
+
+
+def renumber_mesh(xy, cells):
+    """The mesh in locality order: (xy, cells, node_order, cell_order) with xy_new = xy[node_order] and the
+    cells' vertices mapped to the new node numbers, the cells listed in cell_order."""
+    nodes, order = locality_order(xy.shape[0], cells)
+    new_id = np.empty_like(nodes)
+    new_id[nodes] = np.arange(nodes.size, dtype=nodes.dtype)
+    return xy[nodes], new_id[np.asarray(cells)[order]].astype(np.int32), nodes, order
 
 
 def stream_copy(ctx, nbytes, launches=10):

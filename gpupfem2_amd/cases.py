@@ -126,11 +126,12 @@ def poiseuille_case(nx=400, ny=100, jitter=0.2, mesh_seed=2, ppc=16, particle_se
 
 
 def cylinder_case(h=1.0 / 112, ppc=20, particle_seed=103, dt=2e-4, steps=500, tol=1e-12, tol_mode=REL,
-                  pres_precond=IC0, shuffle_seed=None):
+                  pres_precond=IC0, shuffle_seed=None, locality=False):
     """Flow past a cylinder (D = 1, centre (4, 2.1)) in the channel [0,20]x[0,4]; U_mean = 1 (U_max = 1.5),
     mu = 0.01 (Re = 100), scheme B, two outer iterations. shuffle_seed: number the nodes and order the
     cells at random (seeded) instead of the generator's structured order -- the same mesh with the
-    numbering of an unordered unstructured mesh."""
+    numbering of an unordered unstructured mesh. locality: then renumber it with the library's locality
+    order (pfem_mesh_locality_order: nodes in reverse Cuthill-McKee order, cells by their first node)."""
     xy, tri, info = meshgen.cylinder_channel(h=h)
     if shuffle_seed is not None:
         rng = np.random.default_rng(shuffle_seed)
@@ -138,6 +139,9 @@ def cylinder_case(h=1.0 / 112, ppc=20, particle_seed=103, dt=2e-4, steps=500, to
         xy2 = np.empty_like(xy)
         xy2[new_id] = xy
         xy, tri = xy2, new_id[tri][rng.permutation(tri.shape[0])].astype(np.int32)
+    if locality:
+        from . import capi
+        xy, tri, _, _ = capi.renumber_mesh(xy, tri)
     L, H, R = info["L"], info["H"], info["R"]
     cx, cy = info["center"]
     cls = dict(x0=0.0, x1=L, y0=0.0, y1=H, ids=(0, 1, 2, 2), circle=(cx, cy, R), circle_tol=1e-3, circle_id=3)

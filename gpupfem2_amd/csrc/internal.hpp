@@ -82,6 +82,8 @@ struct Pattern {
 };
 
 std::shared_ptr<Pattern> make_pattern(Context* c, int n, std::vector<int>&& rowptr, std::vector<int>&& col);
+// Reverse Cuthill-McKee order of a pattern's graph (host arrays only): order[new] = old.
+std::vector<int> rcm_order(const Pattern& A);
 
 }  // namespace pfem
 

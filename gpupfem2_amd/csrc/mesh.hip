@@ -1,5 +1,6 @@
 // Context, device memory, mesh import and O(T) setup builders, geometry kernels.
 #include <algorithm>
+#include <array>
 #include <cctype>
 #include <cmath>
 #include <cstdio>
@@ -504,6 +505,61 @@ int pfem_memcpy_d2d(pfem_ctx ctx, void* d_dst, const void* d_src, size_t bytes)
 int pfem_memset_zero(pfem_ctx ctx, void* d_ptr, size_t bytes)
 {
     PFEM_TRY(ctx, zero_async(C(ctx), d_ptr, bytes));
+}
+
+// Node and cell order for locality (host only): the nodes in reverse Cuthill-McKee order of the vertex
+// adjacency graph, the cells by the smallest new index among their vertices (ties: the next smallest,
+// then the old index), so that neighbouring rows, cells and the particles sorted by cell lie close in
+// memory whatever the input numbering.
+int pfem_mesh_locality_order(int nv, int nt, int nvc, const int* cells, int* node_order, int* cell_order)
+{
+    PFEM_TRY(nullptr, {
+        PFEM_REQUIRE(nv > 0 && nt >= 0 && (nvc == 3 || nvc == 4) && (cells || nt == 0) && node_order &&
+                         (cell_order || nt == 0),
+                     "bad arguments");
+        std::vector<std::vector<int>> adj(nv);
+        for (long c = 0; c < nt; ++c)
+            for (int a = 0; a < nvc; ++a) {
+                const int u = cells[c * nvc + a];
+                PFEM_REQUIRE(u >= 0 && u < nv, "cell vertex out of range");
+                for (int b = 0; b < nvc; ++b) adj[u].push_back(cells[c * nvc + b]);
+            }
+        Pattern P;
+        P.n = nv;
+        P.h_rowptr.assign(nv + 1, 0);
+        for (int u = 0; u < nv; ++u) {
+            auto& a = adj[u];
+            a.push_back(u);
+            std::sort(a.begin(), a.end());
+            a.erase(std::unique(a.begin(), a.end()), a.end());
+            P.h_rowptr[u + 1] = P.h_rowptr[u] + (int)a.size();
+        }
+        P.h_col.reserve(P.h_rowptr[nv]);
+        for (auto& a : adj) {
+            P.h_col.insert(P.h_col.end(), a.begin(), a.end());
+            std::vector<int>().swap(a);
+        }
+        P.nnz = P.h_rowptr[nv];
+        const std::vector<int> order = rcm_order(P);
+        std::vector<int> pos(nv);
+        for (int k = 0; k < nv; ++k) {
+            node_order[k] = order[k];
+            pos[order[k]] = k;
+        }
+        if (nt > 0) {
+            std::vector<std::array<int, 4>> key(nt);
+            for (long c = 0; c < nt; ++c) {
+                std::array<int, 4> k{INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX};
+                for (int a = 0; a < nvc; ++a) k[a] = pos[cells[c * nvc + a]];
+                std::sort(k.begin(), k.begin() + nvc);
+                key[c] = k;
+            }
+            std::vector<int> idx(nt);
+            for (int c = 0; c < nt; ++c) idx[c] = c;
+            std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return key[x] < key[y]; });
+            for (int c = 0; c < nt; ++c) cell_order[c] = idx[c];
+        }
+    });
 }
 
 int pfem_mesh2d_load_dat(pfem_ctx ctx, const char* path, int fill_neighbors, double scale, pfem_mesh2d* out)

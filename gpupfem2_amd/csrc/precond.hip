@@ -589,7 +589,7 @@ __global__ void k_f16_lost_diag(int n, const _Float16* __restrict__ d, int* lost
 // Reverse Cuthill-McKee order of the matrix graph: breadth-first from a pseudo-peripheral node of each
 // connected component (the last node reached by a BFS from the component's minimum-degree node),
 // neighbours visited by ascending degree (ties by index), the whole sequence reversed.
-static std::vector<int> rcm_order(const Pattern& A)
+std::vector<int> rcm_order(const Pattern& A)
 {
     const int n = A.n;
     std::vector<int> deg(n);

@@ -54,6 +54,14 @@ int pfem_memcpy_d2h(pfem_ctx ctx, void* dst, const void* d_src, size_t bytes);
 int pfem_memcpy_d2d(pfem_ctx ctx, void* d_dst, const void* d_src, size_t bytes);
 int pfem_memset_zero(pfem_ctx ctx, void* d_ptr, size_t bytes);
 
+/* Node and cell order for memory locality, host only (no counterpart in the reference, which keeps the
+ * DAT file's numbering, src/mesh_2d.cu:36-96 and src/mesh_3d.cu:84-147): for nt cells of nvc = 3
+ * (triangles) or 4 (tetrahedra) vertices, node_order[new] = old in reverse Cuthill-McKee order of the
+ * vertex adjacency graph, and cell_order[new] = old with the cells sorted by the smallest new index of
+ * their vertices (ties: the next smallest, then the old index). A mesh renumbered by these orders gives
+ * the same solutions, permuted; its rows, cells and cell-sorted particles lie close in memory. */
+int pfem_mesh_locality_order(int nv, int nt, int nvc, const int* cells, int* node_order, int* cell_order);
+
 /* ------------------------------------------------------------------ 2D mesh (src/mesh_2d.cuh:11-69) */
 /* Mesh2D::loadMeshFromFile(filename, fillNeighborLists, scale) (src/mesh_2d.cu:36-96), including the
  * reader's quirk that a trailing newline drops the last cell from the device mesh. */
