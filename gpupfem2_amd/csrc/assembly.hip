@@ -615,8 +615,9 @@ void prediction_bc_values(pfem_mesh2d m, pfem_dirichlet target, pfem_dirichlet s
     PFEM_HIP(hipGetLastError());
 }
 
-void body_forces(pfem_mesh2d m, int body_id, double cx, double cy, const double* u0, const double* u1,
-                 const double* p, const pfem_pfem2_params* prm, double* d_out /* 4 doubles, device */)
+// The body's (cell, edge) list, found once per body ID (host work and an upload: never inside a captured
+// launch sequence)
+void body_forces_prepare(pfem_mesh2d m, int body_id)
 {
     Context* c = C(m->ctx);
     if (m->body_id_cached != body_id) {
@@ -629,7 +630,15 @@ void body_forces(pfem_mesh2d m, int body_id, double cx, double cy, const double*
         upload(c, m->body_edges, ce);
         m->body_loads.alloc(4 * (size_t)std::max(m->n_body, 1) + 4);
         m->body_id_cached = body_id;
+        ++m->generation;
     }
+}
+
+void body_forces(pfem_mesh2d m, int body_id, double cx, double cy, const double* u0, const double* u1,
+                 const double* p, const pfem_pfem2_params* prm, double* d_out /* 4 doubles, device */)
+{
+    Context* c = C(m->ctx);
+    body_forces_prepare(m, body_id);
     const Quad1 qe = edge_rule(prm->edge_quad);
     if (m->n_body > 0)
         hipLaunchKernelGGL(k_body_edge_loads, dim3(blocks_for(m->n_body)), dim3(kBlock), 0, c->stream, m->n_body,

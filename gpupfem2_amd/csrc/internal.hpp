@@ -137,6 +137,9 @@ struct pfem_mesh2d_t {
     pfem::DevArray<int> body_edges;
     int n_body = 0;
     pfem::DevArray<double> body_loads;
+    // counts reallocations of the device arrays above after creation (edge IDs, the body cache), so that
+    // captured launch sequences that hold their addresses know to be captured again
+    long generation = 0;
 };
 
 struct pfem_mesh3d_t {

@@ -646,6 +646,7 @@ int pfem_mesh2d_set_edge_ids(pfem_mesh2d m, const int* ids)
         m->h_edge_ids.assign(ids, ids + 3 * (size_t)m->nt);
         upload(C(m->ctx), m->edge_ids, m->h_edge_ids);
         m->body_id_cached = -1000;
+        ++m->generation;
     });
 }
 
@@ -677,6 +678,7 @@ int pfem_mesh2d_classify_edges(pfem_mesh2d m, double x0, double x1, double y0, d
         m->h_edge_ids = ids;
         upload(C(m->ctx), m->edge_ids, m->h_edge_ids);
         m->body_id_cached = -1000;
+        ++m->generation;
     });
 }
 

@@ -36,6 +36,7 @@ void assemble_correction(pfem_mesh2d m, pfem_csr a, double* rhs0, double* rhs1, 
                          const double* p, const double* p_old, const pfem_pfem2_params* prm);
 void prediction_bc_values(pfem_mesh2d m, pfem_dirichlet target, pfem_dirichlet source, const double* p,
                           const double* p_old, int component, const pfem_pfem2_params* prm);
+void body_forces_prepare(pfem_mesh2d m, int body_id);
 void body_forces(pfem_mesh2d m, int body_id, double cx, double cy, const double* u0, const double* u1,
                  const double* p, const pfem_pfem2_params* prm, double* d_out);
 }  // namespace pfem
@@ -118,6 +119,13 @@ struct Phase {
         s->phase_ms[k] += ms;
     }
 };
+
+// The addresses a step's launches use change only when the particle buffers or the mesh's edge IDs or
+// body cache are reallocated: a graph captured under another key is captured again.
+long graph_key(const pfem_pfem2_t* s)
+{
+    return particles_generation(s->ph) * 1000003L + s->m->generation;
+}
 
 // Runs body (launches only) directly, or from segment k's graph (variant v, captured for key on first use
 // and whenever the key changes) when graphs are on and a first step has made the lazy set-ups.
@@ -281,8 +289,7 @@ int pfem_pfem2_step(pfem_pfem2 s, double forces[2])
             Phase ph(s, 0);
             particles_advect_prepare(s->ph);
             const int b = particles_current(s->ph);
-            const long key = particles_generation(s->ph);
-            run_segment(s, 0, b, key, [&] {
+            run_segment(s, 0, b, graph_key(s), [&] {
                 particles_advect_launch(s->ph, b, s->field(U0), s->field(U1), prm->dt, cfg.substeps);
             });
             particles_set_current(s->ph, 1 - b);
@@ -290,7 +297,7 @@ int pfem_pfem2_step(pfem_pfem2 s, double forces[2])
         }
         {
             Phase ph(s, 1);
-            run_segment(s, 1, particles_current(s->ph), particles_generation(s->ph), [&] {
+            run_segment(s, 1, particles_current(s->ph), graph_key(s), [&] {
                 particles_project(s->ph, s->field(U0), s->field(U1));
                 launch_copy(c, s->field(UO0), s->field(U0), n);
                 launch_copy(c, s->field(UO1), s->field(U1), n);
@@ -300,7 +307,7 @@ int pfem_pfem2_step(pfem_pfem2 s, double forces[2])
         for (int outer = 0; outer < cfg.outer_iterations; ++outer) {
             {
                 Phase ph(s, 2);
-                run_segment(s, 2, 0, 0, [&] {
+                run_segment(s, 2, 0, graph_key(s), [&] {
                     assemble_prediction(s->m, s->A0, s->A1, s->field(RHS0), s->field(RHS1), s->field(U0),
                                         s->field(U1), s->field(UO0), s->field(UO1), pold, prm);
                     for (int k = 0; k < 2; ++k) {
@@ -326,7 +333,7 @@ int pfem_pfem2_step(pfem_pfem2 s, double forces[2])
             }
             {
                 Phase ph(s, 2);
-                run_segment(s, 3, 0, 0, [&] {
+                run_segment(s, 3, 0, graph_key(s), [&] {
                     assemble_pressure(s->m, s->Ap, s->field(RHS0), s->field(VP0), s->field(VP1), pold, prm);
                     dirichlet_apply(c, s->bcp, nullptr, s->Ap, s->field(RHS0), nullptr, cfg.dirichlet_mode);
                 });
@@ -337,7 +344,7 @@ int pfem_pfem2_step(pfem_pfem2 s, double forces[2])
             }
             {
                 Phase ph(s, 2);
-                run_segment(s, 4, 0, 0, [&] {
+                run_segment(s, 4, 0, graph_key(s), [&] {
                     assemble_correction(s->m, s->Ac, s->field(RHS0), s->field(RHS1), s->field(VP0), s->field(VP1),
                                         s->field(P), pold, prm);
                     dirichlet_apply(c, s->bcu[0], s->bcu[1], s->Ac, s->field(RHS0), s->field(RHS1),
@@ -366,7 +373,8 @@ int pfem_pfem2_step(pfem_pfem2 s, double forces[2])
         {
             Phase ph(s, 6);
             double* fd = s->f.p + (size_t)NF * n;
-            run_segment(s, 5, particles_current(s->ph), particles_generation(s->ph), [&] {
+            if (cfg.body_id >= 0) body_forces_prepare(s->m, cfg.body_id);   // outside any capture
+            run_segment(s, 5, particles_current(s->ph), graph_key(s), [&] {
                 particles_correct(s->ph, s->field(U0), s->field(U1), s->field(UO0), s->field(UO1));
                 if (cfg.body_id >= 0)
                     body_forces(s->m, cfg.body_id, cfg.body_cx, cfg.body_cy, s->field(U0), s->field(U1), s->field(P),

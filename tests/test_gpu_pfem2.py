@@ -256,3 +256,28 @@ def test_step_graphs_after_growth(ctx):
         for k in a[2]:
             assert np.array_equal(a[2][k], b[2][k]), (s, k)
         assert a[3] == b[3], s
+
+
+def test_step_graphs_after_mesh_update(ctx):
+    """The step graphs hold the mesh's edge-ID and body-cache addresses; setting the edge IDs again (which
+    reallocates them and drops the body cache) between steps makes the next graph step capture afresh and
+    rebuild the body cache outside the capture, and the steps keep the eager steps' bits."""
+    from gpupfem2_amd import cases
+    case = small_cases()[1]
+    print("step graphs after mesh update", case.name, flush=True)
+    runs = []
+    for graphs in (False, True):
+        mesh, ph, stepper = cases.build_gpu(ctx, case)
+        if graphs:
+            stepper.set_graphs(True)
+        out = []
+        for s in range(5):
+            if s == 3:
+                mesh.set_edge_ids(case.edge_ids)
+            out.append((stepper.step().copy(), stepper.fields()))
+        runs.append(out)
+        del stepper, ph, mesh
+    for s, (a, b) in enumerate(zip(*runs)):
+        assert np.array_equal(a[0], b[0]), (s, a[0], b[0])
+        for k in a[1]:
+            assert np.array_equal(a[1][k], b[1][k]), (s, k)
