@@ -5,6 +5,7 @@ PCG + Jacobi, relative tolerance 1e-12), in the mixed-precision pipeline and in 
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
     python bench.py --kernels      # only the pressure-solve kernels (for counter collection)
+    python bench.py --blocks       # multicolour against blocked IC0 ordering on the pressure system
     python bench.py --workload poisson3d   # 8-RHS 3D Poisson solves (secondary workload)
     python bench.py --workload batched     # 256 independent Poisson systems (secondary workload)
     python bench.py --workload history --steps 500   # C_D / C_L histories, FP64 and mixed
@@ -310,6 +311,50 @@ def kernels_only(h, launches):
     print(json.dumps(res))
 
 
+def blocks_only(h, launches):
+    """The pressure system of kernels_only with the two orderings of its IC0 factor, 16-bit copy, FP64
+    vectors: multicolour (the step's) and blocked (PFEM_ORDER_BLOCKS: blocks of 8 nodes coloured as
+    blocks, fewer PCG iterations, longer sweeps). Per ordering: `launches` tagged applies, the fused PCG
+    iteration (pfem_time_kernel op 11) and one solve to relative 1e-10 from zero on a seeded right-hand
+    side (iterations and wall time). Prints one JSON line."""
+    import numpy as np
+    from gpupfem2_amd import capi, cases
+    case = cases.cylinder_case(h=h)
+    ctx = capi.Context(0)
+    m = capi.Mesh2D(ctx, case.xy, case.tri, fill_neighbors=False)
+    a = capi.CSR(ctx, mesh=m)
+    n = case.xy.shape[0]
+    rhs = ctx.zeros(n)
+    capi.assemble_poisson2d(m, a, rhs, coeff=1.0, rhs_kind=2, rhs_param=0.0)
+    capi.Dirichlet(ctx, n, case.bcp_node, case.bcp_val).apply(a, rhs, capi.DIRICHLET_MASK)
+    b = ctx.to_device(np.random.default_rng(1).standard_normal(n))
+    res = {"n": n, "nnz": a.nnz}
+    for name, ordering in (("multicolour", capi.ORDER_MULTICOLOR), ("blocks", capi.ORDER_BLOCKS)):
+        t = time.perf_counter()
+        pc = capi.Preconditioner(a, capi.PRECOND_IC0, ordering)
+        pc.set_factor_bits(16)
+        pc.initialize(a)
+        ctx.sync()
+        r = {"setup_s": round(time.perf_counter() - t, 2)}
+        r["apply_us"] = round(capi.time_kernel(ctx, 8, a.h, pc.h, launches=launches) * 1e3, 2)
+        r["pcg_iteration_us"] = round(capi.time_kernel(ctx, 11, a.h, pc.h, launches=launches) * 1e3, 2)
+        s = capi.Solver(a, capi.SOLVER_CG, 1e-10, 20000, 0, capi.TOL_RELATIVE, pc)
+        s.set_precision(capi.PRECISION_MIXED_PRECOND, 1e-5, 0)
+        for rep in range(2):   # the first solve also captures nothing and allocates the solver's buffers
+            x = ctx.zeros(n)
+            ctx.sync()
+            t = time.perf_counter()
+            st = s.solve(a, x, b)
+            ctx.sync()
+            dt = time.perf_counter() - t
+        r["solve_iterations"] = st.iterations
+        r["solve_ms"] = round(dt * 1e3, 1)
+        res[name] = r
+        print(f"[blocks] {name}: {r}", file=sys.stderr, flush=True)
+        del s, pc
+    print(json.dumps(res))
+
+
 def multi_iteration_bytes(n, nnz, g, vec_bytes, val_bytes):
     """Algorithmic HBM bytes of one multi-RHS PCG iteration (Jacobi, g interleaved columns): the SpMV
     streams the matrix once (values, column indices, row offsets) and reads p, writes q; the update
@@ -489,6 +534,7 @@ def main():
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch the work between the solves kernel by kernel instead of from HIP graphs")
     ap.add_argument("--kernels", action="store_true", help="pressure-solve kernels only (counter runs)")
+    ap.add_argument("--blocks", action="store_true", help="multicolour against blocked IC0 on the pressure system")
     ap.add_argument("--launches", type=int, default=20)
     ap.add_argument("--workload", choices=["cylinder", "poisson3d", "batched", "history"], default="cylinder",
                     help="cylinder: the headline PFEM-2 step; poisson3d: 8-RHS 3D Poisson solves; "
@@ -510,6 +556,9 @@ def main():
         return
     if args.kernels:
         kernels_only(args.h, args.launches)
+        return
+    if args.blocks:
+        blocks_only(args.h, args.launches)
         return
     from gpupfem2_amd import capi, cases
     case = cases.cylinder_case(h=args.h)

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libpfem2_hip.so")
 PFEM_OK = 0
 DIRICHLET_PARITY, DIRICHLET_MASK = 0, 1
 PRECOND_NONE, PRECOND_JACOBI, PRECOND_IC0, PRECOND_ILU0 = 0, 1, 2, 3
-ORDER_NATURAL, ORDER_MULTICOLOR = 0, 1
+ORDER_NATURAL, ORDER_MULTICOLOR, ORDER_BLOCKS = 0, 1, 2
 SOLVER_CG, SOLVER_GMRES = 0, 1
 TOL_ABSOLUTE, TOL_RELATIVE = 0, 1
 PRECISION_FP64, PRECISION_MIXED, PRECISION_MIXED_PRECOND = 0, 1, 2

@@ -649,7 +649,9 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
     const unsigned g = vec_grid(n);
     const bool jac = A.kind == PFEM_PRECOND_JACOBI, none = A.kind == PFEM_PRECOND_NONE;
     // x/r update fused into the forward sweep: multicolour IC0/ILU0, single-pass sweeps, factor ordering
-    const bool fused = A.factor_order && A.pc && A.pc->sweep && A.pc->contiguous &&
+    // (a blocked factor has the fused forms in their tagged form only)
+    const bool fused = A.factor_order && A.pc && A.pc->sweep &&
+                       (A.pc->contiguous || (A.pc->blocked && s->fuse >= 3)) &&
                        std::max(A.pc->nuchunks, A.pc->nlchunks) <= s->pb_cap && s->fuse >= 1;
     // level 2: four launches per iteration, the reductions finished inside the launches that produce
     // them: SpMV (+ alpha), forward sweep (+ x/r), backward sweep (+ beta and the stopping test), p update
@@ -695,7 +697,7 @@ static void cg_run(Context* c, pfem_solver_t* s, const KArgs<T>& A, CgState* st,
     if (fused2) {
         hipLaunchKernelGGL((k_fill_sentinel<double>), dim3(blocks_for(P.ntiles)), dim3(kBlock), 0, c->stream,
                            (long)P.ntiles, s->pa);
-        const int nslots = std::max(A.pc->nuchunks, A.pc->nlchunks);
+        const int nslots = std::max(std::max(A.pc->nuchunks, A.pc->nlchunks), std::max(A.pc->nuslices, A.pc->nlslices));
         hipLaunchKernelGGL((k_fill_sentinel<double>), dim3(blocks_for(nslots)), dim3(kBlock), 0, c->stream,
                            (long)nslots, s->pb);
         // z_0 has been copied into p: z's storage now becomes the sweeps' tagged buffer

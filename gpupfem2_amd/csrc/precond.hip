@@ -523,6 +523,240 @@ __global__ __launch_bounds__(kBlock) void k_sweep_upper_xp(SweepArgs<TV, TS, TF>
     }
 }
 
+// ------------------------------------------------------------------ blocked sweeps (PFEM_ORDER_BLOCKS)
+// A lane per row, as the multicolour sweeps: each row first subtracts its entries outside its block
+// (other colours, ascending, waiting on tags), keeping the coefficients of its block's own entries; then
+// the block's rows finish one after the other inside the wave, each value passed to the block's other
+// rows by a shuffle. A row is finished by the reciprocal of its diagonal (one division per row rather
+// than one per step of the block), so values differ from a division by rounding. Forward, the terms of
+// every row come in the sequential order (the block's columns are the row's largest); backward, the
+// block's entries (the smallest columns) come after the outside ones instead of before them.
+
+// entries of a row loaded together by the blocked sweeps (fewer than kDeps: the block's coefficients take
+// registers of their own)
+constexpr int kBlockDeps = 4;
+
+// Outside entries of the calling lane's row subtracted from s (tagged buffer, ready once a value carries
+// tag WAIT); the coefficients of the row's in-block entries go to cf[t'] with bit t' of mask.
+template <int WAIT, typename TV, typename TS, typename TF>
+__device__ __forceinline__ void block_row_sums(const SweepArgs<TV, TS, TF>& a, TF& s, TF (&cf)[kBlockRows],
+                                               unsigned& mask, int e0, int stride, int width, TF* buf)
+{
+    using U = unsigned long long;
+    constexpr int K = kBlockDeps;
+    for (int w0 = 0; w0 < width; w0 += K) {
+        int cv[K];
+        TF fv[K];
+        U yv[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            cv[k] = -1;
+            if (w0 + k < width) {
+                const int e = e0 + (w0 + k) * stride;
+                cv[k] = a.col[e];
+                fv[k] = (TF)a.f[e];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (cv[k] >= 0)
+                yv[k] = __hip_atomic_load(reinterpret_cast<U*>(buf + cv[k]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (cv[k] >= 0) {
+                if ((yv[k] & 1ull) != (U)WAIT) yv[k] = __builtin_bit_cast(U, await_tagged<WAIT>(buf + cv[k], a.err, a.limit));
+                s -= fv[k] * __builtin_bit_cast(TF, yv[k]);
+            } else if (cv[k] <= -2) {
+                const int tp = -2 - cv[k];
+#pragma unroll
+                for (int q = 0; q < kBlockRows; ++q)
+                    if (q == tp) cf[q] = fv[k];
+                mask |= 1u << tp;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ double shfl_f64(double v, int src)
+{
+    return __shfl(v, src, 64);
+}
+
+// Forward sweep, tagged (y published with tag 1). XR: the fused PCG's x/r update of the row first (as
+// k_sweep_lower); with an IC0 factor and a.cg set the chunks also publish their partials of (y, y) and the
+// last workgroup records the iteration (as k_sweep_lower).
+template <typename TV, typename TS, typename TF, bool UNIT, bool XR>
+__global__ __launch_bounds__(kBlock) void k_bsweep_lower(SweepArgs<TV, TS, TF> a)
+{
+    static_assert(sizeof(TF) == 8 && sizeof(TV) == 8, "tagged sweeps: FP64 vectors and values");
+    constexpr bool kYY = XR && !UNIT;
+    __shared__ double wsum[kWaves];
+    __shared__ unsigned arrived;
+    if (a.done && *a.done) return;
+    const bool yy = kYY && a.cg;
+    if (yy) {
+        if (threadIdx.x == 0) arrived = 0u;
+        __syncthreads();
+    }
+    const int slot = hdr_slot();
+    const int* h = a.chunks + 4 * slot;
+    const int first = h[0], cnt = h[1], base = h[2], width = h[3];
+    const int tc = a.cbase[4 * slot];
+    const int u = (int)threadIdx.x;
+    const bool has = u < cnt;
+    const int t = u & (tc - 1);
+    const int i = first + u;
+    TF s = 0, d = 1;
+    TF cf[kBlockRows];
+#pragma unroll
+    for (int q = 0; q < kBlockRows; ++q) cf[q] = 0;
+    unsigned mask = 0;
+    if (has) {
+        TV b;
+        if constexpr (XR) {
+            const TV al = (TV)(a.acg ? a.acg->alpha[a.acg->it & 1] : *a.alpha);
+            if (a.x) a.x[i] += al * a.p[i];
+            b = a.r[i] - al * a.q[i];
+            a.r[i] = b;
+        } else {
+            b = a.src[i];
+        }
+        s = (TF)b;
+        if constexpr (sizeof(TS) == 2) s *= (TF)a.in_scale[1];
+        if constexpr (!UNIT) d = (TF)a.d[i];
+        block_row_sums<1>(a, s, cf, mask, base + u, cnt, width, a.sy);
+    }
+    // the block's rows in order: at step tau row tau finishes (times the reciprocal of its diagonal,
+    // computed once) and the block's later rows subtract its term; lb: the block's first lane in the wave
+    const int lb = (u & 63) - t;
+    const TF dinv = UNIT ? (TF)1 : (TF)1 / d;
+    TF yv = 0;
+#pragma unroll
+    for (int tau = 0; tau < kBlockRows; ++tau) {
+        if (tau >= tc) break;
+        if (t == tau) yv = UNIT ? s : s * dinv;
+        const TF yt = shfl_f64(tagged<1>(yv), lb + tau);   // what a dependent row reads from the buffer
+        if (mask & (1u << tau)) s -= cf[tau] * yt;
+    }
+    double v = 0.0;
+    if (has) {
+        publish_tagged<1>(a.sy + i, yv);
+        v = (double)yv * (double)yv;
+    }
+    if constexpr (kYY) {
+        if (!yy) return;
+        v = wave_sum(v);
+        if ((threadIdx.x & 63) == 0) {
+            wsum[threadIdx.x >> 6] = v;
+            __threadfence_block();
+            if (atomicAdd(&arrived, 1u) == kWaves - 1) {
+                __threadfence_block();
+                double tot = wsum[0];
+                for (int w = 1; w < kWaves; ++w) tot += wsum[w];
+                publish(a.part + blockIdx.x, tot);
+            }
+        }
+        if (blockIdx.x == gridDim.x - 1) {
+            __syncthreads();
+            double g = collect_partials(a.part, gridDim.x, a.err, a.limit);
+            if constexpr (sizeof(TS) == 2) g /= a.in_scale[1];
+            if (threadIdx.x == 0) cg_record(a.cg, g, a.cg->it);
+        }
+    }
+}
+
+// Backward sweep, tagged (z published with tag 0 in place of y). XP: the fused IC0-PCG's end-of-iteration
+// x and p updates of the row (as k_sweep_upper_xp); otherwise out (when distinct) and the optional (r, z)
+// partials with the last workgroup's cg_record (as k_sweep_upper).
+template <typename TV, typename TS, typename TF, bool XP>
+__global__ __launch_bounds__(kBlock) void k_bsweep_upper(SweepArgs<TV, TS, TF> a)
+{
+    static_assert(sizeof(TF) == 8 && sizeof(TV) == 8, "tagged sweeps: FP64 vectors and values");
+    __shared__ double wsum[kWaves];
+    __shared__ unsigned arrived;
+    TV al = 0, beta = 0;
+    bool last = false;
+    if constexpr (XP) {
+        const CgState* st = a.acg;
+        const int it = st->it;
+        if (it == st->xp_seen) return;
+        const int kk = it - 1;
+        last = st->done;
+        al = (TV)st->alpha[kk & 1];
+        if (!last) beta = (TV)(st->gam[(kk + 1) & 1] / st->gam[kk & 1]);
+    } else {
+        if (a.done && *a.done) return;
+        if (a.part) {
+            if (threadIdx.x == 0) arrived = 0u;
+            __syncthreads();
+        }
+    }
+    const int slot = hdr_slot();
+    const int* h = a.chunks + 4 * slot;
+    const int first = h[0], cnt = h[1], base = h[2], width = h[3];
+    const int tc = a.cbase[4 * slot];
+    const int u = (int)threadIdx.x;
+    const bool has = u < cnt;
+    const int t = u & (tc - 1);
+    const int i = first + u;
+    if (XP && last) {
+        if (has) a.xu[i] += al * a.pu[i];
+        return;
+    }
+    TF s = 0, d = 1;
+    TF cf[kBlockRows];
+#pragma unroll
+    for (int q = 0; q < kBlockRows; ++q) cf[q] = 0;
+    unsigned mask = 0;
+    if (has) {
+        s = a.sy[i];
+        d = (TF)a.d[i];
+        block_row_sums<0>(a, s, cf, mask, base + u, cnt, width, a.sw);
+    }
+    const int lb = (u & 63) - t;
+    const TF dinv = (TF)1 / d;
+    TF zmine = 0;
+#pragma unroll
+    for (int tau = kBlockRows - 1; tau >= 0; --tau) {
+        if (tau >= tc) continue;
+        if (t == tau) zmine = tagged<0>(s * dinv);
+        const TF zt = shfl_f64(zmine, lb + tau);
+        if (mask & (1u << tau)) s -= cf[tau] * zt;
+    }
+    double v = 0.0;
+    if (has) {
+        publish_tagged<0>(a.sw + i, zmine);
+        if constexpr (XP) {
+            const TV pi = a.pu[i];
+            a.xu[i] += al * pi;
+            a.pu[i] = (TV)zmine + beta * pi;
+        } else {
+            if ((const void*)a.out != (const void*)a.sw) a.out[i] = (TV)zmine;
+            if (a.part) v = (double)(a.rdot[i] * (TV)zmine);
+        }
+    }
+    if constexpr (!XP) {
+        if (!a.part) return;
+        v = wave_sum(v);
+        if ((threadIdx.x & 63) == 0) {
+            wsum[threadIdx.x >> 6] = v;
+            __threadfence_block();
+            if (atomicAdd(&arrived, 1u) == kWaves - 1) {
+                __threadfence_block();
+                double tot = wsum[0];
+                for (int w = 1; w < kWaves; ++w) tot += wsum[w];
+                if (a.cg) publish(a.part + blockIdx.x, tot);
+                else a.part[blockIdx.x] = tot;
+            }
+        }
+        if (a.cg && blockIdx.x == gridDim.x - 1) {
+            __syncthreads();
+            const double g = collect_partials(a.part, gridDim.x, a.err, a.limit);
+            if (threadIdx.x == 0) cg_record(a.cg, g, a.cg->it);
+        }
+    }
+}
+
 template <typename T>
 __global__ void k_copyT(long n, T* __restrict__ d, const T* __restrict__ s)
 {
@@ -638,14 +872,14 @@ std::vector<int> rcm_order(const Pattern& A)
 // v's a-coloured neighbours, allowed when that union holds none of v's b-coloured neighbours (every
 // such swap keeps the colouring proper). When no pair works, the chain through one neighbour is swapped
 // and the search repeats, a bounded number of times. Deterministic; stragglers that resist keep their
-// colour. Returns the colour count.
-static int reduce_colours(const Pattern& A, std::vector<int>& colour, int ncol, int K)
+// colour. More than max_bad stragglers are left as they are. Returns the colour count.
+static int reduce_colours(const Pattern& A, std::vector<int>& colour, int ncol, int K, size_t max_bad)
 {
     const int n = A.n;
     std::vector<int> bad;
     for (int i = 0; i < n; ++i)
         if (colour[i] >= K) bad.push_back(i);
-    if (bad.empty() || bad.size() > (size_t)n / 100 + 16) return ncol;
+    if (bad.empty() || bad.size() > max_bad) return ncol;
     constexpr size_t kCap = 1u << 18;   // largest chain considered
     std::vector<int> seen(n, -1), comp, nb;
     std::vector<char> forbid(n, 0);
@@ -722,12 +956,173 @@ static int reduce_colours(const Pattern& A, std::vector<int>& colour, int ncol, 
     return m;
 }
 
+// PFEM_ORDER_BLOCKS. Blocks of at most kBlockRows nodes, each grown breadth-first over unassigned nodes
+// from a seed taken in reverse Cuthill-McKee order (on a triangulation nearly every block is full and
+// compact), a block of another size split into power-of-two pieces in its own node order; the block
+// graph coloured greedily in block order, reduced to 4 colours where Kempe interchanges allow. New order:
+// colour by colour; within a colour, chunks of up to kBlock rows made of blocks of one size tc (full
+// blocks first, in block order), a block's nodes consecutive, so a block's rows are tc neighbouring
+// lanes of one wave. A zero-fill factor depends only on which of two coupled nodes comes first, which is
+// what the order (colour, block, node within block) fixes. chunks[colour] lists {first row, rows | tc << 16}.
+static void block_order(const Pattern& A, std::vector<int>& perm, std::vector<int>& cptr,
+                        std::vector<std::vector<std::pair<int, int>>>& chunks)
+{
+    const int n = A.n;
+    const std::vector<int> order = rcm_order(A);
+    std::vector<int> blk(n, -1), bptr{0}, bnode, grown;
+    bnode.reserve(n);
+    for (int s : order) {
+        if (blk[s] >= 0) continue;
+        grown.assign(1, s);
+        blk[s] = 0;
+        for (size_t h = 0; h < grown.size() && grown.size() < (size_t)kBlockRows; ++h) {
+            const int u = grown[h];
+            for (int k = A.h_rowptr[u]; k < A.h_rowptr[u + 1] && grown.size() < (size_t)kBlockRows; ++k) {
+                const int v = A.h_col[k];
+                if (blk[v] < 0) {
+                    blk[v] = 0;
+                    grown.push_back(v);
+                }
+            }
+        }
+        // power-of-two pieces, largest first
+        size_t o = 0;
+        for (int piece = kBlockRows; piece >= 1; piece >>= 1)
+            while (grown.size() - o >= (size_t)piece) {
+                const int b = (int)bptr.size() - 1;
+                for (int q = 0; q < piece; ++q) {
+                    blk[grown[o + q]] = b;
+                    bnode.push_back(grown[o + q]);
+                }
+                o += piece;
+                bptr.push_back((int)bnode.size());
+            }
+    }
+    const int nb = (int)bptr.size() - 1;
+    // block graph (host CSR only; each row lists the block itself too)
+    Pattern B;
+    B.n = nb;
+    B.h_rowptr.assign(nb + 1, 0);
+    {
+        std::vector<int> seen(nb, -1), row;
+        for (int b = 0; b < nb; ++b) {
+            row.clear();
+            for (int q = bptr[b]; q < bptr[b + 1]; ++q) {
+                const int u = bnode[q];
+                for (int k = A.h_rowptr[u]; k < A.h_rowptr[u + 1]; ++k) {
+                    const int cb = blk[A.h_col[k]];
+                    if (seen[cb] != b) {
+                        seen[cb] = b;
+                        row.push_back(cb);
+                    }
+                }
+            }
+            std::sort(row.begin(), row.end());
+            B.h_col.insert(B.h_col.end(), row.begin(), row.end());
+            B.h_rowptr[b + 1] = (int)B.h_col.size();
+        }
+    }
+    std::vector<int> colour(nb, -1), mark;
+    int ncol = 0;
+    for (int b = 0; b < nb; ++b) {
+        mark.assign(ncol + 1, 0);
+        for (int k = B.h_rowptr[b]; k < B.h_rowptr[b + 1]; ++k) {
+            const int j = B.h_col[k];
+            if (j != b && colour[j] >= 0) mark[colour[j]] = 1;
+        }
+        int cc = 0;
+        while (cc < ncol && mark[cc]) ++cc;
+        colour[b] = cc;
+        if (cc == ncol) ++ncol;
+    }
+    ncol = reduce_colours(B, colour, ncol, 4, (size_t)nb);   // a colour is a dependency level of the sweeps
+    for (int b = 0; b < nb; ++b)
+        for (int k = B.h_rowptr[b]; k < B.h_rowptr[b + 1]; ++k)
+            PFEM_REQUIRE(B.h_col[k] == b || colour[B.h_col[k]] != colour[b], "block colouring is not proper");
+    perm.clear();
+    perm.reserve(n);
+    cptr.assign(1, 0);
+    chunks.assign(ncol, {});
+    std::vector<std::vector<int>> by_size(kBlockRows + 1);
+    for (int cc = 0; cc < ncol; ++cc) {
+        for (auto& v : by_size) v.clear();
+        for (int b = 0; b < nb; ++b)
+            if (colour[b] == cc) by_size[bptr[b + 1] - bptr[b]].push_back(b);
+        for (int size = kBlockRows; size >= 1; size >>= 1) {
+            const std::vector<int>& bl = by_size[size];
+            const size_t per = (size_t)(kBlock / size);
+            for (size_t q = 0; q < bl.size(); q += per) {
+                const int m = (int)std::min(per, bl.size() - q);
+                chunks[cc].emplace_back((int)perm.size(), (m * size) | (size << 16));
+                for (int l = 0; l < m; ++l)
+                    for (int t = 0; t < size; ++t) perm.push_back(bnode[bptr[bl[q + l]] + t]);
+            }
+        }
+        cptr.push_back((int)perm.size());
+    }
+    PFEM_REQUIRE((int)perm.size() == n, "block ordering lost a node");
+}
+
+// Chunks of each level, dealt to workgroup slots XCD by XCD: workgroup b runs on XCD b mod 8, and each XCD
+// receives one contiguous band of the level's chunks (in the level's own order, which follows the mesh).
+// The reordering stays inside the level, so every workgroup still waits only on lower-numbered ones.
+// Returns the flat {first, second} pairs in workgroup order.
+static std::vector<int> deal_chunks(const std::vector<std::vector<std::pair<int, int>>>& levels)
+{
+    constexpr int kXcds = 8;
+    std::vector<int> ch;
+    for (const auto& level : levels) {
+        const int m = (int)level.size();
+        const int base = (int)ch.size() / 2;   // workgroup index of the level's first chunk
+        std::vector<int> next(kXcds), end(kXcds);
+        for (int x = 0; x < kXcds; ++x) {
+            next[x] = (int)((long)m * x / kXcds);
+            end[x] = (int)((long)m * (x + 1) / kXcds);
+        }
+        std::vector<char> used(m, 0);
+        int spill = 0;   // lowest chunk not yet dealt, for slots whose band is exhausted
+        for (int b = 0; b < m; ++b) {
+            const int x = (base + b) % kXcds;
+            int q = -1;
+            while (next[x] < end[x] && used[next[x]]) ++next[x];
+            if (next[x] < end[x]) q = next[x]++;
+            else {
+                while (used[spill]) ++spill;
+                q = spill;
+            }
+            used[q] = 1;
+            ch.push_back(level[q].first);
+            ch.push_back(level[q].second);
+        }
+    }
+    return ch;
+}
+
+// Chunk q's header and base entries (4 ints each) moved to hdr_slot's position for workgroup q (grid =
+// number of chunks).
+static void to_slot_order(std::vector<int>& hdr, std::vector<int>& cbase)
+{
+    const int G = (int)(hdr.size() / 4), q8 = G >> 3, rem = G & 7;
+    std::vector<int> h2(hdr.size()), c2(cbase.size());
+    for (int b = 0; b < G; ++b) {
+        const int x = b & 7, sl = x * q8 + std::min(x, rem) + (b >> 3);
+        for (int k = 0; k < 4; ++k) {
+            h2[4 * sl + k] = hdr[4 * b + k];
+            c2[4 * sl + k] = cbase[4 * b + k];
+        }
+    }
+    hdr.swap(h2);
+    cbase.swap(c2);
+}
+
 static void build_factor_structure(Context* c, pfem_precond_t* pc)
 {
     const Pattern& A = *pc->pat;
     const int n = A.n;
     std::vector<int> perm(n), iperm(n);
-    std::vector<int> cptr;   // multicolour: colour c occupies rows [cptr[c], cptr[c+1]) of the new order
+    std::vector<int> cptr;   // multicolour, blocks: colour c occupies rows [cptr[c], cptr[c+1]) of the new order
+    std::vector<std::vector<std::pair<int, int>>> bchunks;   // blocks: the chunks of each colour
+    const bool blocked = pc->ordering == PFEM_ORDER_BLOCKS;
     if (pc->ordering == PFEM_ORDER_MULTICOLOR) {
         // greedy colouring visiting the nodes in reverse Cuthill-McKee order, then Kempe-chain reduction of
         // the few nodes above colour 3; new order = (colour, RCM position). On the cylinder channel this
@@ -749,7 +1144,7 @@ static void build_factor_structure(Context* c, pfem_precond_t* pc)
             colour[i] = cc;
             if (cc == ncol) ++ncol;
         }
-        ncol = reduce_colours(A, colour, ncol, 4);
+        ncol = reduce_colours(A, colour, ncol, 4, (size_t)n / 100 + 16);
         for (int i = 0; i < n; ++i)
             for (int k = A.h_rowptr[i]; k < A.h_rowptr[i + 1]; ++k)
                 PFEM_REQUIRE(A.h_col[k] == i || colour[A.h_col[k]] != colour[i], "colouring is not proper");
@@ -758,6 +1153,8 @@ static void build_factor_structure(Context* c, pfem_precond_t* pc)
         cptr.assign(ncol + 1, 0);
         for (int i = 0; i < n; ++i) cptr[colour[i] + 1]++;
         for (int k = 0; k < ncol; ++k) cptr[k + 1] += cptr[k];
+    } else if (pc->ordering == PFEM_ORDER_BLOCKS) {
+        block_order(A, perm, cptr, bchunks);
     } else {
         std::iota(perm.begin(), perm.end(), 0);
     }
@@ -782,8 +1179,9 @@ static void build_factor_structure(Context* c, pfem_precond_t* pc)
     for (int i = 0; i < n; ++i) PFEM_REQUIRE(F.h_diag[i] >= 0, "IC0/ILU0 needs every diagonal entry in the pattern");
     // forward levels: longest-path levels of the lower triangle; in the multicolour order a colour is an
     // independent set whose lower neighbours all have smaller colours, so the colours are the levels
-    // (contiguous row ranges, forward in colour order and backward in reverse colour order)
-    const bool colour_levels = !cptr.empty();
+    // (contiguous row ranges, forward in colour order and backward in reverse colour order). The blocked
+    // order keeps the longest-path levels (a colour holds several: the rows of a block depend on each other)
+    const bool colour_levels = !cptr.empty() && !blocked;
     std::vector<int> lev(n, 0);
     int nl = 0;
     for (int i = 0; i < n; ++i) {
@@ -857,36 +1255,11 @@ static void build_factor_structure(Context* c, pfem_precond_t* pc)
     // row's dependencies were published on its own XCD (its L2) rather than on any of the eight. The
     // reordering stays inside the level, so every workgroup still waits only on lower-numbered ones.
     auto chunk_table = [&](const std::vector<int>& ptr, const std::vector<int>& rows, int rc) {
-        constexpr int kXcds = 8;
-        std::vector<int> ch;
-        for (size_t l = 0; l + 1 < ptr.size(); ++l) {
-            std::vector<std::pair<int, int>> level;
+        std::vector<std::vector<std::pair<int, int>>> levels(ptr.size() - 1);
+        for (size_t l = 0; l + 1 < ptr.size(); ++l)
             for (int o = ptr[l]; o < ptr[l + 1]; o += rc)
-                level.emplace_back(colour_levels ? rows[o] : o, std::min(rc, ptr[l + 1] - o));
-            const int m = (int)level.size();
-            const int base = (int)ch.size() / 2;   // workgroup index of the level's first chunk
-            std::vector<int> next(kXcds), end(kXcds);
-            for (int x = 0; x < kXcds; ++x) {
-                next[x] = (int)((long)m * x / kXcds);
-                end[x] = (int)((long)m * (x + 1) / kXcds);
-            }
-            std::vector<char> used(m, 0);
-            int spill = 0;   // lowest chunk not yet dealt, for slots whose band is exhausted
-            for (int b = 0; b < m; ++b) {
-                const int x = (base + b) % kXcds;
-                int q = -1;
-                while (next[x] < end[x] && used[next[x]]) ++next[x];
-                if (next[x] < end[x]) q = next[x]++;
-                else {
-                    while (used[spill]) ++spill;
-                    q = spill;
-                }
-                used[q] = 1;
-                ch.push_back(level[q].first);
-                ch.push_back(level[q].second);
-            }
-        }
-        return ch;
+                levels[l].emplace_back(colour_levels ? rows[o] : o, std::min(rc, ptr[l + 1] - o));
+        return deal_chunks(levels);
     };
     const std::vector<int> lch = chunk_table(lptr, lrows, kBlock), uch = chunk_table(uptr, urows, kBlock);
     pc->contiguous = colour_levels;
@@ -919,7 +1292,7 @@ static void build_factor_structure(Context* c, pfem_precond_t* pc)
         upload(c, pc->split_pos, spos);
         pc->split.alloc(F.nnz);
     }
-    if (colour_levels) {
+    if (colour_levels || blocked) {
         // Sliced layouts: a lane finds its row's entries from its chunk's header alone (no row offsets to
         // load first), and the loads of one entry index are unit-stride over the chunk. Column indices
         // are stored in 16 bits where a chunk allows it: 2 bits of colour and a 14-bit offset from the
@@ -976,20 +1349,7 @@ static void build_factor_structure(Context* c, pfem_precond_t* pc)
                 pos.push_back(-1);
                 c16.push_back(0xFFFF);
             }
-            {
-                // chunk q's entries at hdr_slot's position for workgroup q (grid = number of chunks)
-                const int G = (int)(hdr.size() / 4), q8 = G >> 3, rem = G & 7;
-                std::vector<int> h2(hdr.size()), c2(cbase.size());
-                for (int b = 0; b < G; ++b) {
-                    const int x = b & 7, sl = x * q8 + std::min(x, rem) + (b >> 3);
-                    for (int k = 0; k < 4; ++k) {
-                        h2[4 * sl + k] = hdr[4 * b + k];
-                        c2[4 * sl + k] = cbase[4 * b + k];
-                    }
-                }
-                hdr.swap(h2);
-                cbase.swap(c2);
-            }
+            to_slot_order(hdr, cbase);
             upload(c, out.hdr, hdr);
             upload(c, out.cbase, cbase);
             upload(c, out.col, col);
@@ -997,12 +1357,70 @@ static void build_factor_structure(Context* c, pfem_precond_t* pc)
             upload(c, out.pos, pos);
             val_d.alloc(col.size());
         };
-        // the sliced sweeps take kSliceRows rows per chunk (kLaneRows per lane)
-        const std::vector<int> lch2 = chunk_table(lptr, lrows, kSliceRows), uch2 = chunk_table(uptr, urows, kSliceRows);
-        pc->nlslices = (int)lch2.size() / 2;
-        pc->nuslices = (int)uch2.size() / 2;
-        slice(lch2, 0, pc->sl_lower, pc->sell_l);
-        slice(uch2, 1, pc->sl_upper, pc->sell_u);
+        // Blocked layouts: the chunk-sliced triangle of each chunk (as above, a lane per row), with the
+        // columns inside the row's own block coded -2 - t' (t' the column's position in the block);
+        // cbase per chunk: {tc, 0, 0, 0}. A column inside the chunk's rows must be in the row's block.
+        auto slice_blocks = [&](const std::vector<int>& ch, int part, SlicedCols& out, DevArray<double>& val_d) {
+            std::vector<int> hdr, col, pos, cbase;
+            for (size_t q = 0; q < ch.size(); q += 2) {
+                const int r0 = ch[q], cnt = ch[q + 1] & 0xFFFF, tc = ch[q + 1] >> 16;
+                PFEM_REQUIRE(cnt >= 1 && cnt <= kBlock && tc >= 1 && tc <= kBlockRows && (tc & (tc - 1)) == 0 &&
+                                 cnt % tc == 0,
+                             "bad block chunk");
+                auto k0 = [&](int i) { return part == 1 ? F.h_diag[i] + 1 : F.h_rowptr[i]; };
+                auto k1 = [&](int i) { return part == 0 ? F.h_diag[i] : F.h_rowptr[i + 1]; };
+                int width = 0;
+                for (int i = r0; i < r0 + cnt; ++i) width = std::max(width, k1(i) - k0(i));
+                const int base = (int)col.size();
+                col.resize(col.size() + (size_t)width * cnt, -1);
+                pos.resize(col.size(), -1);
+                for (int u = 0; u < cnt; ++u) {
+                    const int i = r0 + u, b0 = i - u % tc;
+                    for (int k = k0(i); k < k1(i); ++k) {
+                        const int j = F.h_col[k];
+                        const size_t e = base + (size_t)(k - k0(i)) * cnt + u;
+                        if (j >= r0 && j < r0 + cnt) {
+                            PFEM_REQUIRE(j >= b0 && j < b0 + tc, "block ordering: a coupling across blocks of one colour");
+                            col[e] = -2 - (j - b0);
+                        } else {
+                            col[e] = j;
+                        }
+                        pos[e] = k;
+                    }
+                }
+                hdr.insert(hdr.end(), {r0, cnt, base, width});
+                cbase.insert(cbase.end(), {tc, 0, 0, 0});
+            }
+            if (col.empty()) {
+                col.push_back(-1);
+                pos.push_back(-1);
+            }
+            to_slot_order(hdr, cbase);
+            std::vector<unsigned short> c16(1, 0xFFFF);
+            upload(c, out.hdr, hdr);
+            upload(c, out.cbase, cbase);
+            upload(c, out.col, col);
+            upload(c, out.col16, c16);
+            upload(c, out.pos, pos);
+            val_d.alloc(col.size());
+        };
+        if (blocked) {
+            // the forward sweep takes the colours in order, the backward sweep in reverse order
+            const std::vector<int> lch2 = deal_chunks(bchunks);
+            const std::vector<int> uch2 = deal_chunks(std::vector<std::vector<std::pair<int, int>>>(bchunks.rbegin(), bchunks.rend()));
+            pc->nlslices = (int)lch2.size() / 2;
+            pc->nuslices = (int)uch2.size() / 2;
+            slice_blocks(lch2, 0, pc->sl_lower, pc->sell_l);
+            slice_blocks(uch2, 1, pc->sl_upper, pc->sell_u);
+            pc->blocked = true;
+        } else {
+            // the sliced sweeps take kSliceRows rows per chunk (kLaneRows per lane)
+            const std::vector<int> lch2 = chunk_table(lptr, lrows, kSliceRows), uch2 = chunk_table(uptr, urows, kSliceRows);
+            pc->nlslices = (int)lch2.size() / 2;
+            pc->nuslices = (int)uch2.size() / 2;
+            slice(lch2, 0, pc->sl_lower, pc->sell_l);
+            slice(uch2, 1, pc->sl_upper, pc->sell_u);
+        }
         pc->sell = true;
         // The whole matrix over the SpMV tiles, dealt by region across colours: XCD x (workgroups b = x mod
         // 8) takes the tiles of the x-th eighth of every colour's range. A row's neighbours lie in the other
@@ -1205,8 +1623,8 @@ static void sweeps(Context* c, pfem_precond_t* pc, const TV* src, TV* out, const
     a.cg = part ? o.cg : nullptr;
     a.acg = o.alpha_cg;
     const bool xr = o.alpha != nullptr;
-    PFEM_REQUIRE(!xr || (o.factor_order && pc->contiguous && o.q && (o.p || !o.x)),
-                 "the fused x/r update needs a multicolour factor and vectors in its ordering");
+    PFEM_REQUIRE(!xr || (o.factor_order && (pc->contiguous || (pc->blocked && o.tagged)) && o.q && (o.p || !o.x)),
+                 "the fused x/r update needs a multicolour or blocked factor and vectors in its ordering");
     dim3 gl(pc->nlchunks), gu(pc->nuchunks);
     const dim3 bl(kBlock);
     SweepArgs<TV, TS, TF> lo = a, up = a;
@@ -1214,6 +1632,56 @@ static void sweeps(Context* c, pfem_precond_t* pc, const TV* src, TV* out, const
     up.chunks = pc->uchunks.p;
     lo.ptr = pc->split_ptr.p;
     up.ptr = pc->split_ptr.p + (F.n + 1);
+    if constexpr (sizeof(TS) == 2) lo.in_scale = up.in_scale = pc->scale16.p;
+    if constexpr (sizeof(TF) == 8 && sizeof(TV) == 8) {
+        if (pc->blocked && o.tagged) {
+            // y and z share out: y with tag 1, then z with tag 0 (out must carry tag 0 on entry)
+            PFEM_REQUIRE(o.factor_order && (const void*)out != (const void*)src,
+                         "tagged sweeps need vectors in the factor's ordering and z distinct from r");
+            lo.chunks = pc->sl_lower.hdr.p;
+            up.chunks = pc->sl_upper.hdr.p;
+            lo.col = pc->sl_lower.col.p;
+            up.col = pc->sl_upper.col.p;
+            lo.col16 = up.col16 = nullptr;
+            lo.cbase = pc->sl_lower.cbase.p;
+            up.cbase = pc->sl_upper.cbase.p;
+            if constexpr (sizeof(TS) == 2) {
+                lo.f = pc->sell_l16.p;
+                up.f = pc->sell_u16.p;
+            } else if constexpr (sizeof(TS) == 4) {
+                lo.f = pc->sell_l32.p;
+                up.f = pc->sell_u32.p;
+            } else {
+                lo.f = pc->sell_l.p;
+                up.f = pc->sell_u.p;
+            }
+            lo.sy = lo.sw = up.sy = up.sw = reinterpret_cast<TF*>(out);
+            gl = dim3(pc->nlslices);
+            gu = dim3(pc->nuslices);
+            bool xp = false;
+            if (xr && ic && up.cg) {
+                // as in the multicolour form: the forward sweep ends the iteration, the backward sweep
+                // applies the x and p updates given o.xpx
+                up.part = nullptr;
+                up.cg = nullptr;
+                up.xu = o.xpx;
+                up.pu = o.xpp;
+                xp = o.xpx != nullptr;
+            } else {
+                lo.cg = nullptr;
+            }
+            if (xr) {
+                if (ic) hipLaunchKernelGGL((k_bsweep_lower<TV, TS, TF, false, true>), gl, bl, 0, c->stream, lo);
+                else hipLaunchKernelGGL((k_bsweep_lower<TV, TS, TF, true, true>), gl, bl, 0, c->stream, lo);
+            } else {
+                if (ic) hipLaunchKernelGGL((k_bsweep_lower<TV, TS, TF, false, false>), gl, bl, 0, c->stream, lo);
+                else hipLaunchKernelGGL((k_bsweep_lower<TV, TS, TF, true, false>), gl, bl, 0, c->stream, lo);
+            }
+            if (xp) hipLaunchKernelGGL((k_bsweep_upper<TV, TS, TF, true>), gu, bl, 0, c->stream, up);
+            else hipLaunchKernelGGL((k_bsweep_upper<TV, TS, TF, false>), gu, bl, 0, c->stream, up);
+            return;
+        }
+    }
     if (pc->contiguous && pc->sell) {
         gl = dim3(pc->nlslices);
         gu = dim3(pc->nuslices);
@@ -1326,8 +1794,8 @@ int precond_apply_t(Context* c, pfem_precond_t* pc, TV* z, const TV* r, bool fp3
                 // factor stored in FP32, solves in FP64: an exact apply of the rounded factor
                 PFEM_REQUIRE(pc->fac32.p && pc->sweep, "FP32 factor not enabled (or per-level apply selected)");
                 if (pc->factor_bits == 16 && pc->half_ok) {
-                    PFEM_REQUIRE(pc->split16.p && pc->sell && pc->contiguous,
-                                 "a 16-bit factor needs a multicolour factor with the sliced sweep layout");
+                    PFEM_REQUIRE(pc->split16.p && pc->sell && (pc->contiguous || pc->blocked),
+                                 "a 16-bit factor needs a multicolour or blocked factor with the sliced sweep layout");
                     sweeps<double, _Float16, double>(c, pc, r, z, pc->split16.p, pc->sy.p, pc->sw.p, part, opt);
                 } else {
                     sweeps<double, float, double>(c, pc, r, z, pc->split32.p, pc->sy.p, pc->sw.p, part, opt);
@@ -1345,7 +1813,7 @@ int precond_apply_t(Context* c, pfem_precond_t* pc, TV* z, const TV* r, bool fp3
     }
     PFEM_HIP(hipGetLastError());
     // partials written: one per backward launch workgroup
-    const int nbw = pc->contiguous && pc->sell ? pc->nuslices : pc->nuchunks;
+    const int nbw = (pc->contiguous && pc->sell) || (pc->blocked && opt.tagged) ? pc->nuslices : pc->nuchunks;
     return part && (pc->kind == PFEM_PRECOND_IC0 || pc->kind == PFEM_PRECOND_ILU0) ? nbw : 0;
 }
 
@@ -1420,6 +1888,7 @@ int pfem_precond_create(pfem_csr a, int kind, int ordering, pfem_precond* out)
     PFEM_TRY(a ? a->ctx : nullptr, {
         PFEM_REQUIRE(a && out, "null argument");
         PFEM_REQUIRE(kind >= PFEM_PRECOND_NONE && kind <= PFEM_PRECOND_ILU0, "unknown preconditioner kind");
+        PFEM_REQUIRE(ordering >= PFEM_ORDER_NATURAL && ordering <= PFEM_ORDER_BLOCKS, "unknown ordering");
         auto pc = std::make_unique<pfem_precond_t>();
         pc->ctx = a->ctx;
         pc->kind = kind;

@@ -18,6 +18,12 @@ struct SlicedCols {
     DevArray<unsigned short> col16;
 };
 
+// Blocked layout (PFEM_ORDER_BLOCKS): a chunk holds blocks of one colour and one size tc (a power of two,
+// at most kBlockRows), a block's rows consecutive, one row per lane, so a block lies in one wave; the
+// triangle is chunk-sliced as for the multicolour order, except that a column inside the row's own block
+// is coded -2 - t' (t' the column's position in the block); cbase per chunk: {tc, 0, 0, 0}.
+constexpr int kBlockRows = 8;
+
 // A chunk's four colour bases, loaded once (uniform values) so that decoding a column is register work
 struct ChunkBases {
     int b0, b1, b2, b3;
@@ -234,6 +240,10 @@ struct pfem_precond_t {
     pfem::DevArray<_Float16> split16, sell_l16, sell_u16;   // c L, c = scale16[0] (a power of two)
     pfem::DevArray<double> scale16;       // c, c^2, and the largest |L| found (as bits)
     bool sell = false;
+    // PFEM_ORDER_BLOCKS: sl_lower / sl_upper hold the blocked layout (one block per lane; the sliced slots
+    // take the dependencies outside the block, a dense triangle per block the ones inside); the per-level
+    // tables (lrows, lchunks, ...) hold the longest-path levels of the same factor
+    bool blocked = false;
 };
 
 struct pfem_solver_t {

@@ -157,10 +157,14 @@ int pfem_dirichlet_apply2(pfem_dirichlet bc0, pfem_dirichlet bc1, pfem_csr a, do
 #define PFEM_PRECOND_IC0 2
 #define PFEM_PRECOND_ILU0 3
 #define PFEM_PRECOND_CUSTOM 4   /* host callbacks (pfem_precond_create_custom) */
-/* ordering for IC0/ILU0: natural (exact reference factor, level-scheduled) or multicolour (the
- * factor of the symmetrically permuted matrix; few levels) */
+/* ordering for IC0/ILU0: natural (exact reference factor, level-scheduled), multicolour (the
+ * factor of the symmetrically permuted matrix; few levels) or blocks (blocks of up to 8 neighbouring
+ * nodes, coloured as blocks: the factor keeps the couplings inside each block, which takes fewer Krylov
+ * iterations than multicolour; each block is solved by one lane). No counterpart in the reference,
+ * whose factor is the natural-order one. */
 #define PFEM_ORDER_NATURAL 0
 #define PFEM_ORDER_MULTICOLOR 1
+#define PFEM_ORDER_BLOCKS 2
 /* PreconditionerJacobi(n, LA) / PreconditionerIC(matrix, LA) / PreconditionerILU(matrix, LA)
  * (preconditioners.cuh:9-90); the pattern of `a` fixes the structure */
 int pfem_precond_create(pfem_csr a, int kind, int ordering, pfem_precond* out);

@@ -135,14 +135,15 @@ def test_step_parity(ctx, oracle, ci, steps):
     assert its.size == 5 * case.outer_iterations and np.all(its > 0)
 
 
-@pytest.mark.parametrize("ordering", [0, 1])
-def test_step_mixed_precision(ctx, oracle, ordering):
+@pytest.mark.parametrize("precision,ordering", [(1, 0), (1, 1), (1, 2), (0, 2)])
+def test_step_mixed_precision(ctx, oracle, precision, ordering):
     """Mixed precision (FP32 inner Krylov + FP64 refinement for the velocity solves; the pressure IC0 factor
-    applied from an FP32 copy, or with the multicolour ordering from a 16-bit copy): fields within 1e-8 of
-    the FP64 oracle after 2 steps."""
+    applied from an FP32 copy, or with the multicolour and blocked orderings from a 16-bit copy), and the
+    FP64 step with the blocked pressure factor: fields within 1e-8 of the FP64 oracle after 2 steps."""
     from gpupfem2_amd import cases
     case = small_cases()[1]
-    mesh, ph, stepper = cases.build_gpu(ctx, case, precision=1, inner_tol=1e-5, pres_ordering=ordering)
+    print("step precision", precision, "ordering", ordering, flush=True)
+    mesh, ph, stepper = cases.build_gpu(ctx, case, precision=precision, inner_tol=1e-5, pres_ordering=ordering)
     oph, ost = oracle_stepper(oracle, case)
     for s in range(2):
         fg, fo = stepper.step(), ost.step()
