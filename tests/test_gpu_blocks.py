@@ -197,3 +197,62 @@ def test_block_order_3d(ctx, oracle):
         assert st.iterations == 30
         got.append(x.download())
     assert rel_l2(got[1], got[0]) < 1e-12
+
+
+def path_matrix(n, seed=0):
+    """SPD tridiagonal matrix of a path of n nodes (2 on the diagonal plus a random shift, -1 off it)."""
+    rng = np.random.default_rng(seed)
+    rp, col, val = [0], [], []
+    for i in range(n):
+        for j in (i - 1, i, i + 1):
+            if 0 <= j < n:
+                col.append(j)
+                val.append(2.0 + rng.uniform(0.0, 0.5) if j == i else -1.0)
+        rp.append(len(col))
+    return np.array(rp, np.int32), np.array(col, np.int32), np.array(val)
+
+
+@pytest.mark.parametrize("n", [1, 3, 8, 13, 300])
+@pytest.mark.parametrize("kind", [2, 3])
+def test_block_order_small(ctx, oracle, n, kind):
+    """Matrices smaller than a block, of one block, and of a few blocks of mixed sizes (a path graph: blocks
+    of 8, 4, 2 and 1 nodes, 2 block colours): the factor is the oracle's of P A P^T bit for bit, the apply
+    its sequential apply bit for bit, and the fused blocked PCG solves the system (residual 1e-10) in the
+    iteration count of the unfused PCG within 1."""
+    import scipy.sparse as sp
+    from gpupfem2_amd import capi
+    rp, col, val = path_matrix(n, seed=n)
+    a = capi.CSR(ctx, rowptr=rp, col=col, val=val)
+    pc = capi.Preconditioner(a, kind, capi.ORDER_BLOCKS)
+    pc.initialize(a)
+    fac, nl, perm = pc.download()
+    assert np.array_equal(np.sort(perm), np.arange(n))
+    B = sp.csr_matrix((val, col, rp), shape=(n, n))[perm][:, perm].tocsr()
+    B.sort_indices()
+    rpb, colb = B.indptr.astype(np.int32), B.indices.astype(np.int32)
+    ref = (oracle.ic0 if kind == 2 else oracle.ilu0)(rpb, colb, B.data.copy())[1]
+    if kind == 2:
+        low = colb <= np.repeat(np.arange(n), np.diff(rpb))
+        assert np.array_equal(fac[low], ref[low])
+    else:
+        assert np.array_equal(fac, ref)
+    r = np.random.default_rng(1).standard_normal(n)
+    zp = (oracle.ic0_apply if kind == 2 else oracle.ilu0_apply)(rpb, colb, ref, r[perm])
+    z = np.empty(n)
+    z[perm] = zp
+    dz = ctx.empty(n)
+    pc.apply(dz, ctx.to_device(r))
+    assert np.array_equal(dz.download(), z)
+    b = np.random.default_rng(2).standard_normal(n)
+    its = []
+    for fuse in (0, 3):
+        s = capi.Solver(a, capi.SOLVER_CG, 1e-12, 1000, 0, capi.TOL_RELATIVE,
+                        capi.Preconditioner(a, kind, capi.ORDER_BLOCKS))
+        s.set_option(capi.SOLVER_OPT_FUSE, fuse)
+        x = ctx.zeros(n)
+        st = s.solve(a, x, ctx.to_device(b))
+        assert st.converged
+        xs = x.download()
+        assert np.linalg.norm(b - sp.csr_matrix((val, col, rp), shape=(n, n)) @ xs) <= 1e-10 * np.linalg.norm(b)
+        its.append(st.iterations)
+    assert abs(its[0] - its[1]) <= 1, its
