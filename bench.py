@@ -479,9 +479,10 @@ def batched_workload(args):
 
 def history_workload(args):
     """The cylinder configuration's drag and lift histories (Cylinder2D/main.cu:401-428, 872-874): `steps`
-    time steps from rest in plain FP64 and in the mixed pipeline, C_D and C_L after every step. Prints one
-    JSON line with the per-mode histories (every step), the largest relative difference between the modes
-    and the time per step; --out also writes the histories as CSV."""
+    time steps from rest in plain FP64 and in the mixed pipeline, C_D and C_L after every step, the work
+    between the solves replayed from graphs as in the main workload (--no-graphs: kernel by kernel).
+    Prints one JSON line with the per-mode histories (every step), the largest relative difference
+    between the modes and the time per step; --out also writes the histories as CSV."""
     import numpy as np
     from gpupfem2_amd import capi, cases
     case = cases.cylinder_case(h=args.h)
@@ -490,6 +491,8 @@ def history_workload(args):
     for prec in (0, 1):
         mode = "mixed" if prec else "fp64"
         mesh, ph, st = cases.build_gpu(ctx, case, precision=prec, inner_tol=args.inner_tol)
+        if not args.no_graphs:
+            st.set_graphs(True)
         f = np.zeros((args.steps, 2))
         ctx.sync()
         t0 = time.perf_counter()
