@@ -68,3 +68,48 @@ def test_concurrent_solves_two_streams(orderings):
     del solvers, systems, together, threads
     for c in ctxs:
         c.close()
+
+
+def test_concurrent_steppers_with_graphs():
+    """Two PFEM-2 steppers (the small cylinder and the small Poiseuille channel) on two contexts, stepped
+    at the same time from two host threads with the step graphs on: one thread captures its graphs (a
+    thread-local capture) while the other launches, allocates and synchronises. Each gives, step by
+    step, the forces, fields, iteration counts and particle counts it gives alone."""
+    from gpupfem2_amd import capi, cases
+    print("concurrent steppers", flush=True)
+    case_list = [cases.cylinder_case(h=1.0 / 8, ppc=6, dt=2e-3), cases.poiseuille_case(nx=40, ny=10, ppc=9, dt=2e-3)]
+    ctxs = [capi.Context(0), capi.Context(0)]
+    steps = 5
+
+    def run(k, out):
+        mesh, ph, stepper = cases.build_gpu(ctxs[k], case_list[k], precision=k, inner_tol=1e-5)
+        stepper.set_graphs(True)
+        res = []
+        for s in range(steps):
+            f = stepper.step()
+            res.append((f.copy(), stepper.fields(), stepper.iterations().copy(), ph.count()))
+        out[k] = res
+        del stepper, ph, mesh
+
+    alone = {}
+    for k in range(2):
+        run(k, alone)
+    together = {}
+    threads = [threading.Thread(target=run, args=(k, together)) for k in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=200)
+    assert not any(t.is_alive() for t in threads)
+    for k in range(2):
+        assert k in together, k
+        for s, (a, b) in enumerate(zip(alone[k], together[k])):
+            assert np.array_equal(a[0], b[0]), (k, s)
+            for f in a[1]:
+                assert np.array_equal(a[1][f], b[1][f]), (k, s, f)
+            assert np.array_equal(a[2], b[2]) and a[3] == b[3], (k, s)
+    for c in ctxs:
+        c.sync()
+    del together, alone, threads
+    for c in ctxs:
+        c.close()
