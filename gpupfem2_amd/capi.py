@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpfem2_hip.so")
 
 PFEM_OK = 0
+PFEM_EINVAL, PFEM_EHIP, PFEM_EIO, PFEM_ENUMERIC, PFEM_ECAPACITY = -1, -2, -3, -4, -5
 DIRICHLET_PARITY, DIRICHLET_MASK = 0, 1
 PRECOND_NONE, PRECOND_JACOBI, PRECOND_IC0, PRECOND_ILU0 = 0, 1, 2, 3
 ORDER_NATURAL, ORDER_MULTICOLOR, ORDER_BLOCKS = 0, 1, 2

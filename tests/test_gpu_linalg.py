@@ -603,6 +603,10 @@ def test_solver_failure_paths(ctx, oracle):
     x.upload(np.full(xy.shape[0], 0.25))
     st = capi.Solver(a, capi.SOLVER_GMRES, 1e-30, 6, 0, 0, capi.Preconditioner(a)).solve(a, x, rhs)
     assert not st.converged and st.iterations == 6 and np.all(x.download() == 0.25)
+    # an ordering the library does not know is refused with PFEM_EINVAL
+    with pytest.raises(capi.PfemError) as e:
+        capi.Preconditioner(a, capi.PRECOND_IC0, 7)
+    assert e.value.code == capi.PFEM_EINVAL
 
 
 def test_half_factor(ctx, oracle):
@@ -665,12 +669,14 @@ def test_half_factor(ctx, oracle):
         capi.Preconditioner(a, capi.PRECOND_IC0, capi.ORDER_MULTICOLOR).set_factor_bits(8)
 
 
-def test_half_factor_wide_range_falls_back(ctx, oracle):
+@pytest.mark.parametrize("ordering", [1, 2])
+def test_half_factor_wide_range_falls_back(ctx, oracle, ordering):
     """A factor whose entries span more than the 16-bit copy's range (the system D A D with D from 1 to
     1e-10 along x: IC0(D A D) = D IC0(A), so the factor's diagonal runs from about 1 to 1e-10, past the
     2^28 that half precision holds between its largest entry and its smallest normal) would lose diagonals
     to zero or subnormal halves and divide by nothing; the preconditioner then applies its 32-bit copy
-    instead: the 16-bit request takes the 32-bit iterations exactly and reaches the direct solution."""
+    instead: the 16-bit request takes the 32-bit iterations exactly and reaches the direct solution
+    (multicolour and blocked orderings)."""
     import scipy.sparse as sp
     import scipy.sparse.linalg as spla
     from gpupfem2_amd import capi
@@ -685,7 +691,7 @@ def test_half_factor_wide_range_falls_back(ctx, oracle):
     exact = spla.spsolve(sp.csr_matrix((vs, col, rp)).tocsc(), b)
     its = {}
     for bits in (32, 16):
-        pc = capi.Preconditioner(ms, capi.PRECOND_IC0, capi.ORDER_MULTICOLOR)
+        pc = capi.Preconditioner(ms, capi.PRECOND_IC0, ordering)
         pc.set_factor_bits(bits)
         s = capi.Solver(ms, capi.SOLVER_CG, 1e-12, 5000, 0, capi.TOL_RELATIVE, pc)
         s.set_precision(capi.PRECISION_MIXED_PRECOND, 1e-5, 20)
