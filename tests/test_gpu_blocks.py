@@ -256,3 +256,47 @@ def test_block_order_small(ctx, oracle, n, kind):
         assert np.linalg.norm(b - sp.csr_matrix((val, col, rp), shape=(n, n)) @ xs) <= 1e-10 * np.linalg.norm(b)
         its.append(st.iterations)
     assert abs(its[0] - its[1]) <= 1, its
+
+
+def test_block_gmres_and_fp32_refinement(ctx):
+    """The other solver paths over a blocked factor (the generic single-pass sweeps, in the factor's
+    ordering): GMRES(30) with ILU0 on a nonsymmetric system (the pressure matrix with its diagonal
+    shifted by 1 and its off-diagonal entries perturbed by up to 10%), and the FP32 inner PCG with FP64
+    refinement with IC0 (PFEM_PRECISION_MIXED) on the symmetric matrix with the shifted diagonal, each
+    against the same solve over the multicolour factor and a direct solve."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    from gpupfem2_amd import capi
+    print("block gmres and fp32 refinement", flush=True)
+    m, a, rhs, n = cylinder_pressure(ctx, 1.0 / 28)
+    rp, col, val = a.download()
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    rng = np.random.default_rng(9)
+    v = val + np.where(col == rows, 1.0, 0.1 * rng.uniform(-1, 1, col.size) * np.abs(val))
+    an = capi.CSR(ctx, like=a)
+    an.upload_values(v)
+    b = rng.standard_normal(n)
+    exact = spla.spsolve(sp.csr_matrix((v, col, rp), shape=(n, n)).tocsc(), b)
+    sols = []
+    for ordering in (capi.ORDER_MULTICOLOR, capi.ORDER_BLOCKS):
+        s = capi.Solver(an, capi.SOLVER_GMRES, 1e-12, 20000, 30, capi.TOL_RELATIVE,
+                        capi.Preconditioner(an, capi.PRECOND_ILU0, ordering))
+        x = ctx.zeros(n)
+        st = s.solve(an, x, ctx.to_device(b))
+        assert st.converged, ordering
+        sols.append(x.download())
+        assert rel_l2(sols[-1], exact) < 1e-8, ordering
+    # FP32 inner iterations need a condition number FP32 can carry: the symmetric matrix with the shifted
+    # diagonal (the pressure matrix itself stalls them, with either ordering)
+    vs = val + np.where(col == rows, 1.0, 0.0)
+    asym = capi.CSR(ctx, like=a)
+    asym.upload_values(vs)
+    exact_s = spla.spsolve(sp.csr_matrix((vs, col, rp), shape=(n, n)).tocsc(), b)
+    for ordering in (capi.ORDER_MULTICOLOR, capi.ORDER_BLOCKS):
+        s = capi.Solver(asym, capi.SOLVER_CG, 1e-12, 20000, 0, capi.TOL_RELATIVE,
+                        capi.Preconditioner(asym, capi.PRECOND_IC0, ordering))
+        s.set_precision(capi.PRECISION_MIXED, 1e-5, 30)
+        x = ctx.zeros(n)
+        st = s.solve(asym, x, ctx.to_device(b))
+        assert st.converged and st.refinements >= 1, ordering
+        assert rel_l2(x.download(), exact_s) < 1e-9, ordering
