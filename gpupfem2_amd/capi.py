@@ -97,6 +97,10 @@ SIGNATURES = {
     "pfem_spmv": (_i, [_vp, _vp, _vp]),
     "pfem_csr_refresh_fp32": (_i, [_vp]),
     "pfem_spmv_fp32": (_i, [_vp, _vp, _vp]),
+    "pfem_dot": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "pfem_nrm2": (_i, [_vp, _i, _vp, _vp]),
+    "pfem_trsv_upper": (_i, [_vp, _i, _vp, _i, _vp]),
+    "pfem_gemv": (_i, [_vp, _i, _i, _vp, _vp, _i, _vp, _vp, _vp]),
     "pfem_dirichlet_create": (_i, [_vp, _i, _i, _vp, _vp, _pp]),
     "pfem_dirichlet_destroy": (_i, [_vp]),
     "pfem_dirichlet_set_values": (_i, [_vp, _vp]),
@@ -527,6 +531,41 @@ class Solver(_Handle):
         st = (SolveStats * nrhs)()
         self.ctx.check(lib().pfem_solver_solve_multi(self.h, a.h, int(nrhs), X.ptr, B.ptr, st))
         return list(st)
+
+
+class LinearAlgebra:
+    """Dense BLAS-1/2 on the context's stream (LinearAlgebra, src/linear_algebra/linear_algebra.h:27-54).
+    Operands are DeviceArrays or raw device addresses (ints); results stay in device memory, as with the
+    reference's device pointer mode."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+
+    @staticmethod
+    def _p(a):
+        return a if isinstance(a, int) else a.ptr
+
+    def dot(self, x, y, res, n=None):
+        """res[0] = x . y (linear_algebra.h:27-38); n defaults to the arrays' common length."""
+        if n is None:
+            if x.n != y.n:
+                raise PfemError(PFEM_EINVAL, "Inconsistent dimensions of vectors in dot product")
+            n = x.n
+        self.ctx.check(lib().pfem_dot(self.ctx.h, int(n), self._p(x), self._p(y), self._p(res)))
+
+    def norm_squared(self, x, res, n=None):
+        """res[0] = ||x||_2, the Euclidean norm itself (normSquared, linear_algebra.h:40-43, is cublasDnrm2)."""
+        n = x.n if n is None else n
+        self.ctx.check(lib().pfem_nrm2(self.ctx.h, int(n), self._p(x), self._p(res)))
+
+    def solve_upper_triangular(self, a, x, n, lda):
+        """x <- U^-1 x, U upper triangular in column-major a (solveGeneralTriangularSystem, :45-49)."""
+        self.ctx.check(lib().pfem_trsv_upper(self.ctx.h, int(n), self._p(a), int(lda), self._p(x)))
+
+    def general_mv(self, a, x, y, alpha, beta, m, n, lda):
+        """y = alpha A x + beta y, A m x n column-major; alpha, beta device scalars (generalMV, :51-54)."""
+        self.ctx.check(lib().pfem_gemv(self.ctx.h, int(m), int(n), self._p(alpha), self._p(a), int(lda),
+                                       self._p(x), self._p(beta), self._p(y)))
 
 
 def quadrature(kind, index):

@@ -1,4 +1,5 @@
 // C ABI: CSR matrices, Dirichlet boundary conditions, VTK export, SimulationParameters defaults.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -147,6 +148,44 @@ int pfem_spmv(pfem_csr a, const double* d_x, double* d_y)
     PFEM_TRY(a ? a->ctx : nullptr, {
         PFEM_REQUIRE(a && d_x && d_y, "null argument");
         launch_spmv(C(a->ctx), *a->pat, a->val.p, d_x, d_y);
+        PFEM_HIP(hipGetLastError());
+    });
+}
+
+int pfem_dot(pfem_ctx ctx, int n, const double* d_x, const double* d_y, double* d_res)
+{
+    PFEM_TRY(ctx, {
+        PFEM_REQUIRE(ctx && d_res && n >= 0 && (n == 0 || (d_x && d_y)), "invalid argument");
+        launch_dot(C(ctx), n, d_x, d_y, d_res);
+        PFEM_HIP(hipGetLastError());
+    });
+}
+
+int pfem_nrm2(pfem_ctx ctx, int n, const double* d_x, double* d_res)
+{
+    PFEM_TRY(ctx, {
+        PFEM_REQUIRE(ctx && d_res && n >= 0 && (n == 0 || d_x), "invalid argument");
+        launch_nrm2(C(ctx), n, d_x, d_res);
+        PFEM_HIP(hipGetLastError());
+    });
+}
+
+int pfem_trsv_upper(pfem_ctx ctx, int n, const double* d_a, int lda, double* d_x)
+{
+    PFEM_TRY(ctx, {
+        PFEM_REQUIRE(ctx && n >= 0 && lda >= std::max(1, n) && (n == 0 || (d_a && d_x)), "invalid argument");
+        launch_trsv_upper(C(ctx), n, d_a, lda, d_x);
+        PFEM_HIP(hipGetLastError());
+    });
+}
+
+int pfem_gemv(pfem_ctx ctx, int m, int n, const double* d_alpha, const double* d_a, int lda, const double* d_x,
+              const double* d_beta, double* d_y)
+{
+    PFEM_TRY(ctx, {
+        PFEM_REQUIRE(ctx && m >= 0 && n >= 0 && lda >= std::max(1, m), "invalid argument");
+        PFEM_REQUIRE(m == 0 || n == 0 || (d_alpha && d_a && d_x && d_beta && d_y), "null argument");
+        launch_gemv(C(ctx), m, n, d_alpha, d_a, lda, d_x, d_beta, d_y);
         PFEM_HIP(hipGetLastError());
     });
 }

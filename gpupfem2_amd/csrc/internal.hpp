@@ -94,7 +94,12 @@ struct pfem_context {
     int* pinned_i = nullptr;      // small pinned host buffers for scalar read-backs
     double* pinned_d = nullptr;
     double* scratch = nullptr;    // device address of this device's copy of the allocation-free scratch array
+    double* la_part = nullptr;    // 2 * kLaMaxBlocks partials of the dot / nrm2 reductions
 };
+
+namespace pfem {
+constexpr int kLaMaxBlocks = 1024;   // largest grid of the dot / nrm2 partial kernels
+}
 
 namespace pfem {
 struct Context : pfem_context {};
@@ -181,6 +186,12 @@ void launch_spmv(Context* c, const Pattern& P, const double* val, const double* 
 void launch_spmv32(Context* c, const Pattern& P, const float* val, const float* x, float* y);
 void launch_fill(Context* c, double* d, double v, long n);
 void launch_copy(Context* c, double* dst, const double* src, long n);
+// dense BLAS-1/2 behind the LinearAlgebra entry points; scalars (res, alpha, beta) are in device memory
+void launch_dot(Context* c, long n, const double* x, const double* y, double* res);
+void launch_nrm2(Context* c, long n, const double* x, double* res);
+void launch_trsv_upper(Context* c, int n, const double* a, long lda, double* x);
+void launch_gemv(Context* c, int m, int n, const double* alpha, const double* a, long lda, const double* x,
+                 const double* beta, double* y);
 void exclusive_scan_int(Context* c, const int* in, int* out, long n, int* total_dev);
 void dirichlet_rows(Context* c, pfem_dirichlet_t* bc, const Pattern& P);
 }  // namespace pfem

@@ -65,6 +65,155 @@ __global__ void k_to_f32(float* __restrict__ d, const double* __restrict__ s, lo
     if (i < n) d[i] = (float)s[i];
 }
 
+// ------------------------------------------------------------------ dense BLAS-1/2 (LinearAlgebra wrappers)
+// LinearAlgebra::dot / normSquared / solveGeneralTriangularSystem / generalMV
+// (src/linear_algebra/linear_algebra.h:27-54: cublasDdot, cublasDnrm2, cublasDtrsv, cublasDgemv with the
+// scalars in device memory). Reductions are two launches: per-workgroup partials into the context's
+// partials buffer, then one workgroup sums them in a fixed order. The grid depends on n only, so a result's
+// bits depend only on the inputs.
+static int la_grid(long n)
+{
+    return (int)std::min<long>(kLaMaxBlocks, std::max<long>(1, (n + 4L * kBlock - 1) / (4L * kBlock)));
+}
+
+__global__ __launch_bounds__(kBlock) void k_dot_part(long n, const double* __restrict__ x,
+                                                     const double* __restrict__ y, double* __restrict__ part)
+{
+    double v[1] = {0.0};
+    const long stride = (long)gridDim.x * kBlock;
+#pragma unroll 4
+    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) v[0] = fma(x[i], y[i], v[0]);
+    block_partials<1>(v, part);
+}
+
+__global__ __launch_bounds__(kBlock) void k_dot_total(int nb, const double* __restrict__ part, double* __restrict__ res)
+{
+    double t[1];
+    sum_partials<1>(part, nb, t);
+    if (threadIdx.x == 0) *res = t[0];
+}
+
+// Sum of squares and largest magnitude per workgroup: part[b] and part[nb + b].
+__global__ __launch_bounds__(kBlock) void k_nrm2_part(long n, const double* __restrict__ x, double* __restrict__ part)
+{
+    __shared__ double wmax[kWaves];
+    double v[1] = {0.0}, m = 0.0;
+    const long stride = (long)gridDim.x * kBlock;
+#pragma unroll 4
+    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        const double a = fabs(x[i]);
+        v[0] = fma(a, a, v[0]);
+        m = fmax(m, a);
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    block_partials<1>(v, part);   // its barriers order the wmax stores before the read below
+    if (threadIdx.x == 0) {
+        double mm = wmax[0];
+        for (int w = 1; w < kWaves; ++w) mm = fmax(mm, wmax[w]);
+        part[gridDim.x + blockIdx.x] = mm;
+    }
+}
+
+// The plain sum of squares is kept when it is finite and the largest square is well inside the normal range
+// (entries whose squares underflow then weigh less than 2^-100 of it). Otherwise (overflow, or every entry
+// tiny) the vector is summed again scaled by its largest magnitude M, as M sqrt(sum (x_i/M)^2); that second
+// pass runs in this one workgroup, so it is slow, and only extreme inputs take it.
+__global__ __launch_bounds__(kBlock) void k_nrm2_total(long n, const double* __restrict__ x, int nb,
+                                                       const double* __restrict__ part, double* __restrict__ res)
+{
+    __shared__ double shm;
+    double t[1];
+    sum_partials<1>(part, nb, t);
+    if (threadIdx.x == 0) {
+        double mm = 0.0;
+        for (int b = 0; b < nb; ++b) mm = fmax(mm, part[nb + b]);
+        shm = mm;
+    }
+    __syncthreads();
+    const double m = shm, ssq = t[0];
+    if (isnan(ssq) || isinf(m) || m == 0.0 || (isfinite(ssq) && m >= 0x1p-460)) {
+        if (threadIdx.x == 0) *res = isnan(ssq) ? ssq : isinf(m) ? m : sqrt(ssq);
+        return;
+    }
+    double v[1] = {0.0};
+    for (long i = threadIdx.x; i < n; i += kBlock) {
+        const double a = x[i] / m;
+        v[0] = fma(a, a, v[0]);
+    }
+    block_sum<1>(v);
+    if (threadIdx.x == 0) *res = m * sqrt(v[0]);
+}
+
+// x <- U^-1 x for U upper triangular with a non-unit diagonal, column-major with leading dimension lda, by
+// columns from the last: x_j /= u_jj, then x_i -= u_ij x_j for i < j. Row i belongs to thread i mod 256,
+// which alone reads and writes x_i, so every x_i takes its updates in column order; each pivot is passed
+// through LDS. One workgroup: the systems are the Krylov solvers' small Hessenberg triangles. A zero
+// diagonal entry gives infinities or NaNs, as in BLAS.
+__global__ __launch_bounds__(kBlock) void k_trsv_upper(int n, const double* __restrict__ a, long lda, double* x)
+{
+    __shared__ double piv;
+    for (int j = n - 1; j >= 0; --j) {
+        if ((int)threadIdx.x == (j & (kBlock - 1))) {
+            const double v = x[j] / a[j + (long)j * lda];
+            x[j] = v;
+            piv = v;
+        }
+        __syncthreads();
+        const double p = piv;
+        for (int i = threadIdx.x; i < j; i += kBlock) x[i] = fma(-a[i + (long)j * lda], p, x[i]);
+        __syncthreads();
+    }
+}
+
+// y = alpha A x + beta y, A m x n column-major with leading dimension lda, alpha and beta read from device
+// memory. One row per thread, columns in order (a column's loads are contiguous over the workgroup). As in
+// BLAS: beta == 0 does not read y, alpha == 0 does not read A or x, and alpha == 0 with beta == 1 leaves y
+// untouched.
+__global__ __launch_bounds__(kBlock) void k_gemv(int m, int n, const double* __restrict__ alpha,
+                                                 const double* __restrict__ a, long lda, const double* __restrict__ x,
+                                                 const double* __restrict__ beta, double* __restrict__ y)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= m) return;
+    const double al = *alpha, be = *beta;
+    if (al == 0.0 && be == 1.0) return;
+    double s = 0.0;
+    if (al != 0.0) {
+        const double* ai = a + i;
+#pragma unroll 8
+        for (int j = 0; j < n; ++j) s = fma(ai[(long)j * lda], x[j], s);
+    }
+    const double r = al * s;
+    y[i] = be == 0.0 ? r : fma(be, y[i], r);
+}
+
+void launch_dot(Context* c, long n, const double* x, const double* y, double* res)
+{
+    const int nb = la_grid(n);
+    hipLaunchKernelGGL(k_dot_part, dim3(nb), dim3(kBlock), 0, c->stream, n, x, y, c->la_part);
+    hipLaunchKernelGGL(k_dot_total, dim3(1), dim3(kBlock), 0, c->stream, nb, (const double*)c->la_part, res);
+}
+
+void launch_nrm2(Context* c, long n, const double* x, double* res)
+{
+    const int nb = la_grid(n);
+    hipLaunchKernelGGL(k_nrm2_part, dim3(nb), dim3(kBlock), 0, c->stream, n, x, c->la_part);
+    hipLaunchKernelGGL(k_nrm2_total, dim3(1), dim3(kBlock), 0, c->stream, n, x, nb, (const double*)c->la_part, res);
+}
+
+void launch_trsv_upper(Context* c, int n, const double* a, long lda, double* x)
+{
+    if (n > 0) hipLaunchKernelGGL(k_trsv_upper, dim3(1), dim3(kBlock), 0, c->stream, n, a, lda, x);
+}
+
+void launch_gemv(Context* c, int m, int n, const double* alpha, const double* a, long lda, const double* x,
+                 const double* beta, double* y)
+{
+    if (m > 0 && n > 0)
+        hipLaunchKernelGGL(k_gemv, dim3(blocks_for(m)), dim3(kBlock), 0, c->stream, m, n, alpha, a, lda, x, beta, y);
+}
+
 // ------------------------------------------------------------------ Jacobi
 // extractDiagonal (src/linear_algebra/preconditioners.cu:3-14): invD = 1 / a_rr (diagonal positions are
 // precomputed instead of searched; same entry).

@@ -440,7 +440,12 @@ int pfem_ctx_create(int device, pfem_ctx* out)
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return PFEM_EHIP; }
     if (hipHostMalloc((void**)&c->pinned_i, 64 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&c->pinned_d, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&c->la_part, 2 * kLaMaxBlocks * sizeof(double)) != hipSuccess ||
         noalloc_init(c) != hipSuccess) {
+        (void)hipFree(c->la_part);
+        (void)hipHostFree(c->pinned_i);
+        (void)hipHostFree(c->pinned_d);
+        (void)hipStreamDestroy(c->stream);
         delete c;
         return PFEM_EHIP;
     }
@@ -455,6 +460,7 @@ int pfem_ctx_destroy(pfem_ctx ctx)
     (void)hipStreamDestroy(ctx->stream);
     (void)hipHostFree(ctx->pinned_i);
     (void)hipHostFree(ctx->pinned_d);
+    (void)hipFree(ctx->la_part);
     delete C(ctx);
     return PFEM_OK;
 }

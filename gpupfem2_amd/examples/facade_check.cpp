@@ -81,6 +81,48 @@ int main(int argc, char* argv[])
         report(!bc.empty(), "dirichlet_nodes", (double)bc.size());
     }
     LinearAlgebra LA;
+    {
+        // the dense wrappers, scalars in device memory, against host arithmetic
+        const auto hb = rhs.copyToHost();
+        double dd = 0.0, scale = 0.0;
+        for (double v : hb) {
+            dd += v * v;
+            scale += std::fabs(v * v);
+        }
+        deviceVector<double> res(std::vector<double>{0.0, 0.0});
+        LA.dot(rhs, rhs, res.data);
+        LA.normSquared(rhs.data, res.data + 1, n);
+        const auto hr = res.copyToHost();
+        report(std::fabs(hr[0] - dd) <= 1e-13 * scale && std::fabs(hr[1] - std::sqrt(dd)) <= 1e-13 * std::sqrt(dd),
+               "la_dot_nrm2", hr[0], hr[1]);
+        // U x = b for a 5 x 5 upper triangle stored with leading dimension 7, then y = 2 V z + 1 y
+        const int k = 5, lda = 7, m = 1000;
+        std::vector<double> u(lda * k, 1e300), b(k), vm((size_t)m * k), z(k), y(m);
+        for (int j = 0; j < k; ++j) {
+            for (int i = 0; i <= j; ++i) u[i + j * lda] = i == j ? 2.0 + j : 0.25 * (i + 1) - 0.1 * j;
+            b[j] = 1.0 + 0.5 * j;
+            z[j] = 0.3 * j - 0.4;
+            for (int i = 0; i < m; ++i) vm[i + (size_t)j * m] = std::sin(0.01 * i + j);
+        }
+        for (int i = 0; i < m; ++i) y[i] = std::cos(0.02 * i);
+        std::vector<double> hx = b;
+        for (int j = k - 1; j >= 0; --j) {
+            hx[j] /= u[j + j * lda];
+            for (int i = 0; i < j; ++i) hx[i] -= u[i + j * lda] * hx[j];
+        }
+        deviceVector<double> du(u), dx(b), dv(vm), dz(z), dy(y), ab(std::vector<double>{2.0, 1.0});
+        LA.solveGeneralTriangularSystem(du.data, dx.data, k, lda);
+        LA.generalMV(dv.data, dz.data, dy.data, ab.data, ab.data + 1, m, k, m);
+        const auto gx = dx.copyToHost(), gy = dy.copyToHost();
+        double ex = 0.0, ey = 0.0;
+        for (int j = 0; j < k; ++j) ex = std::fmax(ex, std::fabs(gx[j] - hx[j]));
+        for (int i = 0; i < m; ++i) {
+            double s = 0.0;
+            for (int j = 0; j < k; ++j) s += vm[i + (size_t)j * m] * z[j];
+            ey = std::fmax(ey, std::fabs(gy[i] - (2.0 * s + y[i])));
+        }
+        report(ex < 1e-14 && ey < 1e-13, "la_trsv_gemv", ex, ey);
+    }
 
     deviceVector<double> x0, x1;
     x0.allocate(n);
@@ -102,6 +144,19 @@ int main(int argc, char* argv[])
            plug.lastStats().iterations, diff);
     report(ident.inits == 1 && ident.applies >= plug.lastStats().iterations, "custom_preconditioner_calls", ident.inits,
            ident.applies);
+    {
+        // A x0 reproduces the right-hand side to the solve's tolerance
+        deviceVector<double> ax;
+        ax.allocate(n);
+        LA.sparseMV(a, x0.data, ax.data);
+        const auto hax = ax.copyToHost(), hb = rhs.copyToHost();
+        double e = 0.0, nb = 0.0;
+        for (int i = 0; i < n; ++i) {
+            e += (hax[i] - hb[i]) * (hax[i] - hb[i]);
+            nb += hb[i] * hb[i];
+        }
+        report(std::sqrt(e) <= 1e-8 + 1e-9 * std::sqrt(nb), "la_sparse_mv", std::sqrt(e), std::sqrt(nb));
+    }
 
     for (int ordering : {PFEM_ORDER_NATURAL, PFEM_ORDER_MULTICOLOR}) {
         deviceVector<double> x;

@@ -5,7 +5,7 @@
  *   Mesh2D                  src/mesh_2d.cuh:11-69
  *   SparseMatrixCSR         src/linear_algebra/sparse_matrix.cuh:9-58
  *   DirichletNode/BCs       src/Dirichlet_bcs.cuh:8-29
- *   LinearAlgebra           src/linear_algebra/linear_algebra.h (handle holder; empty here)
+ *   LinearAlgebra           src/linear_algebra/linear_algebra.h:10-80 (dense wrappers, sparseMV)
  *   Preconditioner*         src/linear_algebra/preconditioners.cuh:9-90 (virtual plugin point kept)
  *   LinearSolver/SolverCG/SolverGMRES   src/linear_algebra/linear_solver.cuh:15-130
  *   ParticleHandler2D       src/particles/particle_handler_2d.cuh:9-56
@@ -494,8 +494,42 @@ protected:
 };
 
 /* ------------------------------------------------------------------ linear algebra */
-/* the reference keeps cuBLAS / cuSPARSE handles here; the library's context holds its stream */
+/* LinearAlgebra (linear_algebra.h:10-80). The reference keeps cuBLAS / cuSPARSE handles here; the library's
+ * context holds its stream. The dense wrappers keep the reference's device pointer mode: res, alpha and beta
+ * point to device memory. The reference's sparse wrappers take cuSPARSE descriptors; here sparseMV takes the
+ * matrix itself, and the incomplete factorisations and triangular solves are the Preconditioner classes. */
 class LinearAlgebra {
+public:
+    void dot(const deviceVector<double>& v1, const deviceVector<double>& v2, double* res) const
+    {
+        if (v1.size == v2.size)
+            dot(v1.data, v2.data, res, v1.size);
+        else
+            std::printf("Inconsistent dimensions of vectors in dot product\n");
+    }
+    void dot(const double* v1, const double* v2, double* res, int n) const
+    {
+        gpupfem2::check(pfem_dot(gpupfem2::context(), n, v1, v2, res));
+    }
+    /* the Euclidean norm itself, as the reference's cublasDnrm2 */
+    void normSquared(const double* v, double* res, int n) const
+    {
+        gpupfem2::check(pfem_nrm2(gpupfem2::context(), n, v, res));
+    }
+    void solveGeneralTriangularSystem(const double* matrix, double* solution, int n, int lda) const
+    {
+        gpupfem2::check(pfem_trsv_upper(gpupfem2::context(), n, matrix, lda, solution));
+    }
+    void generalMV(const double* matrix, const double* vec, double* res, const double* alpha, const double* beta,
+                   int m, int n, int lda) const
+    {
+        gpupfem2::check(pfem_gemv(gpupfem2::context(), m, n, alpha, matrix, lda, vec, beta, res));
+    }
+    /* result = A vec (sparseMV, linear_algebra.h:75-80, as the solvers call it: alpha 1, beta 0) */
+    void sparseMV(const SparseMatrixCSR& matrix, const double* vec, double* result) const
+    {
+        gpupfem2::check(pfem_spmv(matrix.handle(), vec, result));
+    }
 };
 
 /* Preconditioner (preconditioners.cuh:9-26). A subclass that overrides initialize / applyPreconditioner

@@ -136,6 +136,24 @@ int pfem_spmv(pfem_csr a, const double* d_x, double* d_y);
 int pfem_csr_refresh_fp32(pfem_csr a);
 int pfem_spmv_fp32(pfem_csr a, const float* d_x, float* d_y);
 
+/* Dense BLAS-1/2 of LinearAlgebra (src/linear_algebra/linear_algebra.h:27-54), FP64, on the context's stream;
+ * vectors, matrices and the scalars res, alpha, beta are device pointers (the reference's cuBLAS pointer
+ * mode is DEVICE). Reductions sum in a fixed order, so results are deterministic. */
+/* *d_res = x . y: LinearAlgebra::dot (linear_algebra.h:35-38, cublasDdot); n = 0 gives 0 */
+int pfem_dot(pfem_ctx ctx, int n, const double* d_x, const double* d_y, double* d_res);
+/* *d_res = ||x||_2 (not squared, whatever the name): LinearAlgebra::normSquared (linear_algebra.h:40-43,
+ * cublasDnrm2). Vectors whose squares overflow or underflow are summed again scaled by max |x_i|. */
+int pfem_nrm2(pfem_ctx ctx, int n, const double* d_x, double* d_res);
+/* x <- U^-1 x, U the upper triangle (non-unit diagonal) of the column-major n x n matrix at d_a with leading
+ * dimension lda >= n: LinearAlgebra::solveGeneralTriangularSystem (linear_algebra.h:45-49, cublasDtrsv).
+ * One workgroup, for small systems (the GMRES Hessenberg triangle). */
+int pfem_trsv_upper(pfem_ctx ctx, int n, const double* d_a, int lda, double* d_x);
+/* y = alpha A x + beta y, A m x n column-major with lda >= m: LinearAlgebra::generalMV
+ * (linear_algebra.h:51-54, cublasDgemv, no transpose). As BLAS: m = 0 or n = 0 leaves y unchanged, beta = 0
+ * does not read y. */
+int pfem_gemv(pfem_ctx ctx, int m, int n, const double* d_alpha, const double* d_a, int lda, const double* d_x,
+              const double* d_beta, double* d_y);
+
 /* ------------------------------------------------------------------ Dirichlet BCs (src/Dirichlet_bcs.cuh) */
 #define PFEM_DIRICHLET_PARITY 0  /* reference heuristic: skip row c when |first off-diagonal| < 2e-6 */
 #define PFEM_DIRICHLET_MASK 1    /* skip exactly the Dirichlet rows (symmetric result) */
