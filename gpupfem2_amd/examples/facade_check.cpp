@@ -1,8 +1,10 @@
 // Exercises the C++ facade's plugin point and classes on one small problem and prints one line per check
 // ("ok <name> <numbers>" or "FAIL <name> ..."): a user Preconditioner subclass (identity, applied with a
 // device copy on the library's stream) driving SolverCG must reproduce the unpreconditioned solve
-// exactly; PreconditionerIC (natural and multicolour) and GMRES(restart) must converge; particles seed,
-// advect and project; SimulationParameters defaults and quadrature tables are the reference's.
+// exactly, also when it applies itself through the LinearAlgebra it was given; LinearAlgebra's dense
+// wrappers (dot, norm, triangular solve, gemv, device scalars) and sparseMV must match host arithmetic;
+// PreconditionerIC (natural and multicolour) and GMRES(restart) must converge; particles seed, advect and
+// project; SimulationParameters defaults and quadrature tables are the reference's.
 //
 //   facade_check mesh.dat
 #include <algorithm>
@@ -36,6 +38,23 @@ public:
         ++applies;
         if (dest != src) gpupfem2::check(pfem_memcpy_d2d(gpupfem2::context(), dest, src, sizeof(double) * n));
     }
+};
+
+// the identity again, but through the LinearAlgebra it was given, as a reference plugin would use it:
+// dest = 1 * [src] (an n x 1 matrix) * [1] + 0 * dest, scalars in device memory
+class GemvIdentityPreconditioner : public Preconditioner {
+public:
+    GemvIdentityPreconditioner(int n_, const LinearAlgebra* LA_) : Preconditioner(n_, LA_) {}
+    void initialize(const SparseMatrixCSR& csrMatrix) override
+    {
+        (void)csrMatrix;
+        if (!scalars.data) scalars = deviceVector<double>(std::vector<double>{1.0, 0.0});
+    }
+    void applyPreconditioner(double* dest, const double* src) override
+    {
+        LA->generalMV(src, scalars.data, dest, scalars.data, scalars.data + 1, n, 1, n);
+    }
+    deviceVector<double> scalars;
 };
 
 class NoPreconditioner : public BuiltinPreconditioner {
@@ -144,6 +163,20 @@ int main(int argc, char* argv[])
            plug.lastStats().iterations, diff);
     report(ident.inits == 1 && ident.applies >= plug.lastStats().iterations, "custom_preconditioner_calls", ident.inits,
            ident.applies);
+    {
+        deviceVector<double> x2;
+        x2.allocate(n);
+        x2.clearValues();
+        GemvIdentityPreconditioner gid(n, &LA);
+        SolverCG cg(1e-10, 5000, &LA, &gid);
+        cg.init(a);
+        const bool ok2 = cg.solve(a, x2, rhs);
+        const auto h2 = x2.copyToHost();
+        double d2 = 0.0;
+        for (int i = 0; i < n; ++i) d2 = std::fmax(d2, std::fabs(h0[i] - h2[i]));
+        report(ok2 && cg.lastStats().iterations == ref.lastStats().iterations && d2 == 0.0, "plugin_using_linear_algebra",
+               cg.lastStats().iterations, d2);
+    }
     {
         // A x0 reproduces the right-hand side to the solve's tolerance
         deviceVector<double> ax;
