@@ -38,7 +38,11 @@ typedef struct pfem_particles_t* pfem_particles;
 typedef struct pfem_pfem2_t* pfem_pfem2;
 
 /* ------------------------------------------------------------------ context and device memory */
-/* replaces the implicit CUDA context + legacy default stream (src/common/cuda_memory.cuh:17-148) */
+/* replaces the implicit CUDA context + legacy default stream (src/common/cuda_memory.cuh:17-148).
+ * A context, and every handle made on it, is used from one host thread at a time: its calls are ordered
+ * on its one stream and share its scratch buffers (e.g. the reduction partials of pfem_dot / pfem_nrm2).
+ * Work from several threads at once takes one context per thread (contexts on one device run
+ * concurrently). */
 int pfem_ctx_create(int device, pfem_ctx* out);
 int pfem_ctx_destroy(pfem_ctx ctx);
 const char* pfem_last_error(pfem_ctx ctx);
