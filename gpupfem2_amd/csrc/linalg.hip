@@ -122,16 +122,17 @@ __global__ __launch_bounds__(kBlock) void k_nrm2_part(long n, const double* __re
 __global__ __launch_bounds__(kBlock) void k_nrm2_total(long n, const double* __restrict__ x, int nb,
                                                        const double* __restrict__ part, double* __restrict__ res)
 {
-    __shared__ double shm;
+    __shared__ double wmax[kWaves];
     double t[1];
     sum_partials<1>(part, nb, t);
-    if (threadIdx.x == 0) {
-        double mm = 0.0;
-        for (int b = 0; b < nb; ++b) mm = fmax(mm, part[nb + b]);
-        shm = mm;
-    }
+    double mm = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kBlock) mm = fmax(mm, part[nb + b]);
+    mm = wave_max(mm);
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mm;
     __syncthreads();
-    const double m = shm, ssq = t[0];
+    double m = wmax[0];
+    for (int w = 1; w < kWaves; ++w) m = fmax(m, wmax[w]);
+    const double ssq = t[0];
     if (isnan(ssq) || isinf(m) || m == 0.0 || (isfinite(ssq) && m >= 0x1p-460)) {
         if (threadIdx.x == 0) *res = isnan(ssq) ? ssq : isinf(m) ? m : sqrt(ssq);
         return;
